@@ -1,0 +1,910 @@
+// Host engine behind the C-ABI (include/diffcp.h): device memory, internal
+// particle order, patches, launches, checkpointed rollouts and hipGraphs.
+#include "kernels.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <numeric>
+#include <string>
+#include <tuple>
+#include <vector>
+
+static thread_local std::string g_err;
+static int set_err(const std::string& m) { g_err = m; return -1; }
+#define HIP_OK(x)                                                                             \
+    do {                                                                                      \
+        hipError_t e_ = (x);                                                                  \
+        if (e_ != hipSuccess) return set_err(std::string(#x) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+#define LAUNCH_OK()                                                                                \
+    do {                                                                                           \
+        hipError_t e_ = hipGetLastError();                                                         \
+        if (e_ != hipSuccess) return set_err(std::string("kernel launch: ") + hipGetErrorString(e_)); \
+    } while (0)
+
+namespace {
+
+struct HostInst {
+    int G, nv, ne, nc, no, ngp, nact;
+    std::vector<int> vperm;   // internal vertex -> natural vertex
+    std::vector<int> vinv;    // natural -> internal
+    std::vector<int> eperm;   // internal element -> natural element
+    std::vector<int> einv;
+    std::vector<float> x0v, x0o, x0c;   // creation positions (natural order)
+    std::vector<int> tri;     // natural triangles
+};
+
+uint32_t spread10(uint32_t v) {
+    v &= 1023u;
+    v = (v | (v << 16)) & 0x030000FFu;
+    v = (v | (v << 8)) & 0x0300F00Fu;
+    v = (v | (v << 4)) & 0x030C30C3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+uint32_t morton(float x, float y, float z) {
+    auto q = [](float a) { int t = (int)std::floor(a * 1024.0f); return (uint32_t)std::min(1023, std::max(0, t)); };
+    return spread10(q(x)) | (spread10(q(y)) << 1) | (spread10(q(z)) << 2);
+}
+
+// host FP32 cofactor inverse (same formula as the device minv)
+void inv3h(const float* a, float* r) {
+    float c00 = a[4] * a[8] - a[5] * a[7];
+    float c01 = a[5] * a[6] - a[3] * a[8];
+    float c02 = a[3] * a[7] - a[4] * a[6];
+    float det = a[0] * c00 + a[1] * c01 + a[2] * c02;
+    float id = 1.0f / det;
+    r[0] = c00 * id;
+    r[1] = (a[2] * a[7] - a[1] * a[8]) * id;
+    r[2] = (a[1] * a[5] - a[2] * a[4]) * id;
+    r[3] = c01 * id;
+    r[4] = (a[0] * a[8] - a[2] * a[6]) * id;
+    r[5] = (a[2] * a[3] - a[0] * a[5]) * id;
+    r[6] = c02 * id;
+    r[7] = (a[1] * a[6] - a[0] * a[7]) * id;
+    r[8] = (a[0] * a[4] - a[1] * a[3]) * id;
+}
+
+template <typename T> struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+};
+
+}  // namespace
+
+struct dcp_ctx {
+    int B = 0, NP = 0, NE = 0, n_patches = 0, n_cu = 256;
+    dcp_options opt{};
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    std::vector<HostInst> hi;
+    std::vector<InstDev> inst_h;
+    std::vector<Patch> patches_h;
+    std::vector<int> tri_h;            // internal [3][NE]
+    std::vector<int> elem_inst_h;
+    std::vector<int> vnat_h;           // per point: natural vertex index (vertices) or index within class
+    std::vector<int> toff_h, tcount_h;
+    std::vector<int4> tchunk_h;
+    size_t slot_floats = 0;
+    int max_states = 0, n_adj = 4;
+    long long grid_cells = 0;
+    int n_flags = 0;
+    SimDev sim{};
+    // owned device buffers
+    std::vector<void*> allocs;
+    float* states = nullptr;
+    float* adj = nullptr;
+    float* scratch = nullptr;
+    float* targets = nullptr;
+    int* toff = nullptr; int* tcount = nullptr; int4* tchunks = nullptr;
+    int* vnat = nullptr;
+    float* tmin = nullptr; int* targ = nullptr;
+    double* loss_d = nullptr; double* seed_d = nullptr;
+    int* keys_d = nullptr;
+    int n_targets_total = 0;
+    InstDev* inst_d = nullptr;
+    // graphs
+    std::map<std::tuple<int, int, int, int, int>, hipGraphExec_t> graphs;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double last_ms = 0.0;
+
+    StateView state(int slot) const { return StateView{states + (size_t)slot * slot_floats, NP, NE}; }
+    StateView adjv(int g) const { return StateView{adj + (size_t)g * slot_floats, NP, NE}; }
+    StateView scratch_view() const { return StateView{scratch, NP, NE}; }
+    StateView none() const { return StateView{nullptr, NP, NE}; }
+
+    template <typename T> int alloc(T** p, size_t n) {
+        void* q = nullptr;
+        size_t bytes = std::max<size_t>(n, 1) * sizeof(T);
+        hipError_t e = hipMalloc(&q, bytes);
+        if (e != hipSuccess) return set_err(std::string("hipMalloc ") + std::to_string(bytes) + " B: " + hipGetErrorString(e));
+        e = hipMemset(q, 0, bytes);
+        if (e != hipSuccess) return set_err(std::string("hipMemset: ") + hipGetErrorString(e));
+        allocs.push_back(q);
+        *p = (T*)q;
+        return 0;
+    }
+    template <typename T> int upload(T** p, const std::vector<T>& h) {
+        if (alloc(p, h.size())) return -1;
+        if (!h.empty()) HIP_OK(hipMemcpy(*p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+        return 0;
+    }
+    int grid_patches() const { return std::max(1, std::min(n_patches, n_cu * 8)); }
+    int grid_elems() const { return std::max(1, std::min((NE + DCP_WG - 1) / DCP_WG, n_cu * 16)); }
+    int grid_gridop() const { return opt.grid_wgs > 0 ? opt.grid_wgs : n_cu * 4; }
+};
+
+static bool slot_ok(const dcp_ctx* c, int s) { return s >= 0 && s < c->max_states; }
+static bool gslot_ok(const dcp_ctx* c, int s) { return s >= 0 && s < c->n_adj; }
+
+extern "C" const char* dcp_last_error(void) { return g_err.c_str(); }
+
+// ---------------------------------------------------------------------------
+// launch sequences
+// ---------------------------------------------------------------------------
+static int launch_prep(dcp_ctx* c, StateView S, StateView Sn, int fresh) {
+    if (c->opt.precision == 1)
+        hipLaunchKernelGGL(k_prep<float>, dim3(c->grid_elems()), dim3(DCP_WG), 0, c->stream, c->sim, S, Sn, fresh);
+    else
+        hipLaunchKernelGGL(k_prep<double>, dim3(c->grid_elems()), dim3(DCP_WG), 0, c->stream, c->sim, S, Sn, fresh);
+    LAUNCH_OK();
+    return 0;
+}
+
+static int launch_forward(dcp_ctx* c, int s, int g, StateView S, StateView Sn) {
+    if (launch_prep(c, S, Sn, s == 0)) return -1;
+    hipLaunchKernelGGL(k_p2g, dim3(c->grid_patches()), dim3(DCP_WG), 0, c->stream, c->sim, S, s, g);
+    LAUNCH_OK();
+    hipLaunchKernelGGL(k_gridop, dim3(c->grid_gridop()), dim3(DCP_WG), 0, c->stream, c->sim, 0);
+    LAUNCH_OK();
+    hipLaunchKernelGGL(k_g2p, dim3(c->grid_patches()), dim3(DCP_WG), 0, c->stream, c->sim, S, Sn);
+    LAUNCH_OK();
+    return 0;
+}
+
+static int launch_adjoint(dcp_ctx* c, int s, int g, StateView S, StateView G1, StateView Gout) {
+    const int fresh = (s == 0);
+    if (launch_prep(c, S, c->scratch_view(), fresh)) return -1;
+    hipLaunchKernelGGL(k_p2g, dim3(c->grid_patches()), dim3(DCP_WG), 0, c->stream, c->sim, S, s, g);
+    LAUNCH_OK();
+    hipLaunchKernelGGL(k_gridop, dim3(c->grid_gridop()), dim3(DCP_WG), 0, c->stream, c->sim, 1);
+    LAUNCH_OK();
+    hipLaunchKernelGGL(k_g2p_adj, dim3(c->grid_patches()), dim3(DCP_WG), 0, c->stream, c->sim, S, G1, Gout);
+    LAUNCH_OK();
+    hipLaunchKernelGGL(k_gridop_adj, dim3(c->grid_gridop()), dim3(DCP_WG), 0, c->stream, c->sim);
+    LAUNCH_OK();
+    hipLaunchKernelGGL(k_p2g_adj, dim3(c->grid_patches()), dim3(DCP_WG), 0, c->stream, c->sim, S, Gout, s, g);
+    LAUNCH_OK();
+    if (c->opt.precision == 1)
+        hipLaunchKernelGGL(k_prep_adj<float>, dim3(c->grid_patches()), dim3(DCP_WG), 0, c->stream, c->sim, S, G1, Gout, fresh);
+    else
+        hipLaunchKernelGGL(k_prep_adj<double>, dim3(c->grid_patches()), dim3(DCP_WG), 0, c->stream, c->sim, S, G1, Gout, fresh);
+    LAUNCH_OK();
+    hipLaunchKernelGGL(k_vertex_final, dim3(c->grid_patches()), dim3(DCP_WG), 0, c->stream, c->sim, Gout);
+    LAUNCH_OK();
+    return 0;
+}
+
+static int launch_fill(dcp_ctx* c, float* p, size_t n) {
+    int blocks = (int)std::min<size_t>((n + 255) / 256, (size_t)c->n_cu * 16);
+    hipLaunchKernelGGL(k_fill_f, dim3(std::max(1, blocks)), dim3(256), 0, c->stream, p, n, 0.0f);
+    LAUNCH_OK();
+    return 0;
+}
+
+static int launch_loss(dcp_ctx* c, StateView S) {
+    if (c->n_targets_total == 0) return set_err("no targets set (dcp_set_targets)");
+    int nch = (int)c->tchunk_h.size();
+    hipLaunchKernelGGL(k_loss_min, dim3(std::max(1, std::min(nch, c->n_cu * 8))), dim3(DCP_WG), 0, c->stream, c->sim,
+                       S, c->targets, c->tchunks, nch, c->vnat, c->tmin, c->targ);
+    LAUNCH_OK();
+    hipLaunchKernelGGL(k_loss_sum, dim3(std::min(c->B, 1024)), dim3(DCP_WG), 0, c->stream, c->tmin, c->toff, c->tcount,
+                       c->loss_d, c->B);
+    LAUNCH_OK();
+    return 0;
+}
+
+static int launch_loss_adj(dcp_ctx* c, StateView S, StateView G) {
+    hipLaunchKernelGGL(k_loss_adj, dim3(c->grid_patches()), dim3(DCP_WG), 0, c->stream, c->sim, S, G, c->targets,
+                       c->toff, c->tcount, c->targ, c->seed_d);
+    LAUNCH_OK();
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// creation
+// ---------------------------------------------------------------------------
+static int build(dcp_ctx* c, const dcp_instance_desc* d, int B) {
+    c->B = B;
+    c->hi.resize(B);
+    c->inst_h.resize(B);
+    int np = 0, ne = 0;
+    for (int b = 0; b < B; ++b) {
+        const dcp_instance_desc& D = d[b];
+        if (D.n_grid < 4 || D.n_vertices <= 0 || D.n_elements < 0 || D.n_cube < 0 || D.n_grasper < 0)
+            return set_err("instance " + std::to_string(b) + ": invalid sizes");
+        if (D.n_grasper > 0 && (D.n_grasp_points <= 0 || D.n_grasper % D.n_grasp_points != 0))
+            return set_err("instance " + std::to_string(b) + ": n_grasper must be a multiple of n_grasp_points");
+        if (D.steps_per_frame <= 0) return set_err("steps_per_frame must be > 0");
+        if (!D.vertex_x || (D.n_elements && !D.triangles)) return set_err("missing vertex/triangle arrays");
+        HostInst& H = c->hi[b];
+        H.G = D.n_grid; H.nv = D.n_vertices; H.ne = D.n_elements; H.nc = D.n_cube; H.no = D.n_grasper;
+        H.ngp = D.n_grasp_points; H.nact = D.n_actions;
+        H.x0v.assign(3 * H.nv, 0.f); H.x0o.assign(3 * H.no, 0.f); H.x0c.assign(3 * H.nc, 0.f);
+        for (int i = 0; i < 3 * H.nv; ++i) H.x0v[i] = (float)D.vertex_x[i];
+        for (int i = 0; i < 3 * H.no; ++i) H.x0o[i] = (float)D.grasper_x[i];
+        for (int i = 0; i < 3 * H.nc; ++i) H.x0c[i] = (float)D.cube_x[i];
+        H.tri.assign(D.triangles, D.triangles + 3 * H.ne);
+        for (int t : H.tri)
+            if (t < 0 || t >= H.nv) return set_err("triangle index out of range");
+        // spatial (Morton) order of vertices and element centroids
+        std::vector<uint32_t> code(H.nv);
+        for (int i = 0; i < H.nv; ++i) code[i] = morton(H.x0v[3 * i], H.x0v[3 * i + 1], H.x0v[3 * i + 2]);
+        H.vperm.resize(H.nv);
+        std::iota(H.vperm.begin(), H.vperm.end(), 0);
+        std::stable_sort(H.vperm.begin(), H.vperm.end(), [&](int a, int bb) { return code[a] < code[bb]; });
+        H.vinv.resize(H.nv);
+        for (int i = 0; i < H.nv; ++i) H.vinv[H.vperm[i]] = i;
+        std::vector<uint32_t> ecode(H.ne);
+        for (int e = 0; e < H.ne; ++e) {
+            float cx = 0, cy = 0, cz = 0;
+            for (int k = 0; k < 3; ++k) {
+                int v = H.tri[3 * e + k];
+                cx += H.x0v[3 * v]; cy += H.x0v[3 * v + 1]; cz += H.x0v[3 * v + 2];
+            }
+            ecode[e] = morton(cx / 3, cy / 3, cz / 3);
+        }
+        H.eperm.resize(H.ne);
+        std::iota(H.eperm.begin(), H.eperm.end(), 0);
+        std::stable_sort(H.eperm.begin(), H.eperm.end(), [&](int a, int bb) { return ecode[a] < ecode[bb]; });
+        H.einv.resize(H.ne);
+        for (int i = 0; i < H.ne; ++i) H.einv[H.eperm[i]] = i;
+
+        InstDev& I = c->inst_h[b];
+        memset(&I, 0, sizeof(I));
+        I.G = D.n_grid; I.nb = (D.n_grid + 3) / 4;
+        I.steps_per_frame = D.steps_per_frame; I.frames_per_window = D.frames_per_window;
+        I.group_size = D.n_grasp_points > 0 ? D.n_grasper / D.n_grasp_points : 1;
+        if (I.group_size <= 0) I.group_size = 1;
+        I.n_actions = D.n_actions; I.n_grasp = D.n_grasp_points;
+        I.v_start = np; I.v_count = H.nv; np += H.nv;
+        I.o_start = np; I.o_count = H.no; np += H.no;
+        I.c_start = np; I.c_count = H.nc; np += H.nc;
+        I.e_start = ne; I.e_count = H.ne; ne += H.ne;
+        I.dt = (float)D.dt; I.dx = (float)(1.0 / D.n_grid); I.inv_dx = (float)D.n_grid; I.gravity = (float)D.gravity;
+        I.gdt = (float)(D.dt * D.gravity);
+        I.half_dt2 = (float)(0.5 * D.dt * D.dt);
+        I.m_vertex = (float)D.vertex_p_mass; I.m_quad = (float)D.quadrature_p_mass; I.vol_quad = (float)D.quadrature_p_vol;
+        I.m_cube = (float)D.cube_p_mass; I.m_op = (float)D.operator_p_mass;
+        I.interval = (float)(D.steps_per_frame * D.dt);
+        I.E = 5e3f; I.nu = 0.3f; I.ks = 100.f; I.kc = 100.f;
+    }
+    c->NP = np; c->NE = ne;
+    // grid pools
+    long long cells = 0; int flags = 0, traj = 0;
+    for (int b = 0; b < B; ++b) {
+        InstDev& I = c->inst_h[b];
+        I.grid_base = cells; I.flag_base = flags; I.traj_off = traj;
+        long long nbl = (long long)I.nb * I.nb * I.nb;
+        cells += nbl * 64; flags += (int)nbl; traj += I.n_actions * I.n_grasp;
+    }
+    c->grid_cells = cells; c->n_flags = flags;
+    // static element data in internal order
+    c->tri_h.assign(3 * (size_t)ne, 0);
+    c->elem_inst_h.assign(ne, 0);
+    std::vector<int> inc_cnt(np + 1, 0);
+    for (int b = 0; b < B; ++b) {
+        const HostInst& H = c->hi[b];
+        const InstDev& I = c->inst_h[b];
+        for (int ei = 0; ei < H.ne; ++ei) {
+            int en = H.eperm[ei];
+            int e = I.e_start + ei;
+            c->elem_inst_h[e] = b;
+            for (int k = 0; k < 3; ++k) {
+                int p = I.v_start + H.vinv[H.tri[3 * en + k]];
+                c->tri_h[(size_t)k * ne + e] = p;
+                inc_cnt[p + 1]++;
+            }
+        }
+    }
+    std::vector<int> inc_off(np + 1, 0);
+    for (int p = 0; p < np; ++p) inc_off[p + 1] = inc_off[p] + inc_cnt[p + 1];
+    std::vector<int> inc_list(std::max(1, inc_off[np])), fillp(inc_off.begin(), inc_off.end() - 1);
+    for (int e = 0; e < ne; ++e)
+        for (int k = 0; k < 3; ++k) {
+            int p = c->tri_h[(size_t)k * ne + e];
+            inc_list[fillp[p]++] = e * 3 + k;
+        }
+    // natural index of each point (vertex natural index; grasper/cube index in class)
+    c->vnat_h.assign(np, 0);
+    for (int b = 0; b < B; ++b) {
+        const HostInst& H = c->hi[b];
+        const InstDev& I = c->inst_h[b];
+        for (int i = 0; i < H.nv; ++i) c->vnat_h[I.v_start + i] = H.vperm[i];
+        for (int i = 0; i < H.no; ++i) c->vnat_h[I.o_start + i] = i;
+        for (int i = 0; i < H.nc; ++i) c->vnat_h[I.c_start + i] = i;
+    }
+    // patches: point patches (vertex, grasper, cube) then element patches
+    int ps = c->opt.patch_size > 0 ? std::min(c->opt.patch_size, DCP_PATCH_MAX) : 512;
+    auto chunk = [&](int b, int cls, int start, int count) {
+        for (int o = 0; o < count; o += ps) c->patches_h.push_back(Patch{b, cls, start + o, std::min(ps, count - o)});
+    };
+    for (int b = 0; b < B; ++b) {
+        const InstDev& I = c->inst_h[b];
+        chunk(b, CLS_VERTEX, I.v_start, I.v_count);
+        chunk(b, CLS_GRASPER, I.o_start, I.o_count);
+        chunk(b, CLS_CUBE, I.c_start, I.c_count);
+    }
+    for (int b = 0; b < B; ++b) {
+        const InstDev& I = c->inst_h[b];
+        chunk(b, CLS_ELEMENT, I.e_start, I.e_count);
+    }
+    c->n_patches = (int)c->patches_h.size();
+    // trajectory pool
+    std::vector<float> trajh(3 * (size_t)std::max(1, traj), 0.f);
+    for (int b = 0; b < B; ++b) {
+        const InstDev& I = c->inst_h[b];
+        size_t n = (size_t)I.n_actions * I.n_grasp * 3;
+        for (size_t i = 0; i < n; ++i) trajh[3 * (size_t)I.traj_off + i] = (float)d[b].trajectory[i];
+    }
+
+    // ---- device allocations ----
+    SimDev& S = c->sim;
+    S.B = B; S.NP = np; S.NE = ne; S.n_patches = c->n_patches;
+    S.precision = c->opt.precision; S.svd_grad = c->opt.svd_grad; S.per_elem = c->opt.per_element_moduli;
+    S.max_active = flags;
+    c->slot_floats = 15 * (size_t)np + 12 * (size_t)ne;
+    c->max_states = std::max(2, c->opt.max_states);
+    Patch* pd; int *trid, *eid, *iod, *ild; float* trd;
+    if (c->upload(&pd, c->patches_h) || c->upload(&trid, c->tri_h) || c->upload(&eid, c->elem_inst_h) ||
+        c->upload(&iod, inc_off) || c->upload(&ild, inc_list) || c->upload(&trd, trajh) ||
+        c->upload(&c->inst_d, c->inst_h) || c->upload(&c->vnat, c->vnat_h))
+        return -1;
+    S.inst = c->inst_d; S.patches = pd; S.tri = trid; S.elem_inst = eid; S.inc_off = iod; S.inc_list = ild; S.traj = trd;
+    float *Dinv, *Dii, *emod;
+    if (c->alloc(&Dinv, 9 * (size_t)ne) || c->alloc(&Dii, 9 * (size_t)ne)) return -1;
+    S.Dinv = Dinv; S.Dinvinv = Dii;
+    if (c->opt.per_element_moduli) {
+        if (c->alloc(&emod, 3 * (size_t)ne)) return -1;
+        S.emod = emod;
+    }
+    if (c->alloc(&S.xe, 3 * (size_t)ne) || c->alloc(&S.ve, 3 * (size_t)ne) || c->alloc(&S.ke, 3 * (size_t)ne) ||
+        c->alloc(&S.fc, 9 * (size_t)ne) || c->alloc(&S.nd, 3 * (size_t)ne) || c->alloc(&S.acc, (size_t)cells) ||
+        c->alloc(&S.vout, (size_t)cells) || c->alloc(&S.gacc, (size_t)cells) || c->alloc(&S.gmv, (size_t)cells) ||
+        c->alloc(&S.flags, (size_t)flags) || c->alloc(&S.active, (size_t)flags) || c->alloc(&S.n_active, 1) ||
+        c->alloc(&S.status, 1) || c->alloc(&S.g_xe, 3 * (size_t)ne) || c->alloc(&S.g_ve, 3 * (size_t)ne) ||
+        c->alloc(&S.g_ne, 3 * (size_t)ne) || c->alloc(&S.g_ke, 3 * (size_t)ne) || c->alloc(&S.g_fv, 3 * (size_t)np) ||
+        c->alloc(&S.ec_gx, 9 * (size_t)ne) || c->alloc(&S.ec_gv, 9 * (size_t)ne) ||
+        c->alloc(&S.pgrad_patch, 4 * (size_t)c->n_patches) || c->alloc(&S.pgrad_inst, (size_t)DCP_N_PARAMS * B) ||
+        c->alloc(&S.emod_grad, 3 * (size_t)ne) || c->alloc(&S.traj_grad, 3 * (size_t)std::max(1, traj)))
+        return -1;
+    if (c->alloc(&c->states, c->slot_floats * c->max_states) || c->alloc(&c->adj, c->slot_floats * c->n_adj) ||
+        c->alloc(&c->scratch, c->slot_floats) || c->alloc(&c->loss_d, B) || c->alloc(&c->seed_d, B) ||
+        c->alloc(&c->keys_d, 3 * (size_t)(np + ne)))
+        return -1;
+    std::vector<double> ones(B, 1.0);
+    HIP_OK(hipMemcpy(c->seed_d, ones.data(), B * sizeof(double), hipMemcpyHostToDevice));
+    HIP_OK(hipEventCreate(&c->ev0));
+    HIP_OK(hipEventCreate(&c->ev1));
+    return 0;
+}
+
+extern "C" int dcp_create(const dcp_instance_desc* instances, int32_t n, const dcp_options* options, void* stream,
+                          dcp_ctx** out) {
+    if (!instances || n <= 0 || !out) return set_err("dcp_create: bad arguments");
+    dcp_ctx* c = new dcp_ctx();
+    if (options) c->opt = *options;
+    if (c->opt.max_states <= 0) c->opt.max_states = 2;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) { delete c; return set_err("no HIP device"); }
+    int cu = 0;
+    if (hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && cu > 0) c->n_cu = cu;
+    if (stream) {
+        c->stream = (hipStream_t)stream;
+    } else {
+        if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return set_err("stream"); }
+        c->own_stream = true;
+    }
+    if (build(c, instances, n)) {
+        std::string e = g_err;
+        dcp_destroy(c);
+        return set_err(e);
+    }
+    *out = c;
+    return 0;
+}
+
+extern "C" int dcp_destroy(dcp_ctx* c) {
+    if (!c) return 0;
+    for (auto& kv : c->graphs) hipGraphExecDestroy(kv.second);
+    for (void* p : c->allocs) hipFree(p);
+    if (c->ev0) hipEventDestroy(c->ev0);
+    if (c->ev1) hipEventDestroy(c->ev1);
+    if (c->own_stream) hipStreamDestroy(c->stream);
+    delete c;
+    return 0;
+}
+
+extern "C" int dcp_sizes(dcp_ctx* c, int64_t* o) {
+    if (!c || !o) return set_err("bad args");
+    o[0] = c->B; o[1] = c->NP; o[2] = c->NE; o[3] = (int64_t)c->slot_floats; o[4] = c->max_states;
+    return 0;
+}
+
+extern "C" int dcp_set_params(dcp_ctx* c, const double* p) {
+    if (!c || !p) return set_err("bad args");
+    for (int b = 0; b < c->B; ++b) {
+        const double* q = p + (size_t)b * DCP_N_PARAMS;
+        if (!(q[1] > -1.0 && q[1] < 0.5)) return set_err("nu must lie in (-1, 0.5)");
+        InstDev& I = c->inst_h[b];
+        I.E = (float)q[0]; I.nu = (float)q[1]; I.ks = (float)q[2]; I.kc = (float)q[3];
+        I.damping = (float)q[4]; I.friction = (float)q[5];
+    }
+    HIP_OK(hipMemcpyAsync(c->inst_d, c->inst_h.data(), c->inst_h.size() * sizeof(InstDev), hipMemcpyHostToDevice,
+                          c->stream));
+    HIP_OK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+extern "C" int dcp_set_element_moduli(dcp_ctx* c, const double* m) {
+    if (!c || !m) return set_err("bad args");
+    if (!c->opt.per_element_moduli) return set_err("context built without per_element_moduli");
+    std::vector<float> h(3 * (size_t)c->NE);
+    size_t nat = 0;
+    for (int b = 0; b < c->B; ++b) {
+        const HostInst& H = c->hi[b];
+        const InstDev& I = c->inst_h[b];
+        for (int en = 0; en < H.ne; ++en) {
+            int e = I.e_start + H.einv[en];
+            for (int k = 0; k < 3; ++k) h[(size_t)k * c->NE + e] = (float)m[3 * (nat + en) + k];
+        }
+        nat += H.ne;
+    }
+    HIP_OK(hipMemcpy((void*)c->sim.emod, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice));
+    return 0;
+}
+
+extern "C" int dcp_set_targets(dcp_ctx* c, const double* t, const int32_t* counts) {
+    if (!c || !counts) return set_err("bad args");
+    int tot = 0;
+    c->toff_h.assign(c->B, 0);
+    c->tcount_h.assign(c->B, 0);
+    c->tchunk_h.clear();
+    for (int b = 0; b < c->B; ++b) {
+        if (counts[b] < 0) return set_err("negative target count");
+        c->toff_h[b] = tot; c->tcount_h[b] = counts[b];
+        for (int o = 0; o < counts[b]; o += DCP_WG) c->tchunk_h.push_back(make_int4(b, tot + o, std::min(DCP_WG, counts[b] - o), counts[b]));
+        tot += counts[b];
+    }
+    std::vector<float> h(3 * (size_t)std::max(1, tot));
+    for (int i = 0; i < 3 * tot; ++i) h[i] = (float)t[i];
+    if (tot > c->n_targets_total || !c->targets) {
+        if (c->alloc(&c->targets, h.size()) || c->alloc(&c->tmin, (size_t)std::max(1, tot)) ||
+            c->alloc(&c->targ, (size_t)std::max(1, tot)))
+            return -1;
+    }
+    if (!c->toff) {
+        if (c->alloc(&c->toff, c->B) || c->alloc(&c->tcount, c->B)) return -1;
+    }
+    if (c->tchunks) { /* reallocate chunks each time (small) */ }
+    if (c->upload(&c->tchunks, c->tchunk_h)) return -1;
+    HIP_OK(hipMemcpy(c->targets, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(c->toff, c->toff_h.data(), c->B * sizeof(int), hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(c->tcount, c->tcount_h.data(), c->B * sizeof(int), hipMemcpyHostToDevice));
+    c->n_targets_total = tot;
+    return 0;
+}
+
+// initialize_ti_field (simulation.py:447-477) in FP32 on the host
+extern "C" int dcp_reset(dcp_ctx* c, int32_t slot, const double* vx, const double* ox, const double* cx) {
+    if (!c || !slot_ok(c, slot)) return set_err("dcp_reset: bad slot");
+    const int NP = c->NP, NE = c->NE;
+    std::vector<float> st(c->slot_floats, 0.f);
+    std::vector<float> Dinv(9 * (size_t)NE), Dii(9 * (size_t)NE);
+    size_t nv = 0, no = 0, nc = 0;
+    for (int b = 0; b < c->B; ++b) {
+        HostInst& H = c->hi[b];
+        const InstDev& I = c->inst_h[b];
+        if (vx) for (int i = 0; i < 3 * H.nv; ++i) H.x0v[i] = (float)vx[3 * nv + i];
+        if (ox) for (int i = 0; i < 3 * H.no; ++i) H.x0o[i] = (float)ox[3 * no + i];
+        if (cx) for (int i = 0; i < 3 * H.nc; ++i) H.x0c[i] = (float)cx[3 * nc + i];
+        nv += H.nv; no += H.no; nc += H.nc;
+        for (int i = 0; i < H.nv; ++i) {
+            int p = I.v_start + H.vinv[i];
+            for (int k = 0; k < 3; ++k) st[(size_t)k * NP + p] = H.x0v[3 * i + k];
+        }
+        for (int i = 0; i < H.no; ++i)
+            for (int k = 0; k < 3; ++k) st[(size_t)k * NP + I.o_start + i] = H.x0o[3 * i + k];
+        for (int i = 0; i < H.nc; ++i)
+            for (int k = 0; k < 3; ++k) st[(size_t)k * NP + I.c_start + i] = H.x0c[3 * i + k];
+        for (int ei = 0; ei < H.ne; ++ei) {
+            int en = H.eperm[ei], e = I.e_start + ei;
+            const float* a = &H.x0v[3 * H.tri[3 * en]];
+            const float* bb = &H.x0v[3 * H.tri[3 * en + 1]];
+            const float* cc = &H.x0v[3 * H.tri[3 * en + 2]];
+            float d1[3], d2[3], n1[3], n2[3], c3[3], n3[3];
+            for (int k = 0; k < 3; ++k) { d1[k] = bb[k] - a[k]; d2[k] = cc[k] - a[k]; }
+            float l1 = std::sqrt(d1[0] * d1[0] + d1[1] * d1[1] + d1[2] * d1[2] + 1e-6f);
+            float l2 = std::sqrt(d2[0] * d2[0] + d2[1] * d2[1] + d2[2] * d2[2] + 1e-6f);
+            for (int k = 0; k < 3; ++k) { n1[k] = d1[k] / l1; n2[k] = d2[k] / l2; }
+            c3[0] = n1[1] * n2[2] - n1[2] * n2[1];
+            c3[1] = n1[2] * n2[0] - n1[0] * n2[2];
+            c3[2] = n1[0] * n2[1] - n1[1] * n2[0];
+            float l3 = std::sqrt(c3[0] * c3[0] + c3[1] * c3[1] + c3[2] * c3[2] + 1e-6f);
+            for (int k = 0; k < 3; ++k) n3[k] = c3[k] / l3;
+            float D[9] = {d1[0], d2[0], n3[0], d1[1], d2[1], n3[1], d1[2], d2[2], n3[2]};
+            float Di[9], Dii9[9];
+            inv3h(D, Di);
+            inv3h(Di, Dii9);
+            for (int k = 0; k < 9; ++k) { Dinv[(size_t)k * NE + e] = Di[k]; Dii[(size_t)k * NE + e] = Dii9[k]; }
+            for (int k = 0; k < 3; ++k) st[15 * (size_t)NP + (size_t)(9 + k) * NE + e] = n3[k];
+        }
+    }
+    HIP_OK(hipMemcpyAsync((void*)c->sim.Dinv, Dinv.data(), Dinv.size() * 4, hipMemcpyHostToDevice, c->stream));
+    HIP_OK(hipMemcpyAsync((void*)c->sim.Dinvinv, Dii.data(), Dii.size() * 4, hipMemcpyHostToDevice, c->stream));
+    HIP_OK(hipMemcpyAsync(c->states + (size_t)slot * c->slot_floats, st.data(), st.size() * 4, hipMemcpyHostToDevice,
+                          c->stream));
+    HIP_OK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+extern "C" int dcp_substep(dcp_ctx* c, int32_t s, int32_t g, int32_t si, int32_t so) {
+    if (!c || !slot_ok(c, si) || !slot_ok(c, so) || si == so || s < 0) return set_err("dcp_substep: bad arguments");
+    return launch_forward(c, s, g, c->state(si), c->state(so));
+}
+
+extern "C" int dcp_substep_adjoint(dcp_ctx* c, int32_t s, int32_t g, int32_t si, int32_t gi, int32_t go) {
+    if (!c || !slot_ok(c, si) || !gslot_ok(c, gi) || !gslot_ok(c, go) || gi == go || s < 0)
+        return set_err("dcp_substep_adjoint: bad arguments");
+    return launch_adjoint(c, s, g, c->state(si), c->adjv(gi), c->adjv(go));
+}
+
+extern "C" int dcp_zero_adjoint(dcp_ctx* c, int32_t gs) {
+    if (!c || !gslot_ok(c, gs)) return set_err("bad adjoint slot");
+    return launch_fill(c, c->adj + (size_t)gs * c->slot_floats, c->slot_floats);
+}
+
+extern "C" int dcp_zero_param_grads(dcp_ctx* c) {
+    if (!c) return set_err("bad args");
+    HIP_OK(hipMemsetAsync(c->sim.pgrad_patch, 0, 4 * sizeof(double) * c->n_patches, c->stream));
+    HIP_OK(hipMemsetAsync(c->sim.pgrad_inst, 0, DCP_N_PARAMS * sizeof(double) * c->B, c->stream));
+    HIP_OK(hipMemsetAsync(c->sim.emod_grad, 0, 3 * sizeof(double) * c->NE, c->stream));
+    size_t nt = 0;
+    for (auto& I : c->inst_h) nt += (size_t)I.n_actions * I.n_grasp;
+    HIP_OK(hipMemsetAsync(c->sim.traj_grad, 0, 3 * sizeof(double) * std::max<size_t>(1, nt), c->stream));
+    return 0;
+}
+
+extern "C" int dcp_loss(dcp_ctx* c, int32_t slot, double* out) {
+    if (!c || !slot_ok(c, slot)) return set_err("dcp_loss: bad slot");
+    if (launch_loss(c, c->state(slot))) return -1;
+    if (out) {
+        HIP_OK(hipMemcpyAsync(out, c->loss_d, c->B * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIP_OK(hipStreamSynchronize(c->stream));
+    }
+    return 0;
+}
+
+extern "C" int dcp_loss_adjoint(dcp_ctx* c, int32_t slot, int32_t gs, const double* seed) {
+    if (!c || !slot_ok(c, slot) || !gslot_ok(c, gs)) return set_err("dcp_loss_adjoint: bad slot");
+    if (seed) HIP_OK(hipMemcpyAsync(c->seed_d, seed, c->B * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    if (launch_loss(c, c->state(slot))) return -1;
+    if (launch_loss_adj(c, c->state(slot), c->adjv(gs))) return -1;
+    HIP_OK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+static int reduce_param_grads(dcp_ctx* c, double* out) {
+    std::vector<double> pp(4 * (size_t)c->n_patches), pi((size_t)DCP_N_PARAMS * c->B);
+    HIP_OK(hipMemcpyAsync(pp.data(), c->sim.pgrad_patch, pp.size() * 8, hipMemcpyDeviceToHost, c->stream));
+    HIP_OK(hipMemcpyAsync(pi.data(), c->sim.pgrad_inst, pi.size() * 8, hipMemcpyDeviceToHost, c->stream));
+    HIP_OK(hipStreamSynchronize(c->stream));
+    for (int b = 0; b < c->B; ++b)
+        for (int k = 0; k < DCP_N_PARAMS; ++k) out[b * DCP_N_PARAMS + k] = pi[b * DCP_N_PARAMS + k];
+    for (int p = 0; p < c->n_patches; ++p) {
+        const Patch& P = c->patches_h[p];
+        if (P.cls != CLS_ELEMENT) continue;
+        for (int k = 0; k < 4; ++k) out[P.inst * DCP_N_PARAMS + k] += pp[4 * (size_t)p + k];
+    }
+    return 0;
+}
+
+extern "C" int dcp_param_grads(dcp_ctx* c, double* out) {
+    if (!c || !out) return set_err("bad args");
+    return reduce_param_grads(c, out);
+}
+
+extern "C" int dcp_element_moduli_grads(dcp_ctx* c, double* out) {
+    if (!c || !out) return set_err("bad args");
+    std::vector<double> h(3 * (size_t)c->NE);
+    HIP_OK(hipMemcpyAsync(h.data(), c->sim.emod_grad, h.size() * 8, hipMemcpyDeviceToHost, c->stream));
+    HIP_OK(hipStreamSynchronize(c->stream));
+    size_t nat = 0;
+    for (int b = 0; b < c->B; ++b) {
+        const HostInst& H = c->hi[b];
+        const InstDev& I = c->inst_h[b];
+        for (int en = 0; en < H.ne; ++en) {
+            int e = I.e_start + H.einv[en];
+            for (int k = 0; k < 3; ++k) out[3 * (nat + en) + k] = h[(size_t)k * c->NE + e];
+        }
+        nat += H.ne;
+    }
+    return 0;
+}
+
+extern "C" int dcp_trajectory_grads(dcp_ctx* c, double* out) {
+    if (!c || !out) return set_err("bad args");
+    size_t nt = 0;
+    for (auto& I : c->inst_h) nt += (size_t)I.n_actions * I.n_grasp;
+    HIP_OK(hipMemcpyAsync(out, c->sim.traj_grad, 3 * nt * 8, hipMemcpyDeviceToHost, c->stream));
+    HIP_OK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+// natural-order host arrays <-> one device slot (11 arrays, any may be null)
+struct HostArrays {
+    float* a[11];
+};
+
+static void slot_to_host(const dcp_ctx* c, const std::vector<float>& h, const HostArrays& A) {
+    const int NP = c->NP, NE = c->NE;
+    size_t nv = 0, no = 0, nc = 0, ne = 0;
+    auto put_point = [&](int p, size_t idx, float* x, float* v, float* C) {
+        for (int k = 0; k < 3; ++k) {
+            if (x) x[3 * idx + k] = h[(size_t)k * NP + p];
+            if (v) v[3 * idx + k] = h[(size_t)(3 + k) * NP + p];
+        }
+        if (C) for (int k = 0; k < 9; ++k) C[9 * idx + k] = h[(size_t)(6 + k) * NP + p];
+    };
+    for (int b = 0; b < c->B; ++b) {
+        const HostInst& H = c->hi[b];
+        const InstDev& I = c->inst_h[b];
+        for (int i = 0; i < H.nv; ++i) put_point(I.v_start + H.vinv[i], nv + i, A.a[0], A.a[1], A.a[2]);
+        for (int i = 0; i < H.no; ++i) put_point(I.o_start + i, no + i, A.a[3], A.a[4], A.a[5]);
+        for (int i = 0; i < H.nc; ++i) put_point(I.c_start + i, nc + i, A.a[6], A.a[7], A.a[8]);
+        for (int en0 = 0; en0 < H.ne; ++en0) {
+            int e = I.e_start + H.einv[en0];
+            if (A.a[9]) for (int k = 0; k < 9; ++k) A.a[9][9 * (ne + en0) + k] = h[15 * (size_t)NP + (size_t)k * NE + e];
+            if (A.a[10]) for (int k = 0; k < 3; ++k) A.a[10][3 * (ne + en0) + k] = h[15 * (size_t)NP + (size_t)(9 + k) * NE + e];
+        }
+        nv += H.nv; no += H.no; nc += H.nc; ne += H.ne;
+    }
+}
+
+static void host_to_slot(const dcp_ctx* c, std::vector<float>& h, const HostArrays& A) {
+    const int NP = c->NP, NE = c->NE;
+    size_t nv = 0, no = 0, nc = 0, ne = 0;
+    auto get_point = [&](int p, size_t idx, const float* x, const float* v, const float* C) {
+        for (int k = 0; k < 3; ++k) {
+            if (x) h[(size_t)k * NP + p] = x[3 * idx + k];
+            if (v) h[(size_t)(3 + k) * NP + p] = v[3 * idx + k];
+        }
+        if (C) for (int k = 0; k < 9; ++k) h[(size_t)(6 + k) * NP + p] = C[9 * idx + k];
+    };
+    for (int b = 0; b < c->B; ++b) {
+        const HostInst& H = c->hi[b];
+        const InstDev& I = c->inst_h[b];
+        for (int i = 0; i < H.nv; ++i) get_point(I.v_start + H.vinv[i], nv + i, A.a[0], A.a[1], A.a[2]);
+        for (int i = 0; i < H.no; ++i) get_point(I.o_start + i, no + i, A.a[3], A.a[4], A.a[5]);
+        for (int i = 0; i < H.nc; ++i) get_point(I.c_start + i, nc + i, A.a[6], A.a[7], A.a[8]);
+        for (int en0 = 0; en0 < H.ne; ++en0) {
+            int e = I.e_start + H.einv[en0];
+            if (A.a[9]) for (int k = 0; k < 9; ++k) h[15 * (size_t)NP + (size_t)k * NE + e] = A.a[9][9 * (ne + en0) + k];
+            if (A.a[10]) for (int k = 0; k < 3; ++k) h[15 * (size_t)NP + (size_t)(9 + k) * NE + e] = A.a[10][3 * (ne + en0) + k];
+        }
+        nv += H.nv; no += H.no; nc += H.nc; ne += H.ne;
+    }
+}
+
+static int download_to(dcp_ctx* c, const float* dev, const HostArrays& A) {
+    std::vector<float> h(c->slot_floats);
+    HIP_OK(hipMemcpyAsync(h.data(), dev, h.size() * 4, hipMemcpyDeviceToHost, c->stream));
+    HIP_OK(hipStreamSynchronize(c->stream));
+    slot_to_host(c, h, A);
+    return 0;
+}
+
+static int upload_from(dcp_ctx* c, float* dev, const HostArrays& A) {
+    std::vector<float> h(c->slot_floats, 0.f);
+    host_to_slot(c, h, A);
+    HIP_OK(hipMemcpyAsync(dev, h.data(), h.size() * 4, hipMemcpyHostToDevice, c->stream));
+    HIP_OK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+#define ARRS11 float *vx, float *vv, float *vC, float *ox, float *ov, float *oC, float *cx, float *cv, float *cC, float *eC, float *en
+#define CARRS11 const float *vx, const float *vv, const float *vC, const float *ox, const float *ov, const float *oC, \
+    const float *cx, const float *cv, const float *cC, const float *eC, const float *en
+#define PACK11 HostArrays{{(float*)vx, (float*)vv, (float*)vC, (float*)ox, (float*)ov, (float*)oC, (float*)cx, (float*)cv, (float*)cC, (float*)eC, (float*)en}}
+
+extern "C" int dcp_download_state(dcp_ctx* c, int32_t slot, ARRS11) {
+    if (!c || !slot_ok(c, slot)) return set_err("dcp_download_state: bad slot");
+    return download_to(c, c->states + (size_t)slot * c->slot_floats, PACK11);
+}
+
+extern "C" int dcp_upload_state(dcp_ctx* c, int32_t slot, CARRS11) {
+    if (!c || !slot_ok(c, slot)) return set_err("dcp_upload_state: bad slot");
+    return upload_from(c, c->states + (size_t)slot * c->slot_floats, PACK11);
+}
+
+extern "C" int dcp_download_adjoint(dcp_ctx* c, int32_t gs, ARRS11) {
+    if (!c || !gslot_ok(c, gs)) return set_err("dcp_download_adjoint: bad slot");
+    return download_to(c, c->adj + (size_t)gs * c->slot_floats, PACK11);
+}
+
+extern "C" int dcp_upload_adjoint(dcp_ctx* c, int32_t gs, CARRS11) {
+    if (!c || !gslot_ok(c, gs)) return set_err("dcp_upload_adjoint: bad slot");
+    return upload_from(c, c->adj + (size_t)gs * c->slot_floats, PACK11);
+}
+
+extern "C" int dcp_binning_keys(dcp_ctx* c, int32_t slot, int32_t* out) {
+    if (!c || !slot_ok(c, slot) || !out) return set_err("dcp_binning_keys: bad args");
+    // element positions are the vertex means computed by the prep kernel; a
+    // slot is "fresh" only for slot-0 states right after dcp_reset, which the
+    // caller signals through the sign of nothing: use the non-fresh form.
+    if (launch_prep(c, c->state(slot), c->scratch_view(), 0)) return -1;
+    hipLaunchKernelGGL(k_binning_keys, dim3(c->grid_patches()), dim3(DCP_WG), 0, c->stream, c->sim, c->state(slot), c->keys_d);
+    LAUNCH_OK();
+    std::vector<int> h(3 * (size_t)(c->NP + c->NE));
+    HIP_OK(hipMemcpyAsync(h.data(), c->keys_d, h.size() * 4, hipMemcpyDeviceToHost, c->stream));
+    HIP_OK(hipStreamSynchronize(c->stream));
+    // natural order: per instance, vertices, grasper, cube, then elements
+    size_t o = 0;
+    for (int b = 0; b < c->B; ++b) {
+        const HostInst& H = c->hi[b];
+        const InstDev& I = c->inst_h[b];
+        for (int i = 0; i < H.nv; ++i, ++o) for (int k = 0; k < 3; ++k) out[3 * o + k] = h[3 * (size_t)(I.v_start + H.vinv[i]) + k];
+        for (int i = 0; i < H.no; ++i, ++o) for (int k = 0; k < 3; ++k) out[3 * o + k] = h[3 * (size_t)(I.o_start + i) + k];
+        for (int i = 0; i < H.nc; ++i, ++o) for (int k = 0; k < 3; ++k) out[3 * o + k] = h[3 * (size_t)(I.c_start + i) + k];
+        for (int en = 0; en < H.ne; ++en, ++o)
+            for (int k = 0; k < 3; ++k) out[3 * o + k] = h[3 * (size_t)(c->NP + I.e_start + H.einv[en]) + k];
+    }
+    return 0;
+}
+
+extern "C" int dcp_status(dcp_ctx* c, uint32_t* out, int32_t clear) {
+    if (!c || !out) return set_err("bad args");
+    HIP_OK(hipMemcpyAsync(out, c->sim.status, 4, hipMemcpyDeviceToHost, c->stream));
+    if (clear) HIP_OK(hipMemsetAsync(c->sim.status, 0, 4, c->stream));
+    HIP_OK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// checkpointed rollouts, one hipGraph per segment
+// ---------------------------------------------------------------------------
+// slot map: checkpoints at [0, nck], segment buffer at [nck+1, nck+1+K]
+struct Plan {
+    int S, K, L, nseg, nck, seg0;
+    int ck(int c) const { return c; }
+    int seg(int j) const { return seg0 + j; }
+};
+
+static int plan_rollout(dcp_ctx* c, int S, int K, Plan& p) {
+    if (S < 3) return set_err("n_substeps must be >= 3 (the loss reads state n_substeps-2)");
+    if (K <= 0 || K > S) K = S;
+    p.S = S; p.K = K; p.L = S - 2;
+    p.nseg = (S + K - 1) / K;
+    p.nck = p.nseg;
+    p.seg0 = p.nck + 1;
+    if (p.seg0 + K + 1 > c->max_states)
+        return set_err("max_states too small for checkpoint_every=" + std::to_string(K) + ": need " +
+                       std::to_string(p.seg0 + K + 1));
+    return 0;
+}
+
+// slot holding state t inside segment cseg (states cseg*K .. )
+static int seg_slot(const Plan& p, int cseg, int t) {
+    int base = cseg * p.K;
+    if (t == base) return p.ck(cseg);
+    int end = std::min(base + p.K, p.S);
+    if (t == end) return p.ck(cseg + 1);
+    return p.seg(t - base);
+}
+
+static int record_forward_segment(dcp_ctx* c, const Plan& p, int cseg, int g) {
+    int base = cseg * p.K, end = std::min(base + p.K, p.S);
+    for (int f = base; f < end; ++f) {
+        if (launch_forward(c, f, g, c->state(seg_slot(p, cseg, f)), c->state(seg_slot(p, cseg, f + 1)))) return -1;
+        if (f + 1 == p.L) {
+            if (launch_loss(c, c->state(seg_slot(p, cseg, f + 1)))) return -1;
+            if (launch_fill(c, c->adj, c->slot_floats)) return -1;
+            if (launch_loss_adj(c, c->state(seg_slot(p, cseg, f + 1)), c->adjv(0))) return -1;
+        }
+    }
+    return 0;
+}
+
+// adjoint slots ping-pong: gradient of state t lives in adj[(L - t) & 1]
+static int record_backward_segment(dcp_ctx* c, const Plan& p, int cseg, int g) {
+    int base = cseg * p.K;
+    int end = std::min(base + p.K, p.L);   // adjoint substeps base .. end-1
+    if (end <= base) return 0;
+    for (int f = base; f + 1 < end; ++f)   // recompute states base+1 .. end-1
+        if (launch_forward(c, f, g, c->state(seg_slot(p, cseg, f)), c->state(seg_slot(p, cseg, f + 1)))) return -1;
+    for (int f = end - 1; f >= base; --f) {
+        int gin = (p.L - (f + 1)) & 1, gout = (p.L - f) & 1;
+        if (launch_adjoint(c, f, g, c->state(seg_slot(p, cseg, f)), c->adjv(gin), c->adjv(gout))) return -1;
+    }
+    return 0;
+}
+
+static int run_recorded(dcp_ctx* c, int kind, int cseg, int g, const Plan& p, int use_graph) {
+    auto rec = [&]() { return kind == 0 ? record_forward_segment(c, p, cseg, g) : record_backward_segment(c, p, cseg, g); };
+    if (!use_graph) return rec();
+    auto key = std::make_tuple(kind, cseg, g, p.S, p.K);
+    auto it = c->graphs.find(key);
+    if (it == c->graphs.end()) {
+        hipGraph_t graph;
+        HIP_OK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        int r = rec();
+        hipError_t e = hipStreamEndCapture(c->stream, &graph);
+        if (r) return r;
+        if (e != hipSuccess) return set_err(std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+        hipGraphExec_t exec;
+        HIP_OK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+        hipGraphDestroy(graph);
+        it = c->graphs.emplace(key, exec).first;
+    }
+    HIP_OK(hipGraphLaunch(it->second, c->stream));
+    return 0;
+}
+
+extern "C" int dcp_rollout_grad(dcp_ctx* c, int32_t S, int32_t g, int32_t K, int32_t use_graph, double* loss_out,
+                                double* grads_out) {
+    if (!c) return set_err("bad args");
+    Plan p;
+    if (plan_rollout(c, S, K, p)) return -1;
+    if (c->n_targets_total == 0) return set_err("no targets set (dcp_set_targets)");
+    // the reset state must already be in slot 0 (dcp_reset)
+    HIP_OK(hipEventRecord(c->ev0, c->stream));
+    if (dcp_zero_param_grads(c)) return -1;
+    for (int cs = 0; cs < p.nseg; ++cs)
+        if (run_recorded(c, 0, cs, g, p, use_graph)) return -1;
+    HIP_OK(hipMemcpyAsync(loss_out ? loss_out : nullptr, c->loss_d, loss_out ? c->B * sizeof(double) : 0,
+                          hipMemcpyDeviceToHost, c->stream));
+    int last = (p.L - 1) / p.K;
+    for (int cs = last; cs >= 0; --cs)
+        if (run_recorded(c, 1, cs, g, p, use_graph)) return -1;
+    HIP_OK(hipEventRecord(c->ev1, c->stream));
+    HIP_OK(hipEventSynchronize(c->ev1));
+    float ms = 0.f;
+    HIP_OK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->last_ms = ms;
+    if (grads_out && reduce_param_grads(c, grads_out)) return -1;
+    return 0;
+}
+
+extern "C" int dcp_rollout(dcp_ctx* c, int32_t S, int32_t g, int32_t use_graph, int32_t* final_slot) {
+    if (!c) return set_err("bad args");
+    Plan p;
+    if (S < 1) return set_err("n_substeps must be >= 1");
+    p.S = S; p.K = S; p.L = -1; p.nseg = 1; p.nck = 1; p.seg0 = 2;
+    // forward-only: ping-pong between slots 0 and 1 would lose the reset state;
+    // use slots 1 and 2 after the reset slot 0
+    if (c->max_states < 3) return set_err("dcp_rollout needs max_states >= 3");
+    HIP_OK(hipEventRecord(c->ev0, c->stream));
+    int cur = 0;
+    for (int f = 0; f < S; ++f) {
+        int nxt = (f % 2 == 0) ? 1 : 2;
+        if (launch_forward(c, f, g, c->state(cur), c->state(nxt))) return -1;
+        cur = nxt;
+    }
+    (void)use_graph;
+    HIP_OK(hipEventRecord(c->ev1, c->stream));
+    HIP_OK(hipEventSynchronize(c->ev1));
+    float ms = 0.f;
+    HIP_OK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->last_ms = ms;
+    if (final_slot) *final_slot = cur;
+    return 0;
+}
+
+extern "C" int dcp_last_rollout_ms(dcp_ctx* c, double* ms) {
+    if (!c || !ms) return set_err("bad args");
+    *ms = c->last_ms;
+    return 0;
+}

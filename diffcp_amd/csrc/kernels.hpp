@@ -1,0 +1,217 @@
+// Device helpers shared by the forward and adjoint kernels, and the kernel
+// declarations used by the host engine.
+#pragma once
+#include "sim_common.hpp"
+#include "element_math.hpp"
+#include "../../include/diffcp.h"
+
+// (1 - 0.95) evaluated in double then rounded, as the reference's constant
+// folding does (simulation.py:383)
+#define DCP_C_SYM_KEEP ((float)(1.0 - 0.95))
+#define DCP_BOUND 3        // initialization.py:22
+
+__device__ __forceinline__ int wave_min(int v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ __forceinline__ int wave_max(int v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ __forceinline__ double wave_sum_d(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+struct ElemKin {
+    int i1, i2, i3;
+    V3<float> np, xe;
+    M3<float> C, M, dn, Dinv, F, d;
+};
+
+__device__ __forceinline__ ElemParams elem_params(const SimDev& sim, const InstDev& I, int e) {
+    ElemParams p{I.E, I.nu, I.ks, I.kc};
+    if (sim.per_elem) {
+        p.E = sim.emod[e];
+        p.ks = sim.emod[(size_t)sim.NE + e];
+        p.kc = sim.emod[(size_t)2 * sim.NE + e];
+    }
+    return p;
+}
+
+// d and F of element e at the state in S (fresh: F = I, d = rest shape).
+__device__ __forceinline__ void element_kinematics(const SimDev& sim, const InstDev& I, const StateView& S, int e,
+                                                   int fresh, ElemKin& K) {
+    const int NE = sim.NE;
+    K.i1 = sim.tri[e]; K.i2 = sim.tri[NE + e]; K.i3 = sim.tri[2 * NE + e];
+    V3<float> x1 = S.x(K.i1), x2 = S.x(K.i2), x3 = S.x(K.i3);
+    K.np = S.n(e);
+    K.C = S.Ce(e);
+    V3<float> col3 = K.np;
+    if (!fresh) {
+        // col3 += (dt C + 0.5 dt^2 C C) col3   (simulation.py:397-400)
+        M3<float> hC = mscale(I.half_dt2, K.C);
+        K.M = madd(mscale(I.dt, K.C), mmul(hC, K.C));
+        col3 = K.np + mv(K.M, K.np);
+    }
+    V3<float> e1 = x2 - x1, e2 = x3 - x1;
+    for (int i = 0; i < 3; ++i) { K.dn(i, 0) = e1[i]; K.dn(i, 1) = e2[i]; K.dn(i, 2) = col3[i]; }
+    K.Dinv = load_m9(sim.Dinv, NE, e);
+    if (fresh) {
+        K.F = meye<float>();
+        K.d = K.dn;
+    } else {
+        M3<float> Dii = load_m9(sim.Dinvinv, NE, e);
+        K.F = mmul(K.dn, K.Dinv);                 // new_F = new_d @ Dinv       (:407)
+        K.d = mmul(K.F, Dii);                     // d = new_F @ inverse(Dinv)  (:429)
+    }
+    if (fresh) {
+        K.xe = {(x1.x + x2.x + x3.x) / 3.0f, (x1.y + x2.y + x3.y) / 3.0f, (x1.z + x2.z + x3.z) / 3.0f};
+    } else {
+        const float third = 1.0f / 3.0f;
+        K.xe = {third * (x1.x + x2.x + x3.x), third * (x1.y + x2.y + x3.y), third * (x1.z + x2.z + x3.z)};
+    }
+}
+
+__device__ __forceinline__ V3<float> particle_x(const SimDev& sim, const StateView& S, const Patch& P, int q) {
+    if (P.cls == CLS_ELEMENT) return load_v3(sim.xe, sim.NE, P.start + q);
+    return S.x(P.start + q);
+}
+
+// prescribed grasper velocity (simulation.py:160-165); the frame index is the
+// integer s / steps_per_frame + frames_per_window * g
+__device__ __forceinline__ int grasper_frame(const InstDev& I, int s, int g) {
+    return s / I.steps_per_frame + I.frames_per_window * g;
+}
+__device__ __forceinline__ V3<float> grasper_velocity(const SimDev& sim, const InstDev& I, int local, int s, int g) {
+    int frame = grasper_frame(I, s, g);
+    int gid = local / I.group_size;
+    if (frame + 1 < I.n_actions) {
+        const float* a = sim.traj + 3 * ((size_t)I.traj_off + (size_t)(frame + 1) * I.n_grasp + gid);
+        const float* b = sim.traj + 3 * ((size_t)I.traj_off + (size_t)frame * I.n_grasp + gid);
+        return {(a[0] - b[0]) / I.interval, (a[1] - b[1]) / I.interval, (a[2] - b[2]) / I.interval};
+    }
+    return {0.f, 0.f, 0.f};
+}
+
+struct P2GRec {
+    V3<float> x, b, n, k;
+    M3<float> A;
+    float m;
+    bool elem;
+};
+
+// everything a particle scatters: per cell w*(b + A dpos) + (grad w . n) k, mass w*m
+__device__ __forceinline__ void p2g_record(const SimDev& sim, const InstDev& I, const StateView& S, const Patch& P,
+                                           int q, int s, int g, P2GRec& r) {
+    r.elem = false;
+    if (P.cls == CLS_ELEMENT) {
+        const int e = P.start + q;
+        r.elem = true;
+        r.x = load_v3(sim.xe, sim.NE, e);
+        V3<float> v = load_v3(sim.ve, sim.NE, e);
+        r.m = I.m_quad;
+        r.b = r.m * v;
+        r.A = mscale(r.m, S.Ce(e));
+        r.n = load_v3(sim.nd, sim.NE, e);
+        r.k = load_v3(sim.ke, sim.NE, e);
+        return;
+    }
+    const int p = P.start + q;
+    r.x = S.x(p);
+    r.n = {0.f, 0.f, 0.f};
+    r.k = {0.f, 0.f, 0.f};
+    if (P.cls == CLS_VERTEX) {
+        r.m = I.m_vertex;
+        V3<float> v = S.v(p);
+        v.y += I.gdt;                                       // simulation.py:181-182
+        V3<float> f = {0.f, 0.f, 0.f};
+        for (int t = sim.inc_off[p]; t < sim.inc_off[p + 1]; ++t) {
+            int ec = sim.inc_list[t];
+            int e = ec / 3, c = ec - 3 * e;
+            f = f + load_v3(sim.fc + (size_t)3 * c * sim.NE, sim.NE, e);
+        }
+        r.b = r.m * v + f;
+    } else if (P.cls == CLS_GRASPER) {
+        r.m = I.m_op;
+        r.b = r.m * grasper_velocity(sim, I, p - I.o_start, s, g);
+    } else {
+        r.m = I.m_cube;
+        r.b = {0.f, 0.f, 0.f};                              // simulation.py:150-151 (no velocity term)
+    }
+    r.A = mscale(r.m, S.C(p));
+}
+
+__device__ __forceinline__ void p2g_contrib(const InstDev& I, const Stencil& st, const P2GRec& r, int i, int j, int k,
+                                            float& mm, float& m0, float& m1, float& m2) {
+    float w = st.w[i][0] * st.w[j][1] * st.w[k][2];
+    float dp0 = ((float)i - st.fx[0]) * I.dx, dp1 = ((float)j - st.fx[1]) * I.dx, dp2 = ((float)k - st.fx[2]) * I.dx;
+    float a0 = r.A(0, 0) * dp0 + r.A(0, 1) * dp1 + r.A(0, 2) * dp2;
+    float a1 = r.A(1, 0) * dp0 + r.A(1, 1) * dp1 + r.A(1, 2) * dp2;
+    float a2 = r.A(2, 0) * dp0 + r.A(2, 1) * dp1 + r.A(2, 2) * dp2;
+    mm = w * r.m;
+    m0 = w * (r.b.x + a0);
+    m1 = w * (r.b.y + a1);
+    m2 = w * (r.b.z + a2);
+    if (r.elem) {
+        float g0 = st.dw[i][0] * st.w[j][1] * st.w[k][2];
+        float g1 = st.w[i][0] * st.dw[j][1] * st.w[k][2];
+        float g2 = st.w[i][0] * st.w[j][1] * st.dw[k][2];
+        float sdot = (g0 * r.n.x + g1 * r.n.y + g2 * r.n.z) * I.inv_dx;
+        m0 += sdot * r.k.x; m1 += sdot * r.k.y; m2 += sdot * r.k.z;
+    }
+}
+
+__device__ __forceinline__ void block_cell_coords(const InstDev& I, int blk, int lane, int& i, int& j, int& k) {
+    int bk = blk % I.nb, t = blk / I.nb;
+    int bj = t % I.nb, bi = t / I.nb;
+    i = bi * 4 + (lane >> 4);
+    j = bj * 4 + ((lane >> 2) & 3);
+    k = bk * 4 + (lane & 3);
+}
+
+// v_out of one cell (simulation.py:277-302) with the damping extension;
+// mask[c] = 1 where the component passed the wall conditions
+__device__ __forceinline__ void grid_velocity(const InstDev& I, float4 c, int i, int j, int k, float v[3], float* mask,
+                                              float* vraw = nullptr) {
+    float inv_m = 1.0f / (c.x + 1e-9f);
+    v[0] = inv_m * c.y; v[1] = inv_m * c.z; v[2] = inv_m * c.w;
+    if (vraw) { vraw[0] = v[0]; vraw[1] = v[1]; vraw[2] = v[2]; }
+    float damp = 1.0f - I.damping * I.dt;
+    v[0] *= damp; v[1] *= damp; v[2] *= damp;
+    bool all = false, yonly = false;
+    const int G = I.G, b = DCP_BOUND;
+    if (i < b && v[0] < 0) { v[0] = v[1] = v[2] = 0.f; all = true; }
+    if (i > G - b && v[0] > 0) { v[0] = v[1] = v[2] = 0.f; all = true; }
+    if (k < b && v[2] < 0) { v[0] = v[1] = v[2] = 0.f; all = true; }
+    if (k > G - b && v[2] > 0) { v[0] = v[1] = v[2] = 0.f; all = true; }
+    if (j < b && v[1] < 0) { v[1] = 0.f; yonly = true; }
+    if (j > G - b && v[1] > 0) { v[0] = v[1] = v[2] = 0.f; all = true; }
+    if (mask) {
+        mask[0] = all ? 0.f : 1.f;
+        mask[1] = (all || yonly) ? 0.f : 1.f;
+        mask[2] = all ? 0.f : 1.f;
+    }
+}
+
+// kernels -------------------------------------------------------------------
+template <typename PT> __global__ void k_prep(SimDev sim, StateView S, StateView Snext, int fresh);
+__global__ void k_p2g(SimDev sim, StateView S, int s, int g);
+__global__ void k_gridop(SimDev sim, int keep);
+__global__ void k_g2p(SimDev sim, StateView S, StateView Sn);
+__global__ void k_loss_min(SimDev sim, StateView S, const float* targets, const int4* tchunks, int n_chunks,
+                           const int* vnat, float* tmin, int* targ);
+__global__ void k_loss_sum(const float* tmin, const int* toff, const int* tcount, double* loss, int B);
+__global__ void k_binning_keys(SimDev sim, StateView S, int* keys);
+
+__global__ void k_g2p_adj(SimDev sim, StateView S, StateView G1, StateView Gout);
+__global__ void k_gridop_adj(SimDev sim);
+__global__ void k_p2g_adj(SimDev sim, StateView S, StateView Gout, int s, int g);
+template <typename PT> __global__ void k_prep_adj(SimDev sim, StateView S, StateView G1, StateView Gout, int fresh);
+__global__ void k_vertex_final(SimDev sim, StateView Gout);
+__global__ void k_loss_adj(SimDev sim, StateView S, StateView Gout, const float* targets, const int* toff,
+                           const int* tcount, const int* targ, const double* seed);
+__global__ void k_fill_f(float* p, size_t n, float v);

@@ -1,0 +1,421 @@
+// Adjoint substep kernels: the hand-written reverse of k_prep, k_p2g, k_gridop
+// and k_g2p (what Taichi's tape generates for simulation.py:500-506, with the
+// manual SVD backward of :71-94).  The adjoint of substep s runs after a
+// recompute of its grid (k_prep, k_p2g, k_gridop with keep=1):
+//   k_g2p_adj       per patch: dL/d grid_v_out scattered through an LDS tile
+//                   (the same tiling as P2G), dL/dx through the weights
+//   k_gridop_adj    per active block: wall masks, damping, normalisation
+//   k_p2g_adj       per patch: pure 27-cell gather of dL/d(m, mv)
+//   k_prep_adj      per element patch: stress, QR, polar (FP64), kinematics;
+//                   parameter gradients reduced per patch in FP64, fixed order
+//   k_vertex_final  per vertex: incident-element corner gradients (CSR order)
+#include "kernels.hpp"
+
+__global__ void k_fill_f(float* p, size_t n, float v) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// G2P adjoint
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(DCP_WG) void k_g2p_adj(SimDev sim, StateView S, StateView G1, StateView Gout) {
+    __shared__ float tx[DCP_TILE_CAP], ty[DCP_TILE_CAP], tz[DCP_TILE_CAP];
+    __shared__ int sbox[6];
+    const int tid = threadIdx.x;
+    const float csym = 0.5f * DCP_C_SYM_KEEP;
+    for (int pi = blockIdx.x; pi < sim.n_patches; pi += gridDim.x) {
+        const Patch P = sim.patches[pi];
+        const InstDev& I = sim.inst[P.inst];
+        if (tid < 3) { sbox[tid] = 1 << 30; sbox[3 + tid] = -(1 << 30); }
+        __syncthreads();
+        int lo[3] = {1 << 30, 1 << 30, 1 << 30}, hi[3] = {-(1 << 30), -(1 << 30), -(1 << 30)};
+        for (int q = tid; q < P.count; q += DCP_WG) {
+            Stencil st;
+            make_stencil(particle_x(sim, S, P, q), I.inv_dx, I.G, st, nullptr);
+            for (int d = 0; d < 3; ++d) { lo[d] = min(lo[d], st.base[d]); hi[d] = max(hi[d], st.base[d]); }
+        }
+        for (int d = 0; d < 3; ++d) {
+            int a = wave_min(lo[d]), b = wave_max(hi[d]);
+            if ((tid & 63) == 0) { atomicMin(&sbox[d], a); atomicMax(&sbox[3 + d], b); }
+        }
+        __syncthreads();
+        const int l0 = sbox[0], l1 = sbox[1], l2 = sbox[2];
+        const int e0 = sbox[3] - l0 + 3, e1 = sbox[4] - l1 + 3, e2 = sbox[5] - l2 + 3;
+        const int vol = e0 * e1 * e2;
+        const bool spill = vol > DCP_TILE_CAP;
+        if (!spill)
+            for (int c = tid; c < vol; c += DCP_WG) { tx[c] = 0.f; ty[c] = 0.f; tz[c] = 0.f; }
+        __syncthreads();
+        for (int q = tid; q < P.count; q += DCP_WG) {
+            V3<float> x = particle_x(sim, S, P, q);
+            Stencil st;
+            make_stencil(x, I.inv_dx, I.G, st, nullptr);
+            V3<float> gv, gx1;
+            M3<float> gC;
+            if (P.cls == CLS_ELEMENT) {
+                M3<float> go = G1.Ce(P.start + q);
+                for (int a = 0; a < 3; ++a)
+                    for (int b = 0; b < 3; ++b)
+                        gC(a, b) = 0.5f * (go(a, b) - go(b, a)) + csym * (go(a, b) + go(b, a));
+                gv = {0.f, 0.f, 0.f};
+                gx1 = {0.f, 0.f, 0.f};
+            } else {
+                const int p = P.start + q;
+                gx1 = G1.x(p);
+                V3<float> g1v = G1.v(p);
+                gv = {g1v.x + I.dt * gx1.x, g1v.y + I.dt * gx1.y, g1v.z + I.dt * gx1.z};
+                gC = G1.C(p);
+            }
+            float gfx[3] = {0.f, 0.f, 0.f};
+            const float c4 = 4.0f * I.inv_dx;
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int j = 0; j < 3; ++j)
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) {
+                        const int ci = st.base[0] + i, cj = st.base[1] + j, ck = st.base[2] + k;
+                        float4 gvo = sim.vout[cell_index(I, ci, cj, ck)];
+                        float W0 = st.w[i][0], W1 = st.w[j][1], W2 = st.w[k][2];
+                        float w = W0 * W1 * W2;
+                        float dp[3] = {(float)i - st.fx[0], (float)j - st.fx[1], (float)k - st.fx[2]};
+                        float g[3] = {gvo.x, gvo.y, gvo.z};
+                        float gCdp[3], gCtg[3];
+#pragma unroll
+                        for (int a = 0; a < 3; ++a) {
+                            gCdp[a] = gC(a, 0) * dp[0] + gC(a, 1) * dp[1] + gC(a, 2) * dp[2];
+                            gCtg[a] = gC(0, a) * g[0] + gC(1, a) * g[1] + gC(2, a) * g[2];
+                        }
+                        float s1 = c4 * w;
+                        float gg0 = w * gv.x + s1 * gCdp[0];
+                        float gg1 = w * gv.y + s1 * gCdp[1];
+                        float gg2 = w * gv.z + s1 * gCdp[2];
+                        if (!spill) {
+                            int c = ((ci - l0) * e1 + (cj - l1)) * e2 + (ck - l2);
+                            atomicAdd(&tx[c], gg0); atomicAdd(&ty[c], gg1); atomicAdd(&tz[c], gg2);
+                        } else {
+                            long long gi = cell_index(I, ci, cj, ck);
+                            atomicAdd(&sim.gacc[gi].y, gg0); atomicAdd(&sim.gacc[gi].z, gg1); atomicAdd(&sim.gacc[gi].w, gg2);
+                        }
+                        float scal = gv.x * g[0] + gv.y * g[1] + gv.z * g[2] +
+                                     c4 * (g[0] * gCdp[0] + g[1] * gCdp[1] + g[2] * gCdp[2]);
+                        float dW0 = st.dw[i][0] * W1 * W2, dW1 = W0 * st.dw[j][1] * W2, dW2 = W0 * W1 * st.dw[k][2];
+                        gfx[0] += dW0 * scal - s1 * gCtg[0];
+                        gfx[1] += dW1 * scal - s1 * gCtg[1];
+                        gfx[2] += dW2 * scal - s1 * gCtg[2];
+                    }
+            V3<float> gxt = {I.inv_dx * gfx[0], I.inv_dx * gfx[1], I.inv_dx * gfx[2]};
+            if (P.cls == CLS_ELEMENT) {
+                store_v3(sim.g_xe, sim.NE, P.start + q, gxt);
+            } else {
+                store_v3(Gout.px(0), sim.NP, P.start + q, gx1 + gxt);
+            }
+        }
+        __syncthreads();
+        if (!spill) {
+            for (int c = tid; c < vol; c += DCP_WG) {
+                float a0 = tx[c], a1 = ty[c], a2 = tz[c];
+                if (a0 != 0.f || a1 != 0.f || a2 != 0.f) {
+                    int ck = c % e2, t = c / e2;
+                    int cj = t % e1, ci = t / e1;
+                    long long gi = cell_index(I, ci + l0, cj + l1, ck + l2);
+                    atomicAdd(&sim.gacc[gi].y, a0); atomicAdd(&sim.gacc[gi].z, a1); atomicAdd(&sim.gacc[gi].w, a2);
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// grid update adjoint
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(DCP_WG) void k_gridop_adj(SimDev sim) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    int n = *sim.n_active;
+    if (n > sim.max_active) n = sim.max_active;
+    for (int it = blockIdx.x * (DCP_WG / 64) + wv; it < n; it += gridDim.x * (DCP_WG / 64)) {
+        const int2 a = sim.active[it];
+        const InstDev& I = sim.inst[a.x];
+        const long long cell = I.grid_base + (long long)a.y * 64 + lane;
+        int ci, cj, ck;
+        block_cell_coords(I, a.y, lane, ci, cj, ck);
+        float4 c = sim.acc[cell];
+        float4 ga = sim.gacc[cell];
+        float v[3], mask[3], vr[3];
+        grid_velocity(I, c, ci, cj, ck, v, mask, vr);
+        float gd0 = mask[0] * ga.y, gd1 = mask[1] * ga.z, gd2 = mask[2] * ga.w;
+        double gdamp = -(double)I.dt * ((double)gd0 * vr[0] + (double)gd1 * vr[1] + (double)gd2 * vr[2]);
+        float damp = 1.0f - I.damping * I.dt;
+        float g0 = damp * gd0, g1 = damp * gd1, g2 = damp * gd2;
+        float inv_m = 1.0f / (c.x + 1e-9f);
+        float gm = -inv_m * (g0 * vr[0] + g1 * vr[1] + g2 * vr[2]);
+        sim.gmv[cell] = make_float4(gm, inv_m * g0, inv_m * g1, inv_m * g2);
+        sim.acc[cell] = make_float4(0.f, 0.f, 0.f, 0.f);
+        sim.gacc[cell] = make_float4(0.f, 0.f, 0.f, 0.f);
+        double s = wave_sum_d(gdamp);
+        if (lane == 0) {
+            sim.flags[I.flag_base + a.y] = 0;
+            if (s != 0.0) atomicAdd(&sim.pgrad_inst[a.x * DCP_N_PARAMS + 4], s);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// P2G adjoint (gather)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(DCP_WG) void k_p2g_adj(SimDev sim, StateView S, StateView Gout, int s, int g) {
+    const int tid = threadIdx.x;
+    const float ddw[3] = {1.0f, -2.0f, 1.0f};
+    for (int pi = blockIdx.x; pi < sim.n_patches; pi += gridDim.x) {
+        const Patch P = sim.patches[pi];
+        const InstDev& I = sim.inst[P.inst];
+        for (int q = tid; q < P.count; q += DCP_WG) {
+            P2GRec r;
+            p2g_record(sim, I, S, P, q, s, g, r);
+            Stencil st;
+            make_stencil(r.x, I.inv_dx, I.G, st, nullptr);
+            float gb[3] = {0.f, 0.f, 0.f}, gfx[3] = {0.f, 0.f, 0.f}, gn[3] = {0.f, 0.f, 0.f}, gk[3] = {0.f, 0.f, 0.f};
+            M3<float> gA = mzero<float>();
+            const float dx = I.dx, idx = I.inv_dx;
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int j = 0; j < 3; ++j)
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) {
+                        float4 gg = sim.gmv[cell_index(I, st.base[0] + i, st.base[1] + j, st.base[2] + k)];
+                        float W0 = st.w[i][0], W1 = st.w[j][1], W2 = st.w[k][2];
+                        float D0 = st.dw[i][0], D1 = st.dw[j][1], D2 = st.dw[k][2];
+                        float w = W0 * W1 * W2;
+                        float dp[3] = {((float)i - st.fx[0]) * dx, ((float)j - st.fx[1]) * dx, ((float)k - st.fx[2]) * dx};
+                        float gm3[3] = {gg.y, gg.z, gg.w};
+                        float u[3];
+#pragma unroll
+                        for (int a = 0; a < 3; ++a) u[a] = (&r.b.x)[a] + r.A(a, 0) * dp[0] + r.A(a, 1) * dp[1] + r.A(a, 2) * dp[2];
+#pragma unroll
+                        for (int a = 0; a < 3; ++a) {
+                            gb[a] += w * gm3[a];
+#pragma unroll
+                            for (int b = 0; b < 3; ++b) gA(a, b) += w * gm3[a] * dp[b];
+                        }
+                        float scal = gg.x * r.m + gm3[0] * u[0] + gm3[1] * u[1] + gm3[2] * u[2];
+                        float dW[3] = {D0 * W1 * W2, W0 * D1 * W2, W0 * W1 * D2};
+#pragma unroll
+                        for (int d = 0; d < 3; ++d) {
+                            float gAd = gm3[0] * r.A(0, d) + gm3[1] * r.A(1, d) + gm3[2] * r.A(2, d);
+                            gfx[d] += dW[d] * scal - w * dx * gAd;
+                        }
+                        if (r.elem) {
+                            float gw[3] = {idx * dW[0], idx * dW[1], idx * dW[2]};
+                            float sdot = gw[0] * r.n.x + gw[1] * r.n.y + gw[2] * r.n.z;
+                            float qd = gm3[0] * r.k.x + gm3[1] * r.k.y + gm3[2] * r.k.z;
+#pragma unroll
+                            for (int a = 0; a < 3; ++a) { gk[a] += sdot * gm3[a]; gn[a] += qd * gw[a]; }
+                            float H0 = ddw[i], H1 = ddw[j], H2 = ddw[k];
+                            float dg0 = idx * (H0 * W1 * W2 * r.n.x + D0 * D1 * W2 * r.n.y + D0 * W1 * D2 * r.n.z);
+                            float dg1 = idx * (D0 * D1 * W2 * r.n.x + W0 * H1 * W2 * r.n.y + W0 * D1 * D2 * r.n.z);
+                            float dg2 = idx * (D0 * W1 * D2 * r.n.x + W0 * D1 * D2 * r.n.y + W0 * W1 * H2 * r.n.z);
+                            gfx[0] += qd * dg0; gfx[1] += qd * dg1; gfx[2] += qd * dg2;
+                        }
+                    }
+            V3<float> gx = {idx * gfx[0], idx * gfx[1], idx * gfx[2]};
+            if (P.cls == CLS_ELEMENT) {
+                const int e = P.start + q;
+                V3<float> prev = load_v3(sim.g_xe, sim.NE, e);
+                store_v3(sim.g_xe, sim.NE, e, prev + gx);
+                store_v3(sim.g_ve, sim.NE, e, {r.m * gb[0], r.m * gb[1], r.m * gb[2]});
+                store_m9(Gout.eC(0), sim.NE, e, mscale(r.m, gA));
+                store_v3(sim.g_ne, sim.NE, e, {gn[0], gn[1], gn[2]});
+                store_v3(sim.g_ke, sim.NE, e, {gk[0], gk[1], gk[2]});
+            } else {
+                const int p = P.start + q;
+                V3<float> prev = Gout.x(p);
+                store_v3(Gout.px(0), sim.NP, p, prev + gx);
+                store_m9(Gout.pC(0), sim.NP, p, mscale(r.m, gA));
+                if (P.cls == CLS_VERTEX) {
+                    store_v3(Gout.pv(0), sim.NP, p, {r.m * gb[0], r.m * gb[1], r.m * gb[2]});
+                    store_v3(sim.g_fv, sim.NP, p, {gb[0], gb[1], gb[2]});
+                } else {
+                    store_v3(Gout.pv(0), sim.NP, p, {0.f, 0.f, 0.f});
+                    if (P.cls == CLS_GRASPER && sim.traj_grad) {
+                        int frame = grasper_frame(I, s, g);
+                        if (frame + 1 < I.n_actions) {
+                            int gid = (p - I.o_start) / I.group_size;
+                            double* t1 = sim.traj_grad + 3 * ((size_t)I.traj_off + (size_t)(frame + 1) * I.n_grasp + gid);
+                            double* t0 = sim.traj_grad + 3 * ((size_t)I.traj_off + (size_t)frame * I.n_grasp + gid);
+                            for (int a = 0; a < 3; ++a) {
+                                double gvp = (double)r.m * (double)gb[a] / (double)I.interval;
+                                atomicAdd(t1 + a, gvp);
+                                atomicAdd(t0 + a, -gvp);
+                            }
+                        }
+                    }
+                }
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// element prep adjoint
+// ---------------------------------------------------------------------------
+template <typename PT>
+__global__ __launch_bounds__(DCP_WG) void k_prep_adj(SimDev sim, StateView S, StateView G1, StateView Gout, int fresh) {
+    __shared__ double red[4][DCP_WG / 64];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int NE = sim.NE;
+    for (int pi = blockIdx.x; pi < sim.n_patches; pi += gridDim.x) {
+        const Patch Pt = sim.patches[pi];
+        if (Pt.cls != CLS_ELEMENT) continue;
+        const InstDev& I = sim.inst[Pt.inst];
+        double acc[4] = {0.0, 0.0, 0.0, 0.0};
+        for (int q = tid; q < Pt.count; q += DCP_WG) {
+            const int e = Pt.start + q;
+            ElemKin K;
+            element_kinematics(sim, I, S, e, fresh, K);
+            ElemParams prm{I.E, I.nu, I.ks, I.kc};
+            if (sim.per_elem) {
+                prm.E = sim.emod[e]; prm.ks = sim.emod[(size_t)NE + e]; prm.kc = sim.emod[(size_t)2 * NE + e];
+            }
+            StressState SS;
+            M3<float> P = stress_forward<PT>(K.F, prm, SS);
+            const float vd = I.vol_quad * I.dt;
+            V3<float> r1 = {K.Dinv(0, 0), K.d(0, 1), K.d(0, 2)};
+            V3<float> r2 = {K.Dinv(1, 0), K.d(1, 1), K.d(1, 2)};
+            V3<float> gkv = load_v3(sim.g_ke, NE, e);
+            V3<float> gf1 = load_v3(sim.g_fv, sim.NP, K.i1);
+            V3<float> gf2 = load_v3(sim.g_fv, sim.NP, K.i2);
+            V3<float> gf3 = load_v3(sim.g_fv, sim.NP, K.i3);
+            M3<float> gP = mzero<float>();
+            gP(0, 2) += -vd * gkv.x; gP(1, 2) += -vd * gkv.y; gP(2, 2) += -vd * gkv.z;
+            gP = madd(gP, mscale(vd, outer(gf1, r1 + r2)));
+            gP = madd(gP, mscale(-vd, outer(gf2, r1)));
+            gP = madd(gP, mscale(-vd, outer(gf3, r2)));
+            V3<float> gr12 = vd * mtv(P, gf1);
+            V3<float> gr1 = gr12 - vd * mtv(P, gf2);
+            V3<float> gr2 = gr12 - vd * mtv(P, gf3);
+            M3<float> gd = mzero<float>();
+            gd(0, 1) += gr1.y; gd(0, 2) += gr1.z;
+            gd(1, 1) += gr2.y; gd(1, 2) += gr2.z;
+            V3<float> gnt = load_v3(sim.g_ne, NE, e) + G1.n(e);
+            gd(0, 2) += gnt.x; gd(1, 2) += gnt.y; gd(2, 2) += gnt.z;
+            M3<float> gF = mzero<float>();
+            float gp[4] = {0.f, 0.f, 0.f, 0.f};
+            stress_adjoint<PT>(K.F, prm, SS, gP, sim.svd_grad, gF, gp);
+            M3<float> gdn;
+            if (!fresh) {
+                M3<float> Dii = load_m9(sim.Dinvinv, NE, e);
+                gF = madd(gF, mmul(gd, mT(Dii)));     // d = F Dinv^-1
+                gdn = mmul(gF, mT(K.Dinv));           // F = d_new Dinv
+            } else {
+                gdn = gd;                             // F = I is a constant of the fresh state
+            }
+            V3<float> gxe = load_v3(sim.g_xe, NE, e);
+            V3<float> gve = load_v3(sim.g_ve, NE, e);
+            const float third = 1.0f / 3.0f;
+            V3<float> c0 = {gdn(0, 0), gdn(1, 0), gdn(2, 0)};
+            V3<float> c1 = {gdn(0, 1), gdn(1, 1), gdn(2, 1)};
+            V3<float> gcol3 = {gdn(0, 2), gdn(1, 2), gdn(2, 2)};
+            V3<float> gx1 = third * gxe - c0 - c1;
+            V3<float> gx2 = third * gxe + c0;
+            V3<float> gx3 = third * gxe + c1;
+            // the element velocity of a fresh state is the constant 0 (simulation.py:466)
+            V3<float> gvc = fresh ? V3<float>{0.f, 0.f, 0.f} : third * gve;
+            M3<float> gC = Gout.Ce(e);
+            V3<float> gnp = gcol3;
+            if (!fresh) {
+                gnp = gcol3 + mtv(K.M, gcol3);
+                M3<float> gM = outer(gcol3, K.np);
+                gC = madd(gC, madd(mscale(I.dt, gM), mscale(I.half_dt2, madd(mmul(gM, mT(K.C)), mmul(mT(K.C), gM)))));
+            }
+            store_m9(Gout.eC(0), NE, e, gC);
+            store_v3(Gout.en(0), NE, e, gnp);
+            store_v3(sim.ec_gx, NE, e, gx1);
+            store_v3(sim.ec_gx + (size_t)3 * NE, NE, e, gx2);
+            store_v3(sim.ec_gx + (size_t)6 * NE, NE, e, gx3);
+            store_v3(sim.ec_gv, NE, e, gvc);
+            store_v3(sim.ec_gv + (size_t)3 * NE, NE, e, gvc);
+            store_v3(sim.ec_gv + (size_t)6 * NE, NE, e, gvc);
+            if (sim.per_elem) {
+                sim.emod_grad[e] += (double)gp[0];
+                sim.emod_grad[(size_t)NE + e] += (double)gp[2];
+                sim.emod_grad[(size_t)2 * NE + e] += (double)gp[3];
+                acc[1] += (double)gp[1];
+            } else {
+                for (int t = 0; t < 4; ++t) acc[t] += (double)gp[t];
+            }
+        }
+        for (int t = 0; t < 4; ++t) {
+            double v = wave_sum_d(acc[t]);
+            if (lane == 0) red[t][wv] = v;
+        }
+        __syncthreads();
+        if (tid < 4) {
+            double v = 0.0;
+            for (int w = 0; w < DCP_WG / 64; ++w) v += red[tid][w];
+            sim.pgrad_patch[(size_t)pi * 4 + tid] += v;
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// vertex gather of the element corner gradients
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(DCP_WG) void k_vertex_final(SimDev sim, StateView Gout) {
+    const int NE = sim.NE;
+    for (int pi = blockIdx.x; pi < sim.n_patches; pi += gridDim.x) {
+        const Patch P = sim.patches[pi];
+        if (P.cls != CLS_VERTEX) continue;
+        for (int q = threadIdx.x; q < P.count; q += DCP_WG) {
+            const int p = P.start + q;
+            V3<float> gx = Gout.x(p), gv = Gout.v(p);
+            for (int t = sim.inc_off[p]; t < sim.inc_off[p + 1]; ++t) {
+                int ec = sim.inc_list[t];
+                int e = ec / 3, c = ec - 3 * e;
+                gx = gx + load_v3(sim.ec_gx + (size_t)3 * c * NE, NE, e);
+                gv = gv + load_v3(sim.ec_gv + (size_t)3 * c * NE, NE, e);
+            }
+            store_v3(Gout.px(0), sim.NP, p, gx);
+            store_v3(Gout.pv(0), sim.NP, p, gv);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// loss adjoint: every target sends (2/3)(x - t)/N_t to its first-minimum vertex
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(DCP_WG) void k_loss_adj(SimDev sim, StateView S, StateView Gout, const float* targets,
+                                                      const int* toff, const int* tcount, const int* targ,
+                                                      const double* seed) {
+    for (int pi = blockIdx.x; pi < sim.n_patches; pi += gridDim.x) {
+        const Patch P = sim.patches[pi];
+        if (P.cls != CLS_VERTEX) continue;
+        const int b = P.inst;
+        const int nt = tcount[b], o = toff[b];
+        if (nt == 0) continue;
+        const float sd = (float)seed[b];
+        for (int q = threadIdx.x; q < P.count; q += DCP_WG) {
+            const int p = P.start + q;
+            V3<float> x = S.x(p);
+            float g0 = 0.f, g1 = 0.f, g2 = 0.f;
+            for (int t = 0; t < nt; ++t) {
+                if (targ[o + t] == p) {
+                    const float* tp = targets + 3 * (size_t)(o + t);
+                    g0 += 2.0f * (x.x - tp[0]) / 3.0f;
+                    g1 += 2.0f * (x.y - tp[1]) / 3.0f;
+                    g2 += 2.0f * (x.z - tp[2]) / 3.0f;
+                }
+            }
+            if (g0 != 0.f || g1 != 0.f || g2 != 0.f) {
+                float sc = sd / (float)nt;
+                V3<float> prev = Gout.x(p);
+                store_v3(Gout.px(0), sim.NP, p, {prev.x + sc * g0, prev.y + sc * g1, prev.z + sc * g2});
+            }
+        }
+    }
+}
+
+template __global__ void k_prep_adj<double>(SimDev, StateView, StateView, StateView, int);
+template __global__ void k_prep_adj<float>(SimDev, StateView, StateView, StateView, int);

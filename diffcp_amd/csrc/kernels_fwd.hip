@@ -1,0 +1,298 @@
+// Forward substep kernels (simulation.py:9-430) and the point-cloud loss
+// (parameter_fitting_functions.py:3-15), batched over instances.
+//
+// One forward substep = 4 launches:
+//   k_prep     per element: x_e, v_e (vertex means), d and F rebuilt from the
+//              vertex positions and the stored normal column, QR + FP64 polar,
+//              stress P, the element's normal impulse vector and the three
+//              vertex force impulses (simulation.py:24-42, 193-248, 386-430)
+//   k_p2g      per patch: APIC scatter into an LDS tile over the patch's
+//              bounding box, then one flush of the touched cells into the
+//              instance grid; the element in-plane forces reach the grid with
+//              the vertex particles (sum of incident-element impulses, the same
+//              weights, simulation.py:266-271)
+//   k_gridop   per active 4^3 block: normalise, damping, wall conditions
+//              (simulation.py:274-302)
+//   k_g2p      per patch: APIC gather, x/v/C update, element C damping
+//              (simulation.py:304-383)
+#include "kernels.hpp"
+
+// ---------------------------------------------------------------------------
+// element prep
+// ---------------------------------------------------------------------------
+template <typename PT>
+__global__ __launch_bounds__(DCP_WG) void k_prep(SimDev sim, StateView S, StateView Snext, int fresh) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *sim.n_active = 0;
+    const int NE = sim.NE;
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < NE; e += gridDim.x * blockDim.x) {
+        const InstDev& I = sim.inst[sim.elem_inst[e]];
+        ElemKin K;
+        element_kinematics(sim, I, S, e, fresh, K);
+        V3<float> v1 = S.v(K.i1), v2 = S.v(K.i2), v3 = S.v(K.i3);
+        const float third = 1.0f / 3.0f;
+        V3<float> ve = {third * (v1.x + v2.x + v3.x), third * (v1.y + v2.y + v3.y), third * (v1.z + v2.z + v3.z)};
+        V3<float> n = {K.d(0, 2), K.d(1, 2), K.d(2, 2)};
+        ElemParams prm = elem_params(sim, I, e);
+        StressState SS;
+        M3<float> P = stress_forward<PT>(K.F, prm, SS);
+        float vd = I.vol_quad * I.dt;
+        V3<float> k = {-vd * P(0, 2), -vd * P(1, 2), -vd * P(2, 2)};
+        V3<float> r1 = {K.Dinv(0, 0), K.d(0, 1), K.d(0, 2)};   // simulation.py:247
+        V3<float> r2 = {K.Dinv(1, 0), K.d(1, 1), K.d(1, 2)};   // simulation.py:248
+        V3<float> f1 = vd * mv(P, r1 + r2);
+        V3<float> f2 = (-vd) * mv(P, r1);
+        V3<float> f3 = (-vd) * mv(P, r2);
+        store_v3(sim.xe, NE, e, K.xe);
+        store_v3(sim.ve, NE, e, ve);
+        store_v3(sim.ke, NE, e, k);
+        store_v3(sim.nd, NE, e, n);
+        store_v3(sim.fc, NE, e, f1);
+        store_v3(sim.fc + (size_t)3 * NE, NE, e, f2);
+        store_v3(sim.fc + (size_t)6 * NE, NE, e, f3);
+        if (Snext.p) store_v3(Snext.en(0), NE, e, n);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// P2G
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void mark_block(const SimDev& sim, const InstDev& I, int b, int i, int j, int k) {
+    int blk = ((i >> 2) * I.nb + (j >> 2)) * I.nb + (k >> 2);
+    int* f = sim.flags + I.flag_base + blk;
+    if (__atomic_load_n(f, __ATOMIC_RELAXED) == 0) {
+        if (atomicCAS(f, 0, 1) == 0) {
+            int slot = atomicAdd(sim.n_active, 1);
+            if (slot < sim.max_active) sim.active[slot] = make_int2(b, blk);
+        }
+    }
+}
+
+__global__ __launch_bounds__(DCP_WG) void k_p2g(SimDev sim, StateView S, int s, int g) {
+    __shared__ float tm[DCP_TILE_CAP], tx[DCP_TILE_CAP], ty[DCP_TILE_CAP], tz[DCP_TILE_CAP];
+    __shared__ int sbox[6];
+    const int tid = threadIdx.x;
+    for (int pi = blockIdx.x; pi < sim.n_patches; pi += gridDim.x) {
+        const Patch P = sim.patches[pi];
+        const InstDev& I = sim.inst[P.inst];
+        if (tid < 3) { sbox[tid] = 1 << 30; sbox[3 + tid] = -(1 << 30); }
+        __syncthreads();
+        int lo[3] = {1 << 30, 1 << 30, 1 << 30}, hi[3] = {-(1 << 30), -(1 << 30), -(1 << 30)};
+        for (int q = tid; q < P.count; q += DCP_WG) {
+            V3<float> x = particle_x(sim, S, P, q);
+            Stencil st;
+            make_stencil(x, I.inv_dx, I.G, st, nullptr);
+            for (int d = 0; d < 3; ++d) { lo[d] = min(lo[d], st.base[d]); hi[d] = max(hi[d], st.base[d]); }
+        }
+        for (int d = 0; d < 3; ++d) {
+            int a = wave_min(lo[d]), b = wave_max(hi[d]);
+            if ((tid & 63) == 0) { atomicMin(&sbox[d], a); atomicMax(&sbox[3 + d], b); }
+        }
+        __syncthreads();
+        const int l0 = sbox[0], l1 = sbox[1], l2 = sbox[2];
+        const int e0 = sbox[3] - l0 + 3, e1 = sbox[4] - l1 + 3, e2 = sbox[5] - l2 + 3;
+        const int vol = e0 * e1 * e2;
+        const bool spill = vol > DCP_TILE_CAP;
+        if (!spill) {
+            for (int c = tid; c < vol; c += DCP_WG) { tm[c] = 0.f; tx[c] = 0.f; ty[c] = 0.f; tz[c] = 0.f; }
+        } else if (tid == 0) {
+            atomicOr(sim.status, (unsigned)DCP_STATUS_TILE_SPILL);
+        }
+        __syncthreads();
+        for (int q = tid; q < P.count; q += DCP_WG) {
+            P2GRec r;
+            p2g_record(sim, I, S, P, q, s, g, r);
+            Stencil st;
+            make_stencil(r.x, I.inv_dx, I.G, st, sim.status);
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int j = 0; j < 3; ++j)
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) {
+                        float mm, m0, m1, m2;
+                        p2g_contrib(I, st, r, i, j, k, mm, m0, m1, m2);
+                        int ci = st.base[0] + i, cj = st.base[1] + j, ck = st.base[2] + k;
+                        if (!spill) {
+                            int c = ((ci - l0) * e1 + (cj - l1)) * e2 + (ck - l2);
+                            atomicAdd(&tm[c], mm); atomicAdd(&tx[c], m0); atomicAdd(&ty[c], m1); atomicAdd(&tz[c], m2);
+                        } else {
+                            long long gi = cell_index(I, ci, cj, ck);
+                            atomicAdd(&sim.acc[gi].x, mm); atomicAdd(&sim.acc[gi].y, m0);
+                            atomicAdd(&sim.acc[gi].z, m1); atomicAdd(&sim.acc[gi].w, m2);
+                            mark_block(sim, I, P.inst, ci, cj, ck);
+                        }
+                    }
+        }
+        __syncthreads();
+        if (!spill) {
+            for (int c = tid; c < vol; c += DCP_WG) {
+                float mm = tm[c];
+                if (mm != 0.f || tx[c] != 0.f || ty[c] != 0.f || tz[c] != 0.f) {
+                    int ck = c % e2, t = c / e2;
+                    int cj = t % e1, ci = t / e1;
+                    ci += l0; cj += l1; ck += l2;
+                    long long gi = cell_index(I, ci, cj, ck);
+                    atomicAdd(&sim.acc[gi].x, mm); atomicAdd(&sim.acc[gi].y, tx[c]);
+                    atomicAdd(&sim.acc[gi].z, ty[c]); atomicAdd(&sim.acc[gi].w, tz[c]);
+                    mark_block(sim, I, P.inst, ci, cj, ck);
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// grid update over the active blocks
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(DCP_WG) void k_gridop(SimDev sim, int keep) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    int n = *sim.n_active;
+    if (n > sim.max_active) n = sim.max_active;
+    for (int it = blockIdx.x * (DCP_WG / 64) + wv; it < n; it += gridDim.x * (DCP_WG / 64)) {
+        const int2 a = sim.active[it];
+        const InstDev& I = sim.inst[a.x];
+        const long long cell = I.grid_base + (long long)a.y * 64 + lane;
+        int ci, cj, ck;
+        block_cell_coords(I, a.y, lane, ci, cj, ck);
+        float4 c = sim.acc[cell];
+        float v[3];
+        grid_velocity(I, c, ci, cj, ck, v, nullptr);
+        sim.vout[cell] = make_float4(v[0], v[1], v[2], 0.f);
+        if (!(isfinite(v[0]) && isfinite(v[1]) && isfinite(v[2]))) atomicOr(sim.status, (unsigned)DCP_STATUS_NONFINITE);
+        if (!keep) {
+            sim.acc[cell] = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (lane == 0) sim.flags[I.flag_base + a.y] = 0;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// G2P
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(DCP_WG) void k_g2p(SimDev sim, StateView S, StateView Sn) {
+    const int tid = threadIdx.x;
+    for (int pi = blockIdx.x; pi < sim.n_patches; pi += gridDim.x) {
+        const Patch P = sim.patches[pi];
+        const InstDev& I = sim.inst[P.inst];
+        for (int q = tid; q < P.count; q += DCP_WG) {
+            V3<float> x = particle_x(sim, S, P, q);
+            Stencil st;
+            make_stencil(x, I.inv_dx, I.G, st, nullptr);
+            V3<float> nv = {0.f, 0.f, 0.f};
+            M3<float> nC = mzero<float>();
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int j = 0; j < 3; ++j)
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) {
+                        float4 gv = sim.vout[cell_index(I, st.base[0] + i, st.base[1] + j, st.base[2] + k)];
+                        float w = st.w[i][0] * st.w[j][1] * st.w[k][2];
+                        float dp[3] = {(float)i - st.fx[0], (float)j - st.fx[1], (float)k - st.fx[2]};
+                        nv.x += w * gv.x; nv.y += w * gv.y; nv.z += w * gv.z;
+                        float w4 = 4.0f * w;
+                        float g3[3] = {gv.x, gv.y, gv.z};
+#pragma unroll
+                        for (int a = 0; a < 3; ++a)
+#pragma unroll
+                            for (int b = 0; b < 3; ++b) nC(a, b) += (w4 * (g3[a] * dp[b])) * I.inv_dx;
+                    }
+            if (P.cls == CLS_ELEMENT) {
+                const int e = P.start + q;
+                M3<float> out;
+#pragma unroll
+                for (int a = 0; a < 3; ++a)
+#pragma unroll
+                    for (int b = 0; b < 3; ++b) {
+                        float sk = 0.5f * (nC(a, b) - nC(b, a));
+                        float sy = 0.5f * (nC(a, b) + nC(b, a));
+                        out(a, b) = sk + DCP_C_SYM_KEEP * sy;        // simulation.py:381-383
+                    }
+                store_m9(Sn.eC(0), sim.NE, e, out);
+            } else {
+                const int p = P.start + q;
+                bool ok = isfinite(nv.x) && isfinite(nv.y) && isfinite(nv.z);
+                if (!ok) atomicOr(sim.status, (unsigned)DCP_STATUS_NONFINITE);
+                store_v3(Sn.pv(0), sim.NP, p, nv);
+                store_v3(Sn.px(0), sim.NP, p, {x.x + I.dt * nv.x, x.y + I.dt * nv.y, x.z + I.dt * nv.z});
+                store_m9(Sn.pC(0), sim.NP, p, nC);
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// loss: per target the first-minimum vertex (strict <, lowest natural index)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(DCP_WG) void k_loss_min(SimDev sim, StateView S, const float* __restrict__ targets,
+                                                       const int4* __restrict__ tchunks, int n_chunks,
+                                                       const int* __restrict__ vnat, float* tmin, int* targ) {
+    __shared__ float sx[DCP_WG], sy[DCP_WG], sz[DCP_WG];
+    __shared__ int sn[DCP_WG];
+    const int tid = threadIdx.x;
+    for (int ci = blockIdx.x; ci < n_chunks; ci += gridDim.x) {
+        const int4 ch = tchunks[ci];            // (instance, first target, count, total targets)
+        const InstDev& I = sim.inst[ch.x];
+        const int t = ch.y + tid;
+        const bool act = tid < ch.z;
+        float t0 = 0.f, t1 = 0.f, t2 = 0.f;
+        if (act) { t0 = targets[3 * t]; t1 = targets[3 * t + 1]; t2 = targets[3 * t + 2]; }
+        float best = 0.f; int bnat = 0x7fffffff, bidx = -1;
+        for (int base = 0; base < I.v_count; base += DCP_WG) {
+            __syncthreads();
+            int p = base + tid;
+            if (p < I.v_count) {
+                int gp = I.v_start + p;
+                sx[tid] = S.px(0)[gp]; sy[tid] = S.px(1)[gp]; sz[tid] = S.px(2)[gp]; sn[tid] = vnat[gp];
+            }
+            __syncthreads();
+            int m = min(DCP_WG, I.v_count - base);
+            if (act) {
+                for (int u = 0; u < m; ++u) {
+                    float d0 = sx[u] - t0, d1 = sy[u] - t1, d2 = sz[u] - t2;
+                    float dn = (d0 * d0 + d1 * d1 + d2 * d2) / 3.0f;
+                    int nat = sn[u];
+                    if (bidx < 0 || dn < best || (dn == best && nat < bnat)) { best = dn; bnat = nat; bidx = I.v_start + base + u; }
+                }
+            }
+        }
+        if (act) { tmin[t] = best; targ[t] = bidx; }
+    }
+}
+
+// per-instance FP64 sum in a fixed order
+__global__ __launch_bounds__(DCP_WG) void k_loss_sum(const float* tmin, const int* toff, const int* tcount,
+                                                      double* loss, int B) {
+    __shared__ double red[DCP_WG];
+    for (int b = blockIdx.x; b < B; b += gridDim.x) {
+        double acc = 0.0;
+        const int n = tcount[b], o = toff[b];
+        for (int q = threadIdx.x; q < n; q += DCP_WG) acc += (double)tmin[o + q];
+        red[threadIdx.x] = acc;
+        __syncthreads();
+        for (int st = DCP_WG / 2; st > 0; st >>= 1) {
+            if (threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) loss[b] = n > 0 ? red[0] / (double)n : 0.0;
+        __syncthreads();
+    }
+}
+
+// binning keys of a slot (integer, for the bit-exact tests)
+__global__ void k_binning_keys(SimDev sim, StateView S, int* keys) {
+    for (int pi = blockIdx.x; pi < sim.n_patches; pi += gridDim.x) {
+        const Patch P = sim.patches[pi];
+        const InstDev& I = sim.inst[P.inst];
+        for (int q = threadIdx.x; q < P.count; q += blockDim.x) {
+            V3<float> x = particle_x(sim, S, P, q);
+            int idx = (P.cls == CLS_ELEMENT) ? sim.NP + P.start + q : P.start + q;
+#pragma unroll
+            for (int d = 0; d < 3; ++d) keys[3 * idx + d] = (int)(x[d] * I.inv_dx - 0.5f);
+        }
+    }
+}
+
+template __global__ void k_prep<double>(SimDev, StateView, StateView, int);
+template __global__ void k_prep<float>(SimDev, StateView, StateView, int);

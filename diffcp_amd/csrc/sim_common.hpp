@@ -1,0 +1,212 @@
+// Shared device layout and small-matrix math for the codim-MPM cloth kernels.
+//
+// Layout in HBM (all instances of a batch concatenated):
+//   points   = vertex, grasper and cube particles, SoA, one index space [0, NP)
+//   elements = one particle per triangle, SoA, [0, NE)
+//   a state slot = x[3][NP] v[3][NP] C[9][NP] | Ce[9][NE] n[3][NE]   (floats)
+//     n = third column of d at the previous substep (the rest-shape normal
+//     at a fresh state); F and d are rebuilt from the vertex positions.
+//   grid     = one dense G^3 region per instance, cells in 4x4x4 blocks
+//     (64 cells = 1 KB of float4 contiguous), only active blocks touched.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define DCP_TILE_CAP 2048          // LDS tile cells of a P2G patch (4 floats each)
+#define DCP_PATCH_MAX 1024         // particles per patch
+#define DCP_WG 256                 // threads per particle workgroup
+
+enum : int { CLS_VERTEX = 0, CLS_GRASPER = 1, CLS_CUBE = 2, CLS_ELEMENT = 3 };
+
+struct InstDev {
+    int G, nb;                     // grid side, blocks per side (ceil(G/4))
+    long long grid_base;           // first cell of this instance in the grid pool
+    int flag_base;                 // first block flag of this instance
+    int steps_per_frame, frames_per_window, group_size, n_actions, n_grasp, traj_off;
+    int v_start, v_count, o_start, o_count, c_start, c_count, e_start, e_count;
+    float dt, dx, inv_dx, gravity;
+    float gdt;                     // dt * scene_scaling * gravity   (simulation.py:182)
+    float half_dt2;                // 0.5 * dt * dt                   (simulation.py:399)
+    float m_vertex, m_quad, vol_quad, m_cube, m_op, interval;
+    float E, nu, ks, kc, damping, friction;
+};
+
+struct Patch {                     // a chunk of one particle class of one instance
+    int inst, cls, start, count;
+};
+
+struct SimDev {
+    int B, NP, NE, n_patches;
+    int precision, svd_grad, per_elem, max_active;
+    const InstDev* inst;
+    const Patch* patches;          // point patches then element patches
+    const int* tri;                // [3][NE] point indices
+    const float* Dinv;             // [9][NE] inverse of the rest d
+    const float* Dinvinv;          // [9][NE] inverse(Dinv) (the d <- F @ Dinv^-1 round trip)
+    const float* emod;             // [3][NE] per-element (E, shear, contact) or null
+    const int* inc_off;            // [NP+1] vertex -> incident (element, corner)
+    const int* inc_list;           // [3 NE]   e*3 + corner
+    const float* traj;             // trajectory pool [frames*grasp][3]
+    const int* elem_inst;          // [NE] instance of each element
+    // forward temporaries (element prep outputs)
+    float* xe; float* ve; float* ke; float* fc;          // [3][NE] [3][NE] [3][NE] [9][NE]
+    float* nd;                                            // [3][NE] n of the current d
+    // grid
+    float4* acc;                   // (m, mv) accumulators
+    float4* vout;                  // grid_v_out
+    float4* gacc;                  // d/d grid_v_out accumulators
+    float4* gmv;                   // (d/dm, d/dmv)
+    int* flags;                    // active block flags
+    int2* active;                  // (instance, block) list
+    int* n_active;
+    unsigned* status;
+    // adjoint temporaries
+    float* g_xe; float* g_ve; float* g_ne; float* g_ke;  // [3][NE]
+    float* g_fv;                   // [3][NP] d/d (vertex force impulse)
+    float* ec_gx; float* ec_gv;    // [9][NE] per-corner d/dx, d/dv
+    double* pgrad_patch;           // [n_patches][4]  E, nu, shear, contact (element patches)
+    double* pgrad_inst;            // [B][DCP_N_PARAMS] (damping, friction via atomics)
+    double* emod_grad;             // [3][NE]
+    double* traj_grad;             // [frames*grasp][3]
+};
+
+// ---------------------------------------------------------------------------
+// small fixed-size math
+// ---------------------------------------------------------------------------
+template <typename T> struct V3 {
+    T x, y, z;
+    __host__ __device__ V3() {}
+    __host__ __device__ V3(T a, T b, T c) : x(a), y(b), z(c) {}
+    __host__ __device__ T operator[](int i) const { return i == 0 ? x : (i == 1 ? y : z); }
+    __host__ __device__ T& operator[](int i) { return i == 0 ? x : (i == 1 ? y : z); }
+};
+template <typename T> __device__ __forceinline__ V3<T> operator+(V3<T> a, V3<T> b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+template <typename T> __device__ __forceinline__ V3<T> operator-(V3<T> a, V3<T> b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+template <typename T> __device__ __forceinline__ V3<T> operator*(T s, V3<T> a) { return {s * a.x, s * a.y, s * a.z}; }
+template <typename T> __device__ __forceinline__ T dot(V3<T> a, V3<T> b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+
+template <typename T> struct M3 {
+    T m[9];                                    // row-major
+    __device__ T& operator()(int i, int j) { return m[3 * i + j]; }
+    __device__ T operator()(int i, int j) const { return m[3 * i + j]; }
+};
+template <typename T> __device__ __forceinline__ M3<T> mzero() { M3<T> r; for (int i = 0; i < 9; ++i) r.m[i] = T(0); return r; }
+template <typename T> __device__ __forceinline__ M3<T> meye() { M3<T> r = mzero<T>(); r.m[0] = r.m[4] = r.m[8] = T(1); return r; }
+template <typename T> __device__ __forceinline__ M3<T> mmul(const M3<T>& a, const M3<T>& b) {
+    M3<T> r;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) r(i, j) = a(i, 0) * b(0, j) + a(i, 1) * b(1, j) + a(i, 2) * b(2, j);
+    return r;
+}
+template <typename T> __device__ __forceinline__ M3<T> mT(const M3<T>& a) {
+    M3<T> r;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) r(i, j) = a(j, i);
+    return r;
+}
+template <typename T> __device__ __forceinline__ M3<T> madd(const M3<T>& a, const M3<T>& b) { M3<T> r; for (int i = 0; i < 9; ++i) r.m[i] = a.m[i] + b.m[i]; return r; }
+template <typename T> __device__ __forceinline__ M3<T> mscale(T s, const M3<T>& a) { M3<T> r; for (int i = 0; i < 9; ++i) r.m[i] = s * a.m[i]; return r; }
+template <typename T> __device__ __forceinline__ V3<T> mv(const M3<T>& a, V3<T> v) {
+    return {a(0, 0) * v.x + a(0, 1) * v.y + a(0, 2) * v.z, a(1, 0) * v.x + a(1, 1) * v.y + a(1, 2) * v.z,
+            a(2, 0) * v.x + a(2, 1) * v.y + a(2, 2) * v.z};
+}
+template <typename T> __device__ __forceinline__ V3<T> mtv(const M3<T>& a, V3<T> v) {  // a^T v
+    return {a(0, 0) * v.x + a(1, 0) * v.y + a(2, 0) * v.z, a(0, 1) * v.x + a(1, 1) * v.y + a(2, 1) * v.z,
+            a(0, 2) * v.x + a(1, 2) * v.y + a(2, 2) * v.z};
+}
+template <typename T> __device__ __forceinline__ M3<T> outer(V3<T> a, V3<T> b) {
+    M3<T> r;
+    r(0, 0) = a.x * b.x; r(0, 1) = a.x * b.y; r(0, 2) = a.x * b.z;
+    r(1, 0) = a.y * b.x; r(1, 1) = a.y * b.y; r(1, 2) = a.y * b.z;
+    r(2, 0) = a.z * b.x; r(2, 1) = a.z * b.y; r(2, 2) = a.z * b.z;
+    return r;
+}
+// General 3x3 inverse by cofactors (the same formula as Taichi Matrix.inverse).
+template <typename T> __device__ __forceinline__ M3<T> minv(const M3<T>& a) {
+    T c00 = a(1, 1) * a(2, 2) - a(1, 2) * a(2, 1);
+    T c01 = a(1, 2) * a(2, 0) - a(1, 0) * a(2, 2);
+    T c02 = a(1, 0) * a(2, 1) - a(1, 1) * a(2, 0);
+    T det = a(0, 0) * c00 + a(0, 1) * c01 + a(0, 2) * c02;
+    T id = T(1) / det;
+    M3<T> r;
+    r(0, 0) = c00 * id;
+    r(0, 1) = (a(0, 2) * a(2, 1) - a(0, 1) * a(2, 2)) * id;
+    r(0, 2) = (a(0, 1) * a(1, 2) - a(0, 2) * a(1, 1)) * id;
+    r(1, 0) = c01 * id;
+    r(1, 1) = (a(0, 0) * a(2, 2) - a(0, 2) * a(2, 0)) * id;
+    r(1, 2) = (a(0, 2) * a(1, 0) - a(0, 0) * a(1, 2)) * id;
+    r(2, 0) = c02 * id;
+    r(2, 1) = (a(0, 1) * a(2, 0) - a(0, 0) * a(2, 1)) * id;
+    r(2, 2) = (a(0, 0) * a(1, 1) - a(0, 1) * a(1, 0)) * id;
+    return r;
+}
+
+// ---------------------------------------------------------------------------
+// state and static accessors
+// ---------------------------------------------------------------------------
+struct StateView {
+    float* p; int NP, NE;
+    __device__ float* px(int c) const { return p + (size_t)c * NP; }               // c in 0..2
+    __device__ float* pv(int c) const { return p + (size_t)(3 + c) * NP; }
+    __device__ float* pC(int c) const { return p + (size_t)(6 + c) * NP; }         // c in 0..8
+    __device__ float* eC(int c) const { return p + (size_t)15 * NP + (size_t)c * NE; }
+    __device__ float* en(int c) const { return p + (size_t)15 * NP + (size_t)(9 + c) * NE; }
+    __device__ V3<float> x(int i) const { return {px(0)[i], px(1)[i], px(2)[i]}; }
+    __device__ V3<float> v(int i) const { return {pv(0)[i], pv(1)[i], pv(2)[i]}; }
+    __device__ M3<float> C(int i) const { M3<float> r; for (int c = 0; c < 9; ++c) r.m[c] = pC(c)[i]; return r; }
+    __device__ M3<float> Ce(int e) const { M3<float> r; for (int c = 0; c < 9; ++c) r.m[c] = eC(c)[e]; return r; }
+    __device__ V3<float> n(int e) const { return {en(0)[e], en(1)[e], en(2)[e]}; }
+};
+__device__ __forceinline__ M3<float> load_m9(const float* base, int stride, int i) {
+    M3<float> r; for (int c = 0; c < 9; ++c) r.m[c] = base[(size_t)c * stride + i]; return r;
+}
+__device__ __forceinline__ void store_m9(float* base, int stride, int i, const M3<float>& a) {
+    for (int c = 0; c < 9; ++c) base[(size_t)c * stride + i] = a.m[c];
+}
+__device__ __forceinline__ V3<float> load_v3(const float* base, int stride, int i) {
+    return {base[i], base[(size_t)stride + i], base[(size_t)2 * stride + i]};
+}
+__device__ __forceinline__ void store_v3(float* base, int stride, int i, V3<float> a) {
+    base[i] = a.x; base[(size_t)stride + i] = a.y; base[(size_t)2 * stride + i] = a.z;
+}
+
+// ---------------------------------------------------------------------------
+// B-spline stencil (simulation.py:140-142, 196-197)
+// ---------------------------------------------------------------------------
+struct Stencil {
+    int base[3];
+    float fx[3];
+    float w[3][3];    // [node][dim]
+    float dw[3][3];   // d w / d fx
+};
+
+// base = trunc(x*inv_dx - 0.5) (ti.cast to i32); clamped into [0, G-3] with a status bit.
+__device__ __forceinline__ void make_stencil(V3<float> x, float inv_dx, int G, Stencil& s, unsigned* status) {
+    bool out = false;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        float xs = x[d] * inv_dx;
+        int b = (int)(xs - 0.5f);
+        if (b < 0 || b > G - 3 || !(xs == xs)) { out = true; b = b < 0 || !(xs == xs) ? 0 : G - 3; }
+        s.base[d] = b;
+        float f = xs - (float)b;
+        s.fx[d] = f;
+        s.w[0][d] = 0.5f * (1.5f - f) * (1.5f - f);
+        s.w[1][d] = 0.75f - (f - 1.0f) * (f - 1.0f);
+        s.w[2][d] = 0.5f * (f - 0.5f) * (f - 0.5f);
+        s.dw[0][d] = f - 1.5f;
+        s.dw[1][d] = 2.0f * (1.0f - f);
+        s.dw[2][d] = f - 0.5f;
+    }
+    if (out && status) atomicOr(status, 2u);
+}
+
+// blocked cell index inside an instance grid
+__device__ __forceinline__ long long cell_index(const InstDev& I, int i, int j, int k) {
+    int blk = ((i >> 2) * I.nb + (j >> 2)) * I.nb + (k >> 2);
+    return I.grid_base + (long long)blk * 64 + ((i & 3) << 4) + ((j & 3) << 2) + (k & 3);
+}

@@ -1,0 +1,254 @@
+"""Python handle on one device context of the HIP library (a batch of cloth instances)."""
+from __future__ import annotations
+
+import ctypes
+from typing import Dict, List, Optional, Sequence
+
+import numpy as np
+
+from . import _abi
+from .scene import Instance, N_PARAMS
+
+PRECISION = {"mixed": 0, "fp32": 1}
+SVD_GRAD = {"reference": 0, "stable": 1}
+
+
+def _dp(a: Optional[np.ndarray]):
+    if a is None:
+        return None
+    return a.ctypes.data_as(_abi.c_dbl_p)
+
+
+def _fp(a: Optional[np.ndarray]):
+    if a is None:
+        return None
+    return a.ctypes.data_as(_abi.c_flt_p)
+
+
+class Engine:
+    """Owns a ``dcp_ctx``: all device buffers of B instances.
+
+    Slots: ``max_states`` state slots (0 is where :meth:`reset` writes) and 4
+    adjoint slots.
+    """
+
+    def __init__(self, instances: Sequence[Instance], *, max_states: int = 2,
+                 precision: str = "mixed", svd_grad: str = "reference",
+                 per_element_moduli: bool = False, patch_size: int = 0, p2g_mode: int = 0,
+                 grid_wgs: int = 0, stream: Optional[int] = None):
+        self.lib = _abi.load()
+        self.instances = list(instances)
+        self.B = len(self.instances)
+        self._keep = []
+        descs = (_abi.InstanceDesc * self.B)()
+        for b, inst in enumerate(self.instances):
+            vx = np.ascontiguousarray(inst.vertex_x, dtype=np.float64)
+            tri = np.ascontiguousarray(inst.triangles, dtype=np.int32)
+            cx = np.ascontiguousarray(inst.cube_x, dtype=np.float64).reshape(-1, 3)
+            ox = np.ascontiguousarray(inst.grasper_x, dtype=np.float64).reshape(-1, 3)
+            tr = np.ascontiguousarray(inst.trajectory, dtype=np.float64)
+            self._keep += [vx, tri, cx, ox, tr]
+            d = descs[b]
+            d.n_grid = inst.n_grid
+            d.n_vertices = inst.n_vertices
+            d.n_elements = inst.n_elements
+            d.n_cube = inst.n_cube
+            d.n_grasper = inst.n_grasper
+            d.n_grasp_points = inst.n_grasp_points
+            d.n_actions = inst.n_actions
+            d.steps_per_frame = inst.steps_per_frame
+            d.frames_per_window = inst.frames_per_window
+            d.dt = inst.dt
+            d.gravity = inst.gravity
+            d.vertex_p_mass = inst.vertex_p_mass
+            d.quadrature_p_mass = inst.quadrature_p_mass
+            d.quadrature_p_vol = inst.quadrature_p_vol
+            d.cube_p_mass = inst.cube_p_mass
+            d.operator_p_mass = inst.operator_p_mass
+            d.vertex_x = _dp(vx)
+            d.triangles = tri.ctypes.data_as(_abi.c_int_p)
+            d.cube_x = _dp(cx)
+            d.grasper_x = _dp(ox)
+            d.trajectory = _dp(tr)
+        opt = _abi.Options(max_states=max_states, precision=PRECISION[precision],
+                           svd_grad=SVD_GRAD[svd_grad], per_element_moduli=int(per_element_moduli),
+                           patch_size=patch_size, p2g_mode=p2g_mode, grid_wgs=grid_wgs)
+        ctx = ctypes.c_void_p()
+        _abi.check(self.lib.dcp_create(descs, self.B, ctypes.byref(opt), stream, ctypes.byref(ctx)))
+        self.ctx = ctx
+        self.max_states = max_states
+        self.per_element_moduli = per_element_moduli
+        sz = (ctypes.c_int64 * 5)()
+        _abi.check(self.lib.dcp_sizes(self.ctx, sz))
+        self.n_points, self.n_elements, self.slot_floats = int(sz[1]), int(sz[2]), int(sz[3])
+        self.set_params(np.stack([i.params() for i in self.instances]))
+        if per_element_moduli:
+            self.set_element_moduli(np.concatenate([i.element_moduli for i in self.instances]))
+        if all(i.targets is not None for i in self.instances):
+            self.set_targets([i.targets for i in self.instances])
+
+    # ------------------------------------------------------------------ setup
+    def close(self) -> None:
+        if getattr(self, "ctx", None):
+            self.lib.dcp_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_params(self, params: np.ndarray) -> None:
+        p = np.ascontiguousarray(np.asarray(params, dtype=np.float64).reshape(self.B, N_PARAMS))
+        _abi.check(self.lib.dcp_set_params(self.ctx, _dp(p)))
+        self.params = p.copy()
+
+    def set_element_moduli(self, moduli: np.ndarray) -> None:
+        m = np.ascontiguousarray(np.asarray(moduli, dtype=np.float64).reshape(-1, 3))
+        _abi.check(self.lib.dcp_set_element_moduli(self.ctx, _dp(m)))
+
+    def set_targets(self, targets: List[np.ndarray]) -> None:
+        counts = np.array([len(t) for t in targets], dtype=np.int32)
+        t = np.ascontiguousarray(np.concatenate([np.asarray(x, dtype=np.float64).reshape(-1, 3)
+                                                 for x in targets]))
+        _abi.check(self.lib.dcp_set_targets(self.ctx, _dp(t), counts.ctypes.data_as(_abi.c_int_p)))
+
+    def reset(self, slot: int = 0, vertex_x=None, grasper_x=None, cube_x=None) -> None:
+        def cat(arrs):
+            if arrs is None:
+                return None
+            return np.ascontiguousarray(np.concatenate([np.asarray(a, np.float64).reshape(-1, 3) for a in arrs]))
+        vx, ox, cx = cat(vertex_x), cat(grasper_x), cat(cube_x)
+        _abi.check(self.lib.dcp_reset(self.ctx, slot, _dp(vx), _dp(ox), _dp(cx)))
+
+    # ------------------------------------------------------------------ steps
+    def substep(self, s: int, g: int, slot_in: int, slot_out: int) -> None:
+        _abi.check(self.lib.dcp_substep(self.ctx, s, g, slot_in, slot_out))
+
+    def substep_adjoint(self, s: int, g: int, slot_in: int, gslot_in: int, gslot_out: int) -> None:
+        _abi.check(self.lib.dcp_substep_adjoint(self.ctx, s, g, slot_in, gslot_in, gslot_out))
+
+    def zero_adjoint(self, gslot: int) -> None:
+        _abi.check(self.lib.dcp_zero_adjoint(self.ctx, gslot))
+
+    def zero_param_grads(self) -> None:
+        _abi.check(self.lib.dcp_zero_param_grads(self.ctx))
+
+    def loss(self, slot: int) -> np.ndarray:
+        out = np.zeros(self.B, np.float64)
+        _abi.check(self.lib.dcp_loss(self.ctx, slot, _dp(out)))
+        return out
+
+    def loss_adjoint(self, slot: int, gslot: int, seed=None) -> None:
+        sd = None if seed is None else np.ascontiguousarray(np.broadcast_to(np.asarray(seed, np.float64), (self.B,)))
+        _abi.check(self.lib.dcp_loss_adjoint(self.ctx, slot, gslot, _dp(sd)))
+
+    def param_grads(self) -> np.ndarray:
+        out = np.zeros((self.B, N_PARAMS), np.float64)
+        _abi.check(self.lib.dcp_param_grads(self.ctx, _dp(out)))
+        return out
+
+    def element_moduli_grads(self) -> np.ndarray:
+        out = np.zeros((sum(i.n_elements for i in self.instances), 3), np.float64)
+        _abi.check(self.lib.dcp_element_moduli_grads(self.ctx, _dp(out)))
+        return out
+
+    def trajectory_grads(self) -> List[np.ndarray]:
+        n = sum(i.trajectory.size for i in self.instances)
+        out = np.zeros(n, np.float64)
+        _abi.check(self.lib.dcp_trajectory_grads(self.ctx, _dp(out)))
+        res, o = [], 0
+        for i in self.instances:
+            res.append(out[o:o + i.trajectory.size].reshape(i.trajectory.shape))
+            o += i.trajectory.size
+        return res
+
+    def status(self, clear: bool = False) -> int:
+        v = ctypes.c_uint32()
+        _abi.check(self.lib.dcp_status(self.ctx, ctypes.byref(v), int(clear)))
+        return int(v.value)
+
+    # ------------------------------------------------------------------ I/O
+    STATE_NAMES = ("vertex_x", "vertex_v", "vertex_C", "grasper_x", "grasper_v", "grasper_C",
+                   "cube_x", "cube_v", "cube_C", "element_C", "element_n")
+
+    def _alloc_state(self):
+        nv = sum(i.n_vertices for i in self.instances)
+        ne = sum(i.n_elements for i in self.instances)
+        no = sum(i.n_grasper for i in self.instances)
+        nc = sum(i.n_cube for i in self.instances)
+        shapes = {"vertex": nv, "grasper": no, "cube": nc}
+        out = {}
+        for k in self.STATE_NAMES:
+            cls, f = k.split("_")
+            if cls == "element":
+                out[k] = np.zeros((ne, 3, 3) if f == "C" else (ne, 3), np.float32)
+            else:
+                out[k] = np.zeros((shapes[cls], 3, 3) if f == "C" else (shapes[cls], 3), np.float32)
+        return out
+
+    def download_state(self, slot: int) -> Dict[str, np.ndarray]:
+        st = self._alloc_state()
+        _abi.check(self.lib.dcp_download_state(self.ctx, slot, *[_fp(st[k]) for k in self.STATE_NAMES]))
+        return st
+
+    def download_adjoint(self, gslot: int) -> Dict[str, np.ndarray]:
+        st = self._alloc_state()
+        _abi.check(self.lib.dcp_download_adjoint(self.ctx, gslot, *[_fp(st[k]) for k in self.STATE_NAMES]))
+        return st
+
+    def _conv(self, arrs):
+        return {k: np.ascontiguousarray(np.asarray(arrs[k], np.float32))
+                for k in self.STATE_NAMES if arrs.get(k) is not None}
+
+    def upload_state(self, slot: int, **arrs) -> None:
+        conv = self._conv(arrs)
+        _abi.check(self.lib.dcp_upload_state(self.ctx, slot, *[_fp(conv.get(k)) for k in self.STATE_NAMES]))
+
+    def upload_adjoint(self, gslot: int, **arrs) -> None:
+        conv = self._conv(arrs)
+        _abi.check(self.lib.dcp_upload_adjoint(self.ctx, gslot, *[_fp(conv.get(k)) for k in self.STATE_NAMES]))
+
+    def split(self, arr: np.ndarray, count_attr: str) -> List[np.ndarray]:
+        out, o = [], 0
+        for i in self.instances:
+            n = getattr(i, count_attr)
+            out.append(arr[o:o + n])
+            o += n
+        return out
+
+    def binning_keys(self, slot: int) -> np.ndarray:
+        n = sum(i.n_vertices + i.n_grasper + i.n_cube + i.n_elements for i in self.instances)
+        out = np.zeros((n, 3), np.int32)
+        _abi.check(self.lib.dcp_binning_keys(self.ctx, slot, out.ctypes.data_as(_abi.c_int_p)))
+        return out
+
+    # ------------------------------------------------------------------ rollouts
+    def rollout_grad(self, n_substeps: int, g: int = 0, checkpoint_every: int = 0,
+                     use_graph: bool = True):
+        """Forward + adjoint of a whole window from the reset state in slot 0.
+
+        Returns (loss[B], grads[B, 6], device milliseconds).
+        """
+        loss = np.zeros(self.B, np.float64)
+        grads = np.zeros((self.B, N_PARAMS), np.float64)
+        _abi.check(self.lib.dcp_rollout_grad(self.ctx, n_substeps, g, checkpoint_every,
+                                             int(use_graph), _dp(loss), _dp(grads)))
+        ms = ctypes.c_double()
+        _abi.check(self.lib.dcp_last_rollout_ms(self.ctx, ctypes.byref(ms)))
+        return loss, grads, ms.value
+
+    def rollout(self, n_substeps: int, g: int = 0, use_graph: bool = False):
+        slot = ctypes.c_int32()
+        _abi.check(self.lib.dcp_rollout(self.ctx, n_substeps, g, int(use_graph), ctypes.byref(slot)))
+        ms = ctypes.c_double()
+        _abi.check(self.lib.dcp_last_rollout_ms(self.ctx, ctypes.byref(ms)))
+        return slot.value, ms.value
+
+    @staticmethod
+    def slots_for(n_substeps: int, checkpoint_every: int) -> int:
+        """State slots dcp_rollout_grad needs for a plan."""
+        K = checkpoint_every if 0 < checkpoint_every <= n_substeps else n_substeps
+        nseg = (n_substeps + K - 1) // K
+        return nseg + 1 + K + 1
