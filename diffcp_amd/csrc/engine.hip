@@ -35,6 +35,7 @@ struct HostInst {
     std::vector<int> einv;
     std::vector<float> x0v, x0o, x0c;   // creation positions (natural order)
     std::vector<int> tri;     // natural triangles
+    std::vector<uint64_t> vbin, ebin;   // spatial bin of each internal vertex / element
 };
 
 uint32_t spread10(uint32_t v) {
@@ -48,6 +49,18 @@ uint32_t spread10(uint32_t v) {
 uint32_t morton(float x, float y, float z) {
     auto q = [](float a) { int t = (int)std::floor(a * 1024.0f); return (uint32_t)std::min(1023, std::max(0, t)); };
     return spread10(q(x)) | (spread10(q(y)) << 1) | (spread10(q(z)) << 2);
+}
+
+// Spatial bin (cubes of kBinCells grid cells) in the high bits, Morton code of
+// the position in the low bits: sorting by this key groups particles by bin,
+// and patches never straddle a bin, so a patch's tile stays compact.
+constexpr int kBinCells = 6;
+uint64_t bin_of(float x, float y, float z, int G) {
+    auto b = [&](float a) { int t = (int)std::floor(a * G / kBinCells); return (uint64_t)std::min(63, std::max(0, t)); };
+    return (b(x) * 64 + b(y)) * 64 + b(z);
+}
+uint64_t spatial_key(float x, float y, float z, int G) {
+    return (bin_of(x, y, z, G) << 32) | morton(x, y, z);
 }
 
 // host FP32 cofactor inverse (same formula as the device minv)
@@ -241,27 +254,31 @@ static int build(dcp_ctx* c, const dcp_instance_desc* d, int B) {
         for (int t : H.tri)
             if (t < 0 || t >= H.nv) return set_err("triangle index out of range");
         // spatial (Morton) order of vertices and element centroids
-        std::vector<uint32_t> code(H.nv);
-        for (int i = 0; i < H.nv; ++i) code[i] = morton(H.x0v[3 * i], H.x0v[3 * i + 1], H.x0v[3 * i + 2]);
+        std::vector<uint64_t> code(H.nv);
+        for (int i = 0; i < H.nv; ++i) code[i] = spatial_key(H.x0v[3 * i], H.x0v[3 * i + 1], H.x0v[3 * i + 2], H.G);
         H.vperm.resize(H.nv);
         std::iota(H.vperm.begin(), H.vperm.end(), 0);
         std::stable_sort(H.vperm.begin(), H.vperm.end(), [&](int a, int bb) { return code[a] < code[bb]; });
         H.vinv.resize(H.nv);
         for (int i = 0; i < H.nv; ++i) H.vinv[H.vperm[i]] = i;
-        std::vector<uint32_t> ecode(H.ne);
+        H.vbin.resize(H.nv);
+        for (int i = 0; i < H.nv; ++i) H.vbin[i] = code[H.vperm[i]] >> 32;
+        std::vector<uint64_t> ecode(H.ne);
         for (int e = 0; e < H.ne; ++e) {
             float cx = 0, cy = 0, cz = 0;
             for (int k = 0; k < 3; ++k) {
                 int v = H.tri[3 * e + k];
                 cx += H.x0v[3 * v]; cy += H.x0v[3 * v + 1]; cz += H.x0v[3 * v + 2];
             }
-            ecode[e] = morton(cx / 3, cy / 3, cz / 3);
+            ecode[e] = spatial_key(cx / 3, cy / 3, cz / 3, H.G);
         }
         H.eperm.resize(H.ne);
         std::iota(H.eperm.begin(), H.eperm.end(), 0);
         std::stable_sort(H.eperm.begin(), H.eperm.end(), [&](int a, int bb) { return ecode[a] < ecode[bb]; });
         H.einv.resize(H.ne);
         for (int i = 0; i < H.ne; ++i) H.einv[H.eperm[i]] = i;
+        H.ebin.resize(H.ne);
+        for (int i = 0; i < H.ne; ++i) H.ebin[i] = ecode[H.eperm[i]] >> 32;
 
         InstDev& I = c->inst_h[b];
         memset(&I, 0, sizeof(I));
@@ -332,15 +349,25 @@ static int build(dcp_ctx* c, const dcp_instance_desc* d, int B) {
     auto chunk = [&](int b, int cls, int start, int count) {
         for (int o = 0; o < count; o += ps) c->patches_h.push_back(Patch{b, cls, start + o, std::min(ps, count - o)});
     };
+    // runs of equal spatial bin, each cut into patches of at most ps particles
+    auto chunk_bins = [&](int b, int cls, int start, const std::vector<uint64_t>& bins) {
+        int n = (int)bins.size(), r0 = 0;
+        for (int i = 1; i <= n; ++i)
+            if (i == n || bins[i] != bins[r0]) { chunk(b, cls, start + r0, i - r0); r0 = i; }
+    };
     for (int b = 0; b < B; ++b) {
         const InstDev& I = c->inst_h[b];
-        chunk(b, CLS_VERTEX, I.v_start, I.v_count);
-        chunk(b, CLS_GRASPER, I.o_start, I.o_count);
+        chunk_bins(b, CLS_VERTEX, I.v_start, c->hi[b].vbin);
+        for (int gp = 0; gp < std::max(1, I.n_grasp); ++gp) {    // one grasp point per patch
+            int g0 = gp * I.group_size, g1 = std::min(I.o_count, g0 + I.group_size);
+            if (I.n_grasp == 0) { g0 = 0; g1 = I.o_count; }
+            if (g1 > g0) chunk(b, CLS_GRASPER, I.o_start + g0, g1 - g0);
+        }
         chunk(b, CLS_CUBE, I.c_start, I.c_count);
     }
     for (int b = 0; b < B; ++b) {
         const InstDev& I = c->inst_h[b];
-        chunk(b, CLS_ELEMENT, I.e_start, I.e_count);
+        chunk_bins(b, CLS_ELEMENT, I.e_start, c->hi[b].ebin);
     }
     c->n_patches = (int)c->patches_h.size();
     // trajectory pool

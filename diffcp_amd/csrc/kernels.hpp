@@ -165,6 +165,52 @@ __device__ __forceinline__ void p2g_contrib(const InstDev& I, const Stencil& st,
     }
 }
 
+// Weight and d/dfx of node i (a runtime value 0..2) of the quadratic B-spline
+// at fx, with the same expressions as simulation.py:142,197, selected from
+// scalars so that no per-lane indexed array (scratch) is needed.
+__device__ __forceinline__ void bsp_node(float fx, int i, float& w, float& dw) {
+    float a = 1.5f - fx, b = fx - 1.0f, c = fx - 0.5f;
+    float w0 = 0.5f * (a * a), w1 = 0.75f - b * b, w2 = 0.5f * (c * c);
+    float d0 = fx - 1.5f, d1 = 2.0f * (1.0f - fx), d2 = c;
+    w = i == 0 ? w0 : (i == 1 ? w1 : w2);
+    dw = i == 0 ? d0 : (i == 1 ? d1 : d2);
+}
+
+// Scatter order of a lane: at step t lane l visits stencil node (t + l) mod 27,
+// so lanes whose particles share a base cell update 27 different LDS words
+// instead of serialising on one.
+__device__ __forceinline__ void rotated_offset(int t, int lane, int& i, int& j, int& k) {
+    int o = t + lane;
+    o -= 27 * (o / 27);
+    i = o / 9;
+    j = (o / 3) % 3;
+    k = o % 3;
+}
+
+__device__ __forceinline__ void p2g_contrib_rt(const InstDev& I, const Stencil& st, const P2GRec& r, int i, int j,
+                                               int k, float& mm, float& m0, float& m1, float& m2) {
+    float W0, W1, W2, D0, D1, D2;
+    bsp_node(st.fx[0], i, W0, D0);
+    bsp_node(st.fx[1], j, W1, D1);
+    bsp_node(st.fx[2], k, W2, D2);
+    float w = W0 * W1 * W2;
+    float dp0 = ((float)i - st.fx[0]) * I.dx, dp1 = ((float)j - st.fx[1]) * I.dx, dp2 = ((float)k - st.fx[2]) * I.dx;
+    float a0 = r.A(0, 0) * dp0 + r.A(0, 1) * dp1 + r.A(0, 2) * dp2;
+    float a1 = r.A(1, 0) * dp0 + r.A(1, 1) * dp1 + r.A(1, 2) * dp2;
+    float a2 = r.A(2, 0) * dp0 + r.A(2, 1) * dp1 + r.A(2, 2) * dp2;
+    mm = w * r.m;
+    m0 = w * (r.b.x + a0);
+    m1 = w * (r.b.y + a1);
+    m2 = w * (r.b.z + a2);
+    if (r.elem) {
+        float g0 = D0 * W1 * W2;
+        float g1 = W0 * D1 * W2;
+        float g2 = W0 * W1 * D2;
+        float sdot = (g0 * r.n.x + g1 * r.n.y + g2 * r.n.z) * I.inv_dx;
+        m0 += sdot * r.k.x; m1 += sdot * r.k.y; m2 += sdot * r.k.z;
+    }
+}
+
 __device__ __forceinline__ void block_cell_coords(const InstDev& I, int blk, int lane, int& i, int& j, int& k) {
     int bk = blk % I.nb, t = blk / I.nb;
     int bj = t % I.nb, bi = t / I.nb;

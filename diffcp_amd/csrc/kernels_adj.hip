@@ -10,6 +10,7 @@
 //                   parameter gradients reduced per patch in FP64, fixed order
 //   k_vertex_final  per vertex: incident-element corner gradients (CSR order)
 #include "kernels.hpp"
+#include "patch_sort.hpp"
 
 __global__ void k_fill_f(float* p, size_t n, float v) {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
@@ -18,109 +19,154 @@ __global__ void k_fill_f(float* p, size_t n, float v) {
 // ---------------------------------------------------------------------------
 // G2P adjoint
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(DCP_WG) void k_g2p_adj(SimDev sim, StateView S, StateView G1, StateView Gout) {
-    __shared__ float tx[DCP_TILE_CAP], ty[DCP_TILE_CAP], tz[DCP_TILE_CAP];
-    __shared__ int sbox[6];
-    const int tid = threadIdx.x;
+// adjoint inputs of one particle for the G2P reverse: dL/dv_new (incl. the
+// x_new = x + dt v_new term), dL/dC_new (element: through the damping map) and
+// the direct dL/dx_new
+__device__ __forceinline__ void g2p_adj_inputs(const SimDev& sim, const InstDev& I, const StateView& G1, const Patch& P,
+                                               int q, V3<float>& gv, V3<float>& gx1, M3<float>& gC) {
     const float csym = 0.5f * DCP_C_SYM_KEEP;
+    if (P.cls == CLS_ELEMENT) {
+        M3<float> go = G1.Ce(P.start + q);
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b) gC(a, b) = 0.5f * (go(a, b) - go(b, a)) + csym * (go(a, b) + go(b, a));
+        gv = {0.f, 0.f, 0.f};
+        gx1 = {0.f, 0.f, 0.f};
+    } else {
+        const int p = P.start + q;
+        gx1 = G1.x(p);
+        V3<float> g1v = G1.v(p);
+        gv = {g1v.x + I.dt * gx1.x, g1v.y + I.dt * gx1.y, g1v.z + I.dt * gx1.z};
+        gC = G1.C(p);
+    }
+}
+
+// one stencil node of the G2P reverse: dL/d grid_v_out of the node (gg) and the
+// node's share of dL/dfx
+__device__ __forceinline__ void g2p_adj_node(const InstDev& I, const Stencil& st, int i, int j, int k, float4 gvo,
+                                             const V3<float>& gv, const M3<float>& gC, float gg[3], float gfx[3]) {
+    const float c4 = 4.0f * I.inv_dx;
+    float W0 = st.w[i][0], W1 = st.w[j][1], W2 = st.w[k][2];
+    float w = W0 * W1 * W2;
+    float dp[3] = {(float)i - st.fx[0], (float)j - st.fx[1], (float)k - st.fx[2]};
+    float g[3] = {gvo.x, gvo.y, gvo.z};
+    float gCdp[3], gCtg[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        gCdp[a] = gC(a, 0) * dp[0] + gC(a, 1) * dp[1] + gC(a, 2) * dp[2];
+        gCtg[a] = gC(0, a) * g[0] + gC(1, a) * g[1] + gC(2, a) * g[2];
+    }
+    float s1 = c4 * w;
+    gg[0] = w * gv.x + s1 * gCdp[0];
+    gg[1] = w * gv.y + s1 * gCdp[1];
+    gg[2] = w * gv.z + s1 * gCdp[2];
+    float scal = gv.x * g[0] + gv.y * g[1] + gv.z * g[2] + c4 * (g[0] * gCdp[0] + g[1] * gCdp[1] + g[2] * gCdp[2]);
+    float dW0 = st.dw[i][0] * W1 * W2, dW1 = W0 * st.dw[j][1] * W2, dW2 = W0 * W1 * st.dw[k][2];
+    gfx[0] += dW0 * scal - s1 * gCtg[0];
+    gfx[1] += dW1 * scal - s1 * gCtg[1];
+    gfx[2] += dW2 * scal - s1 * gCtg[2];
+}
+
+__device__ __forceinline__ void g2p_adj_store(const SimDev& sim, const InstDev& I, const StateView& Gout, const Patch& P,
+                                              int q, const V3<float>& gx1, const float gfx[3]) {
+    V3<float> gxt = {I.inv_dx * gfx[0], I.inv_dx * gfx[1], I.inv_dx * gfx[2]};
+    if (P.cls == CLS_ELEMENT) store_v3(sim.g_xe, sim.NE, P.start + q, gxt);
+    else store_v3(Gout.px(0), sim.NP, P.start + q, gx1 + gxt);
+}
+
+__device__ __noinline__ void g2p_adj_spill(const SimDev& sim, const StateView& S, const StateView& G1,
+                                           const StateView& Gout, const Patch& P, const InstDev& I) {
+    for (int q = threadIdx.x; q < P.count; q += DCP_WG) {
+        Stencil st;
+        make_stencil(particle_x(sim, S, P, q), I.inv_dx, I.G, st, nullptr);
+        V3<float> gv, gx1;
+        M3<float> gC;
+        g2p_adj_inputs(sim, I, G1, P, q, gv, gx1, gC);
+        float gfx[3] = {0.f, 0.f, 0.f};
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j)
+                for (int k = 0; k < 3; ++k) {
+                    long long gi = cell_index(I, st.base[0] + i, st.base[1] + j, st.base[2] + k);
+                    float gg[3];
+                    g2p_adj_node(I, st, i, j, k, sim.vout[gi], gv, gC, gg, gfx);
+                    atomicAdd(&sim.gacc[gi].y, gg[0]); atomicAdd(&sim.gacc[gi].z, gg[1]); atomicAdd(&sim.gacc[gi].w, gg[2]);
+                }
+        g2p_adj_store(sim, I, Gout, P, q, gx1, gfx);
+    }
+}
+
+// G2P reverse of one patch: the per-particle dL/dx is a gather; the scatter of
+// dL/d grid_v_out uses the sorted segmented reduction of k_p2g.
+__global__ __launch_bounds__(DCP_WG) void k_g2p_adj(SimDev sim, StateView S, StateView G1, StateView Gout) {
+    __shared__ SortLDS L;
+    __shared__ float4 tiles[DCP_WG / 64][DCP_STILE_CAP];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     for (int pi = blockIdx.x; pi < sim.n_patches; pi += gridDim.x) {
         const Patch P = sim.patches[pi];
-        const InstDev& I = sim.inst[P.inst];
-        if (tid < 3) { sbox[tid] = 1 << 30; sbox[3 + tid] = -(1 << 30); }
-        __syncthreads();
-        int lo[3] = {1 << 30, 1 << 30, 1 << 30}, hi[3] = {-(1 << 30), -(1 << 30), -(1 << 30)};
-        for (int q = tid; q < P.count; q += DCP_WG) {
-            Stencil st;
-            make_stencil(particle_x(sim, S, P, q), I.inv_dx, I.G, st, nullptr);
-            for (int d = 0; d < 3; ++d) { lo[d] = min(lo[d], st.base[d]); hi[d] = max(hi[d], st.base[d]); }
+        const InstDev I = sim.inst[P.inst];
+        PatchBox B;
+        patch_bbox(sim, S, P, I, L, B);
+        if (B.vol > DCP_STILE_CAP || P.count > DCP_SORT_MAX) {
+            g2p_adj_spill(sim, S, G1, Gout, P, I);
+            __syncthreads();
+            continue;
         }
-        for (int d = 0; d < 3; ++d) {
-            int a = wave_min(lo[d]), b = wave_max(hi[d]);
-            if ((tid & 63) == 0) { atomicMin(&sbox[d], a); atomicMax(&sbox[3 + d], b); }
-        }
-        __syncthreads();
-        const int l0 = sbox[0], l1 = sbox[1], l2 = sbox[2];
-        const int e0 = sbox[3] - l0 + 3, e1 = sbox[4] - l1 + 3, e2 = sbox[5] - l2 + 3;
-        const int vol = e0 * e1 * e2;
-        const bool spill = vol > DCP_TILE_CAP;
-        if (!spill)
-            for (int c = tid; c < vol; c += DCP_WG) { tx[c] = 0.f; ty[c] = 0.f; tz[c] = 0.f; }
-        __syncthreads();
-        for (int q = tid; q < P.count; q += DCP_WG) {
-            V3<float> x = particle_x(sim, S, P, q);
+        patch_sort(sim, S, P, I, B, L);
+        float4* T = tiles[wv];
+        for (int c = lane; c < B.vol; c += 64) T[c] = make_float4(0.f, 0.f, 0.f, 0.f);
+        for (int sp0 = 0; sp0 < P.count; sp0 += DCP_WG) {
+            const int sp = sp0 + tid;
+            const bool act = sp < P.count;
+            SegInfo gi = segment_info(L, sp, P.count);
+            const int q = act ? L.perm[sp] : 0;
             Stencil st;
-            make_stencil(x, I.inv_dx, I.G, st, nullptr);
-            V3<float> gv, gx1;
-            M3<float> gC;
-            if (P.cls == CLS_ELEMENT) {
-                M3<float> go = G1.Ce(P.start + q);
-                for (int a = 0; a < 3; ++a)
-                    for (int b = 0; b < 3; ++b)
-                        gC(a, b) = 0.5f * (go(a, b) - go(b, a)) + csym * (go(a, b) + go(b, a));
-                gv = {0.f, 0.f, 0.f};
-                gx1 = {0.f, 0.f, 0.f};
+            V3<float> gv = {0.f, 0.f, 0.f}, gx1 = {0.f, 0.f, 0.f};
+            M3<float> gC = mzero<float>();
+            if (act) {
+                make_stencil(particle_x(sim, S, P, q), I.inv_dx, I.G, st, nullptr);
+                g2p_adj_inputs(sim, I, G1, P, q, gv, gx1, gC);
             } else {
-                const int p = P.start + q;
-                gx1 = G1.x(p);
-                V3<float> g1v = G1.v(p);
-                gv = {g1v.x + I.dt * gx1.x, g1v.y + I.dt * gx1.y, g1v.z + I.dt * gx1.z};
-                gC = G1.C(p);
+                for (int d = 0; d < 3; ++d) {
+                    st.base[d] = 0; st.fx[d] = 1.0f;
+                    for (int n = 0; n < 3; ++n) { st.w[n][d] = 0.f; st.dw[n][d] = 0.f; }
+                }
             }
+            const int c0 = act ? tile_cell(B, st.base[0], st.base[1], st.base[2]) : 0;
             float gfx[3] = {0.f, 0.f, 0.f};
-            const float c4 = 4.0f * I.inv_dx;
 #pragma unroll
             for (int i = 0; i < 3; ++i)
 #pragma unroll
                 for (int j = 0; j < 3; ++j)
 #pragma unroll
                     for (int k = 0; k < 3; ++k) {
-                        const int ci = st.base[0] + i, cj = st.base[1] + j, ck = st.base[2] + k;
-                        float4 gvo = sim.vout[cell_index(I, ci, cj, ck)];
-                        float W0 = st.w[i][0], W1 = st.w[j][1], W2 = st.w[k][2];
-                        float w = W0 * W1 * W2;
-                        float dp[3] = {(float)i - st.fx[0], (float)j - st.fx[1], (float)k - st.fx[2]};
-                        float g[3] = {gvo.x, gvo.y, gvo.z};
-                        float gCdp[3], gCtg[3];
-#pragma unroll
-                        for (int a = 0; a < 3; ++a) {
-                            gCdp[a] = gC(a, 0) * dp[0] + gC(a, 1) * dp[1] + gC(a, 2) * dp[2];
-                            gCtg[a] = gC(0, a) * g[0] + gC(1, a) * g[1] + gC(2, a) * g[2];
+                        float4 gvo = act ? sim.vout[cell_index(I, st.base[0] + i, st.base[1] + j, st.base[2] + k)]
+                                         : make_float4(0.f, 0.f, 0.f, 0.f);
+                        float gg[3];
+                        g2p_adj_node(I, st, i, j, k, gvo, gv, gC, gg, gfx);
+                        gg[0] = seg_sum(gg[0], gi);
+                        gg[1] = seg_sum(gg[1], gi);
+                        gg[2] = seg_sum(gg[2], gi);
+                        if (gi.tail) {
+                            const int c = c0 + (i * B.e1 + j) * B.e2 + k;
+                            float4 t = T[c];
+                            t.y += gg[0]; t.z += gg[1]; t.w += gg[2];
+                            T[c] = t;
                         }
-                        float s1 = c4 * w;
-                        float gg0 = w * gv.x + s1 * gCdp[0];
-                        float gg1 = w * gv.y + s1 * gCdp[1];
-                        float gg2 = w * gv.z + s1 * gCdp[2];
-                        if (!spill) {
-                            int c = ((ci - l0) * e1 + (cj - l1)) * e2 + (ck - l2);
-                            atomicAdd(&tx[c], gg0); atomicAdd(&ty[c], gg1); atomicAdd(&tz[c], gg2);
-                        } else {
-                            long long gi = cell_index(I, ci, cj, ck);
-                            atomicAdd(&sim.gacc[gi].y, gg0); atomicAdd(&sim.gacc[gi].z, gg1); atomicAdd(&sim.gacc[gi].w, gg2);
-                        }
-                        float scal = gv.x * g[0] + gv.y * g[1] + gv.z * g[2] +
-                                     c4 * (g[0] * gCdp[0] + g[1] * gCdp[1] + g[2] * gCdp[2]);
-                        float dW0 = st.dw[i][0] * W1 * W2, dW1 = W0 * st.dw[j][1] * W2, dW2 = W0 * W1 * st.dw[k][2];
-                        gfx[0] += dW0 * scal - s1 * gCtg[0];
-                        gfx[1] += dW1 * scal - s1 * gCtg[1];
-                        gfx[2] += dW2 * scal - s1 * gCtg[2];
                     }
-            V3<float> gxt = {I.inv_dx * gfx[0], I.inv_dx * gfx[1], I.inv_dx * gfx[2]};
-            if (P.cls == CLS_ELEMENT) {
-                store_v3(sim.g_xe, sim.NE, P.start + q, gxt);
-            } else {
-                store_v3(Gout.px(0), sim.NP, P.start + q, gx1 + gxt);
-            }
+            if (act) g2p_adj_store(sim, I, Gout, P, q, gx1, gfx);
         }
         __syncthreads();
-        if (!spill) {
-            for (int c = tid; c < vol; c += DCP_WG) {
-                float a0 = tx[c], a1 = ty[c], a2 = tz[c];
-                if (a0 != 0.f || a1 != 0.f || a2 != 0.f) {
-                    int ck = c % e2, t = c / e2;
-                    int cj = t % e1, ci = t / e1;
-                    long long gi = cell_index(I, ci + l0, cj + l1, ck + l2);
-                    atomicAdd(&sim.gacc[gi].y, a0); atomicAdd(&sim.gacc[gi].z, a1); atomicAdd(&sim.gacc[gi].w, a2);
-                }
+        for (int c = tid; c < B.vol; c += DCP_WG) {
+            float4 a = tiles[0][c];
+#pragma unroll
+            for (int w = 1; w < DCP_WG / 64; ++w) {
+                float4 b = tiles[w][c];
+                a.y += b.y; a.z += b.z; a.w += b.w;
+            }
+            if (a.y != 0.f || a.z != 0.f || a.w != 0.f) {
+                int ck = c % B.e2, t = c / B.e2;
+                int cj = t % B.e1, ci = t / B.e1;
+                long long gidx = cell_index(I, ci + B.l0, cj + B.l1, ck + B.l2);
+                atomicAdd(&sim.gacc[gidx].y, a.y); atomicAdd(&sim.gacc[gidx].z, a.z); atomicAdd(&sim.gacc[gidx].w, a.w);
             }
         }
         __syncthreads();
@@ -136,7 +182,7 @@ __global__ __launch_bounds__(DCP_WG) void k_gridop_adj(SimDev sim) {
     if (n > sim.max_active) n = sim.max_active;
     for (int it = blockIdx.x * (DCP_WG / 64) + wv; it < n; it += gridDim.x * (DCP_WG / 64)) {
         const int2 a = sim.active[it];
-        const InstDev& I = sim.inst[a.x];
+        const InstDev I = sim.inst[a.x];
         const long long cell = I.grid_base + (long long)a.y * 64 + lane;
         int ci, cj, ck;
         block_cell_coords(I, a.y, lane, ci, cj, ck);
@@ -169,7 +215,7 @@ __global__ __launch_bounds__(DCP_WG) void k_p2g_adj(SimDev sim, StateView S, Sta
     const float ddw[3] = {1.0f, -2.0f, 1.0f};
     for (int pi = blockIdx.x; pi < sim.n_patches; pi += gridDim.x) {
         const Patch P = sim.patches[pi];
-        const InstDev& I = sim.inst[P.inst];
+        const InstDev I = sim.inst[P.inst];
         for (int q = tid; q < P.count; q += DCP_WG) {
             P2GRec r;
             p2g_record(sim, I, S, P, q, s, g, r);
@@ -268,7 +314,7 @@ __global__ __launch_bounds__(DCP_WG) void k_prep_adj(SimDev sim, StateView S, St
     for (int pi = blockIdx.x; pi < sim.n_patches; pi += gridDim.x) {
         const Patch Pt = sim.patches[pi];
         if (Pt.cls != CLS_ELEMENT) continue;
-        const InstDev& I = sim.inst[Pt.inst];
+        const InstDev I = sim.inst[Pt.inst];
         double acc[4] = {0.0, 0.0, 0.0, 0.0};
         for (int q = tid; q < Pt.count; q += DCP_WG) {
             const int e = Pt.start + q;

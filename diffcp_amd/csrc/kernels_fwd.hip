@@ -16,6 +16,7 @@
 //   k_g2p      per patch: APIC gather, x/v/C update, element C damping
 //              (simulation.py:304-383)
 #include "kernels.hpp"
+#include "patch_sort.hpp"
 
 // ---------------------------------------------------------------------------
 // element prep
@@ -25,7 +26,7 @@ __global__ __launch_bounds__(DCP_WG) void k_prep(SimDev sim, StateView S, StateV
     if (blockIdx.x == 0 && threadIdx.x == 0) *sim.n_active = 0;
     const int NE = sim.NE;
     for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < NE; e += gridDim.x * blockDim.x) {
-        const InstDev& I = sim.inst[sim.elem_inst[e]];
+        const InstDev I = sim.inst[sim.elem_inst[e]];
         ElemKin K;
         element_kinematics(sim, I, S, e, fresh, K);
         V3<float> v1 = S.v(K.i1), v2 = S.v(K.i2), v3 = S.v(K.i3);
@@ -67,42 +68,71 @@ __device__ __forceinline__ void mark_block(const SimDev& sim, const InstDev& I, 
     }
 }
 
+// Particles of a patch that does not fit the sorted tile scatter straight into
+// the grid with float atomics (slow path, flagged in the status word).
+__device__ __noinline__ void p2g_spill(const SimDev& sim, const StateView& S, const Patch& P, const InstDev& I, int s,
+                                       int g) {
+    for (int q = threadIdx.x; q < P.count; q += DCP_WG) {
+        P2GRec r;
+        p2g_record(sim, I, S, P, q, s, g, r);
+        Stencil st;
+        make_stencil(r.x, I.inv_dx, I.G, st, sim.status);
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j)
+                for (int k = 0; k < 3; ++k) {
+                    float mm, m0, m1, m2;
+                    p2g_contrib(I, st, r, i, j, k, mm, m0, m1, m2);
+                    int ci = st.base[0] + i, cj = st.base[1] + j, ck = st.base[2] + k;
+                    long long gi = cell_index(I, ci, cj, ck);
+                    atomicAdd(&sim.acc[gi].x, mm); atomicAdd(&sim.acc[gi].y, m0);
+                    atomicAdd(&sim.acc[gi].z, m1); atomicAdd(&sim.acc[gi].w, m2);
+                    mark_block(sim, I, P.inst, ci, cj, ck);
+                }
+    }
+}
+
+// P2G of one patch: stable counting sort by base cell, per stencil node a
+// wave-level segmented sum over lanes sharing the base cell, and a plain LDS
+// read-modify-write by the last lane of each run into the wave's private tile
+// (lanes of a run write nothing; runs of a wave have distinct base cells, so a
+// wave's writes never collide).  The four wave tiles are summed in wave order
+// and flushed once into the instance grid.
 __global__ __launch_bounds__(DCP_WG) void k_p2g(SimDev sim, StateView S, int s, int g) {
-    __shared__ float tm[DCP_TILE_CAP], tx[DCP_TILE_CAP], ty[DCP_TILE_CAP], tz[DCP_TILE_CAP];
-    __shared__ int sbox[6];
-    const int tid = threadIdx.x;
+    __shared__ SortLDS L;
+    __shared__ float4 tiles[DCP_WG / 64][DCP_STILE_CAP];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     for (int pi = blockIdx.x; pi < sim.n_patches; pi += gridDim.x) {
         const Patch P = sim.patches[pi];
-        const InstDev& I = sim.inst[P.inst];
-        if (tid < 3) { sbox[tid] = 1 << 30; sbox[3 + tid] = -(1 << 30); }
-        __syncthreads();
-        int lo[3] = {1 << 30, 1 << 30, 1 << 30}, hi[3] = {-(1 << 30), -(1 << 30), -(1 << 30)};
-        for (int q = tid; q < P.count; q += DCP_WG) {
-            V3<float> x = particle_x(sim, S, P, q);
-            Stencil st;
-            make_stencil(x, I.inv_dx, I.G, st, nullptr);
-            for (int d = 0; d < 3; ++d) { lo[d] = min(lo[d], st.base[d]); hi[d] = max(hi[d], st.base[d]); }
+        const InstDev I = sim.inst[P.inst];
+        PatchBox B;
+        patch_bbox(sim, S, P, I, L, B);
+        if (B.vol > DCP_STILE_CAP || P.count > DCP_SORT_MAX) {
+            if (tid == 0) atomicOr(sim.status, (unsigned)DCP_STATUS_TILE_SPILL);
+            p2g_spill(sim, S, P, I, s, g);
+            __syncthreads();
+            continue;
         }
-        for (int d = 0; d < 3; ++d) {
-            int a = wave_min(lo[d]), b = wave_max(hi[d]);
-            if ((tid & 63) == 0) { atomicMin(&sbox[d], a); atomicMax(&sbox[3 + d], b); }
-        }
-        __syncthreads();
-        const int l0 = sbox[0], l1 = sbox[1], l2 = sbox[2];
-        const int e0 = sbox[3] - l0 + 3, e1 = sbox[4] - l1 + 3, e2 = sbox[5] - l2 + 3;
-        const int vol = e0 * e1 * e2;
-        const bool spill = vol > DCP_TILE_CAP;
-        if (!spill) {
-            for (int c = tid; c < vol; c += DCP_WG) { tm[c] = 0.f; tx[c] = 0.f; ty[c] = 0.f; tz[c] = 0.f; }
-        } else if (tid == 0) {
-            atomicOr(sim.status, (unsigned)DCP_STATUS_TILE_SPILL);
-        }
-        __syncthreads();
-        for (int q = tid; q < P.count; q += DCP_WG) {
+        patch_sort(sim, S, P, I, B, L);
+        float4* T = tiles[wv];
+        for (int c = lane; c < B.vol; c += 64) T[c] = make_float4(0.f, 0.f, 0.f, 0.f);
+        for (int sp0 = 0; sp0 < P.count; sp0 += DCP_WG) {
+            const int sp = sp0 + tid;
+            const bool act = sp < P.count;
+            SegInfo gi = segment_info(L, sp, P.count);
             P2GRec r;
-            p2g_record(sim, I, S, P, q, s, g, r);
             Stencil st;
-            make_stencil(r.x, I.inv_dx, I.G, st, sim.status);
+            if (act) {
+                p2g_record(sim, I, S, P, L.perm[sp], s, g, r);
+                make_stencil(r.x, I.inv_dx, I.G, st, sim.status);
+            } else {
+                r.m = 0.f; r.b = {0.f, 0.f, 0.f}; r.A = mzero<float>(); r.elem = false;
+                r.n = {0.f, 0.f, 0.f}; r.k = {0.f, 0.f, 0.f}; r.x = {0.f, 0.f, 0.f};
+                for (int d = 0; d < 3; ++d) {
+                    st.base[d] = 0; st.fx[d] = 1.0f;
+                    for (int n = 0; n < 3; ++n) { st.w[n][d] = 0.f; st.dw[n][d] = 0.f; }
+                }
+            }
+            const int c0 = act ? tile_cell(B, st.base[0], st.base[1], st.base[2]) : 0;
 #pragma unroll
             for (int i = 0; i < 3; ++i)
 #pragma unroll
@@ -111,31 +141,34 @@ __global__ __launch_bounds__(DCP_WG) void k_p2g(SimDev sim, StateView S, int s, 
                     for (int k = 0; k < 3; ++k) {
                         float mm, m0, m1, m2;
                         p2g_contrib(I, st, r, i, j, k, mm, m0, m1, m2);
-                        int ci = st.base[0] + i, cj = st.base[1] + j, ck = st.base[2] + k;
-                        if (!spill) {
-                            int c = ((ci - l0) * e1 + (cj - l1)) * e2 + (ck - l2);
-                            atomicAdd(&tm[c], mm); atomicAdd(&tx[c], m0); atomicAdd(&ty[c], m1); atomicAdd(&tz[c], m2);
-                        } else {
-                            long long gi = cell_index(I, ci, cj, ck);
-                            atomicAdd(&sim.acc[gi].x, mm); atomicAdd(&sim.acc[gi].y, m0);
-                            atomicAdd(&sim.acc[gi].z, m1); atomicAdd(&sim.acc[gi].w, m2);
-                            mark_block(sim, I, P.inst, ci, cj, ck);
+                        mm = seg_sum(mm, gi);
+                        m0 = seg_sum(m0, gi);
+                        m1 = seg_sum(m1, gi);
+                        m2 = seg_sum(m2, gi);
+                        if (gi.tail) {
+                            const int c = c0 + (i * B.e1 + j) * B.e2 + k;
+                            float4 t = T[c];
+                            t.x += mm; t.y += m0; t.z += m1; t.w += m2;
+                            T[c] = t;
                         }
                     }
         }
         __syncthreads();
-        if (!spill) {
-            for (int c = tid; c < vol; c += DCP_WG) {
-                float mm = tm[c];
-                if (mm != 0.f || tx[c] != 0.f || ty[c] != 0.f || tz[c] != 0.f) {
-                    int ck = c % e2, t = c / e2;
-                    int cj = t % e1, ci = t / e1;
-                    ci += l0; cj += l1; ck += l2;
-                    long long gi = cell_index(I, ci, cj, ck);
-                    atomicAdd(&sim.acc[gi].x, mm); atomicAdd(&sim.acc[gi].y, tx[c]);
-                    atomicAdd(&sim.acc[gi].z, ty[c]); atomicAdd(&sim.acc[gi].w, tz[c]);
-                    mark_block(sim, I, P.inst, ci, cj, ck);
-                }
+        for (int c = tid; c < B.vol; c += DCP_WG) {
+            float4 a = tiles[0][c];
+#pragma unroll
+            for (int w = 1; w < DCP_WG / 64; ++w) {
+                float4 b = tiles[w][c];
+                a.x += b.x; a.y += b.y; a.z += b.z; a.w += b.w;
+            }
+            if (a.x != 0.f || a.y != 0.f || a.z != 0.f || a.w != 0.f) {
+                int ck = c % B.e2, t = c / B.e2;
+                int cj = t % B.e1, ci = t / B.e1;
+                ci += B.l0; cj += B.l1; ck += B.l2;
+                long long gi = cell_index(I, ci, cj, ck);
+                atomicAdd(&sim.acc[gi].x, a.x); atomicAdd(&sim.acc[gi].y, a.y);
+                atomicAdd(&sim.acc[gi].z, a.z); atomicAdd(&sim.acc[gi].w, a.w);
+                mark_block(sim, I, P.inst, ci, cj, ck);
             }
         }
         __syncthreads();
@@ -151,7 +184,7 @@ __global__ __launch_bounds__(DCP_WG) void k_gridop(SimDev sim, int keep) {
     if (n > sim.max_active) n = sim.max_active;
     for (int it = blockIdx.x * (DCP_WG / 64) + wv; it < n; it += gridDim.x * (DCP_WG / 64)) {
         const int2 a = sim.active[it];
-        const InstDev& I = sim.inst[a.x];
+        const InstDev I = sim.inst[a.x];
         const long long cell = I.grid_base + (long long)a.y * 64 + lane;
         int ci, cj, ck;
         block_cell_coords(I, a.y, lane, ci, cj, ck);
@@ -174,7 +207,7 @@ __global__ __launch_bounds__(DCP_WG) void k_g2p(SimDev sim, StateView S, StateVi
     const int tid = threadIdx.x;
     for (int pi = blockIdx.x; pi < sim.n_patches; pi += gridDim.x) {
         const Patch P = sim.patches[pi];
-        const InstDev& I = sim.inst[P.inst];
+        const InstDev I = sim.inst[P.inst];
         for (int q = tid; q < P.count; q += DCP_WG) {
             V3<float> x = particle_x(sim, S, P, q);
             Stencil st;
@@ -233,7 +266,7 @@ __global__ __launch_bounds__(DCP_WG) void k_loss_min(SimDev sim, StateView S, co
     const int tid = threadIdx.x;
     for (int ci = blockIdx.x; ci < n_chunks; ci += gridDim.x) {
         const int4 ch = tchunks[ci];            // (instance, first target, count, total targets)
-        const InstDev& I = sim.inst[ch.x];
+        const InstDev I = sim.inst[ch.x];
         const int t = ch.y + tid;
         const bool act = tid < ch.z;
         float t0 = 0.f, t1 = 0.f, t2 = 0.f;
@@ -284,7 +317,7 @@ __global__ __launch_bounds__(DCP_WG) void k_loss_sum(const float* tmin, const in
 __global__ void k_binning_keys(SimDev sim, StateView S, int* keys) {
     for (int pi = blockIdx.x; pi < sim.n_patches; pi += gridDim.x) {
         const Patch P = sim.patches[pi];
-        const InstDev& I = sim.inst[P.inst];
+        const InstDev I = sim.inst[P.inst];
         for (int q = threadIdx.x; q < P.count; q += blockDim.x) {
             V3<float> x = particle_x(sim, S, P, q);
             int idx = (P.cls == CLS_ELEMENT) ? sim.NP + P.start + q : P.start + q;

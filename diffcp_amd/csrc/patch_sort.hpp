@@ -1,0 +1,184 @@
+// Per-patch binning: bounding box of the base cells, stable LDS counting sort
+// of the patch's particles by base cell, and the wave-level segmented
+// reduction used by the P2G-type scatters.
+//
+// The base cell of a particle is trunc(x * inv_dx - 0.5) per axis
+// (simulation.py:140); its key is the linear index of that cell inside the
+// patch's bounding box.  The sort is a counting sort (histogram, exclusive
+// scan, scatter) whose ranks are stable: inside a wave the rank of a lane is
+// the number of lower lanes with the same key (ballot match), and the waves of
+// a round add their counts in wave order.  The resulting order equals a stable
+// argsort of the keys, so lanes holding the same base cell are contiguous.
+#pragma once
+#include "kernels.hpp"
+
+#define DCP_SORT_MAX 1024          // particles per sorted patch (= DCP_PATCH_MAX)
+#define DCP_STILE_CAP 1024         // cells of a sorted-scatter tile (wave-private copies)
+
+struct PatchBox {
+    int l0, l1, l2, e0, e1, e2, vol;
+};
+
+struct SortLDS {
+    int box[6];
+    int cnt[DCP_STILE_CAP];                    // per-key counts, then exclusive offsets
+    unsigned short key[DCP_SORT_MAX];          // key of particle q (input order)
+    unsigned short rank[DCP_SORT_MAX];         // stable rank of q among its key
+    unsigned short perm[DCP_SORT_MAX];         // sorted position -> q
+    unsigned short skey[DCP_SORT_MAX];         // sorted position -> key
+    int wsum[DCP_WG / 64];
+};
+
+__device__ __forceinline__ int stencil_base(float x, float inv_dx, int G) {
+    int b = (int)(x * inv_dx - 0.5f);
+    if (!(x == x) || b < 0) b = 0;
+    if (b > G - 3) b = G - 3;
+    return b;
+}
+
+__device__ __forceinline__ void patch_bbox(const SimDev& sim, const StateView& S, const Patch& P, const InstDev& I,
+                                           SortLDS& L, PatchBox& B) {
+    const int tid = threadIdx.x;
+    if (tid < 3) { L.box[tid] = 1 << 30; L.box[3 + tid] = -(1 << 30); }
+    __syncthreads();
+    int lo[3] = {1 << 30, 1 << 30, 1 << 30}, hi[3] = {-(1 << 30), -(1 << 30), -(1 << 30)};
+    for (int q = tid; q < P.count; q += DCP_WG) {
+        V3<float> x = particle_x(sim, S, P, q);
+        int b0 = stencil_base(x.x, I.inv_dx, I.G), b1 = stencil_base(x.y, I.inv_dx, I.G),
+            b2 = stencil_base(x.z, I.inv_dx, I.G);
+        lo[0] = min(lo[0], b0); lo[1] = min(lo[1], b1); lo[2] = min(lo[2], b2);
+        hi[0] = max(hi[0], b0); hi[1] = max(hi[1], b1); hi[2] = max(hi[2], b2);
+    }
+    for (int d = 0; d < 3; ++d) {
+        int a = wave_min(lo[d]), b = wave_max(hi[d]);
+        if ((tid & 63) == 0) { atomicMin(&L.box[d], a); atomicMax(&L.box[3 + d], b); }
+    }
+    __syncthreads();
+    B.l0 = L.box[0]; B.l1 = L.box[1]; B.l2 = L.box[2];
+    B.e0 = L.box[3] - B.l0 + 3; B.e1 = L.box[4] - B.l1 + 3; B.e2 = L.box[5] - B.l2 + 3;
+    B.vol = B.e0 * B.e1 * B.e2;
+    __syncthreads();
+}
+
+// key of a base cell inside the patch tile (cells index the tile incl. the +2 halo)
+__device__ __forceinline__ int tile_cell(const PatchBox& B, int i, int j, int k) {
+    return ((i - B.l0) * B.e1 + (j - B.l1)) * B.e2 + (k - B.l2);
+}
+
+// Stable counting sort of the patch's particles by tile cell of their base.
+// Requires B.vol <= DCP_STILE_CAP and P.count <= DCP_SORT_MAX.
+__device__ __forceinline__ void patch_sort(const SimDev& sim, const StateView& S, const Patch& P, const InstDev& I,
+                                           const PatchBox& B, SortLDS& L) {
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    for (int c = tid; c < B.vol; c += DCP_WG) L.cnt[c] = 0;
+    __syncthreads();
+    const unsigned long long lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    for (int r0 = 0; r0 < P.count; r0 += DCP_WG) {
+        const int q = r0 + tid;
+        const bool act = q < P.count;
+        int key = -1;
+        if (act) {
+            V3<float> x = particle_x(sim, S, P, q);
+            key = tile_cell(B, stencil_base(x.x, I.inv_dx, I.G), stencil_base(x.y, I.inv_dx, I.G),
+                            stencil_base(x.z, I.inv_dx, I.G));
+            L.key[q] = (unsigned short)key;
+        }
+        // rank among lower lanes of this wave with the same key, and the group size
+        unsigned long long todo = __ballot(act);
+        int wrank = 0, wcount = 0;
+        bool leader = false;
+        while (todo) {
+            int first = __ffsll((long long)todo) - 1;
+            int k0 = __shfl(key, first, 64);
+            unsigned long long grp = __ballot(act && key == k0);
+            if (act && key == k0) {
+                wrank = __popcll(grp & lt_mask);
+                wcount = __popcll(grp);
+                leader = (lane == first);
+            }
+            todo &= ~grp;
+        }
+        // waves of the round add their counts in wave order
+        for (int w = 0; w < DCP_WG / 64; ++w) {
+            if (wv == w && act) {
+                int old = L.cnt[key];
+                L.rank[q] = (unsigned short)(old + wrank);
+                if (leader) L.cnt[key] = old + wcount;
+            }
+            __syncthreads();
+        }
+    }
+    // exclusive scan of the counts (vol <= DCP_STILE_CAP = 4 per thread)
+    int v[4], s = 0;
+    for (int t = 0; t < 4; ++t) {
+        int c = tid * 4 + t;
+        v[t] = c < B.vol ? L.cnt[c] : 0;
+        s += v[t];
+    }
+    int incl = s;
+    for (int o = 1; o < 64; o <<= 1) {
+        int y = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += y;
+    }
+    if (lane == 63) L.wsum[wv] = incl;
+    __syncthreads();
+    int wbase = 0;
+    for (int w = 0; w < wv; ++w) wbase += L.wsum[w];
+    int run = wbase + incl - s;
+    for (int t = 0; t < 4; ++t) {
+        int c = tid * 4 + t;
+        if (c < B.vol) L.cnt[c] = run;
+        run += v[t];
+    }
+    __syncthreads();
+    for (int q = tid; q < P.count; q += DCP_WG) {
+        int k = L.key[q];
+        int pos = L.cnt[k] + L.rank[q];
+        L.perm[pos] = (unsigned short)q;
+        L.skey[pos] = (unsigned short)k;
+    }
+    __syncthreads();
+}
+
+// Per-lane segment description for sorted position sp of a wave: bit s of
+// `same` says that the lane 2^s below holds the same key; `tail` marks the last
+// lane of a run; `steps` is the wave-uniform number of scan steps needed.
+struct SegInfo {
+    unsigned same;
+    bool tail;
+    int steps;
+};
+
+__device__ __forceinline__ SegInfo segment_info(const SortLDS& L, int sp, int count) {
+    const int lane = threadIdx.x & 63;
+    const bool act = sp < count;
+    const int key = act ? L.skey[sp] : -1 - lane;
+    SegInfo g;
+    g.same = 0;
+    unsigned any = 0;
+#pragma unroll
+    for (int s = 0; s < 6; ++s) {
+        int d = 1 << s;
+        int kd = __shfl_up(key, d, 64);
+        bool same = act && lane >= d && kd == key;
+        if (same) g.same |= 1u << s;
+        if (__ballot(same)) any |= 1u << s;
+    }
+    int nxt = __shfl_down(key, 1, 64);
+    g.tail = act && (lane == 63 || nxt != key);
+    g.steps = 0;
+    while (g.steps < 6 && ((any >> g.steps) & 1u)) ++g.steps;
+    return g;
+}
+
+// inclusive segmented scan; after it the tail lane of each run holds the run sum
+__device__ __forceinline__ float seg_sum(float v, const SegInfo& g) {
+#pragma unroll
+    for (int s = 0; s < 6; ++s) {
+        if (s < g.steps) {
+            float t = __shfl_up(v, 1 << s, 64);
+            if ((g.same >> s) & 1u) v += t;
+        }
+    }
+    return v;
+}
