@@ -8,7 +8,8 @@ from __future__ import annotations
 import ctypes
 import os
 
-LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libdiffcp_amd.so")
+LIB_PATH = os.environ.get("DIFFCP_LIB") or os.path.join(os.path.dirname(os.path.abspath(__file__)),
+                                                        "libdiffcp_amd.so")
 
 c_int = ctypes.c_int32
 c_dbl_p = ctypes.POINTER(ctypes.c_double)

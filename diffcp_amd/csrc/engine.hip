@@ -147,7 +147,11 @@ struct dcp_ctx {
     }
     int grid_patches() const { return std::max(1, std::min(n_patches, n_cu * 8)); }
     int grid_elems() const { return std::max(1, std::min((NE + DCP_WG - 1) / DCP_WG, n_cu * 16)); }
-    int grid_gridop() const { return opt.grid_wgs > 0 ? opt.grid_wgs : n_cu * 4; }
+    // B x per workgroups walking the per-instance active block lists
+    int grid_gridop() const {
+        int total = opt.grid_wgs > 0 ? opt.grid_wgs : n_cu * 4;
+        return B * std::max(1, total / std::max(1, B));
+    }
 };
 
 static bool slot_ok(const dcp_ctx* c, int s) { return s >= 0 && s < c->max_states; }
@@ -345,7 +349,8 @@ static int build(dcp_ctx* c, const dcp_instance_desc* d, int B) {
         for (int i = 0; i < H.nc; ++i) c->vnat_h[I.c_start + i] = i;
     }
     // patches: point patches (vertex, grasper, cube) then element patches
-    int ps = c->opt.patch_size > 0 ? std::min(c->opt.patch_size, DCP_PATCH_MAX) : 512;
+    // particles per patch: the scatters stage up to 256 particle records in LDS
+    int ps = c->opt.patch_size > 0 ? std::min(c->opt.patch_size, 256) : 256;
     auto chunk = [&](int b, int cls, int start, int count) {
         for (int o = 0; o < count; o += ps) c->patches_h.push_back(Patch{b, cls, start + o, std::min(ps, count - o)});
     };
@@ -383,6 +388,7 @@ static int build(dcp_ctx* c, const dcp_instance_desc* d, int B) {
     S.B = B; S.NP = np; S.NE = ne; S.n_patches = c->n_patches;
     S.precision = c->opt.precision; S.svd_grad = c->opt.svd_grad; S.per_elem = c->opt.per_element_moduli;
     S.max_active = flags;
+    S.chunk = c->opt.p2g_mode > 0 ? std::min(c->opt.p2g_mode, 64) : 4;
     c->slot_floats = 15 * (size_t)np + 12 * (size_t)ne;
     c->max_states = std::max(2, c->opt.max_states);
     Patch* pd; int *trid, *eid, *iod, *ild; float* trd;
@@ -401,7 +407,7 @@ static int build(dcp_ctx* c, const dcp_instance_desc* d, int B) {
     if (c->alloc(&S.xe, 3 * (size_t)ne) || c->alloc(&S.ve, 3 * (size_t)ne) || c->alloc(&S.ke, 3 * (size_t)ne) ||
         c->alloc(&S.fc, 9 * (size_t)ne) || c->alloc(&S.nd, 3 * (size_t)ne) || c->alloc(&S.acc, (size_t)cells) ||
         c->alloc(&S.vout, (size_t)cells) || c->alloc(&S.gacc, (size_t)cells) || c->alloc(&S.gmv, (size_t)cells) ||
-        c->alloc(&S.flags, (size_t)flags) || c->alloc(&S.active, (size_t)flags) || c->alloc(&S.n_active, 1) ||
+        c->alloc(&S.flags, (size_t)flags) || c->alloc(&S.active, (size_t)flags) || c->alloc(&S.n_active, (size_t)B) ||
         c->alloc(&S.status, 1) || c->alloc(&S.g_xe, 3 * (size_t)ne) || c->alloc(&S.g_ve, 3 * (size_t)ne) ||
         c->alloc(&S.g_ne, 3 * (size_t)ne) || c->alloc(&S.g_ke, 3 * (size_t)ne) || c->alloc(&S.g_fv, 3 * (size_t)np) ||
         c->alloc(&S.ec_gx, 9 * (size_t)ne) || c->alloc(&S.ec_gv, 9 * (size_t)ne) ||

@@ -3,7 +3,7 @@
 #pragma once
 #include "sim_common.hpp"
 #include "element_math.hpp"
-#include "../../include/diffcp.h"
+#include "diffcp.h"
 
 // (1 - 0.95) evaluated in double then rounded, as the reference's constant
 // folding does (simulation.py:383)
@@ -217,6 +217,24 @@ __device__ __forceinline__ void block_cell_coords(const InstDev& I, int blk, int
     i = bi * 4 + (lane >> 4);
     j = bj * 4 + ((lane >> 2) & 3);
     k = bk * 4 + (lane & 3);
+}
+
+// Walk of the per-instance active block lists by a persistent grid of
+// B x `per` workgroups: workgroup (b, k) takes every (per * waves)-th entry of
+// instance b, one 4^3 block (64 cells) per wave.
+struct ActiveWalk {
+    int b, first, stride, n;
+};
+__device__ __forceinline__ bool active_walk(const SimDev& sim, const InstDev* inst, ActiveWalk& w) {
+    const int per = max(1, (int)gridDim.x / sim.B);
+    w.b = blockIdx.x / per;
+    if (w.b >= sim.B) return false;
+    const int nw = DCP_WG / 64;
+    w.first = (blockIdx.x % per) * nw + (threadIdx.x >> 6);
+    w.stride = per * nw;
+    const int nb = inst[w.b].nb;
+    w.n = min(sim.n_active[w.b], nb * nb * nb);
+    return true;
 }
 
 // v_out of one cell (simulation.py:277-302) with the damping extension;

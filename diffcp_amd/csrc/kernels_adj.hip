@@ -96,63 +96,131 @@ __device__ __noinline__ void g2p_adj_spill(const SimDev& sim, const StateView& S
 
 // G2P reverse of one patch: the per-particle dL/dx is a gather; the scatter of
 // dL/d grid_v_out uses the sorted segmented reduction of k_p2g.
-__global__ __launch_bounds__(DCP_WG) void k_g2p_adj(SimDev sim, StateView S, StateView G1, StateView Gout) {
+// 20-float (5 x float4) LDS record of a particle's G2P-reverse inputs:
+// [x gv0] [gv1 gv2 gx0 gx1] [gx2 C00 C01 C02] [C10 C11 C12 C20] [C21 C22 - -]
+__device__ __forceinline__ void stage_g2p(float4* rec, V3<float> x, V3<float> gv, V3<float> gx1, const M3<float>& gC) {
+    rec[0] = make_float4(x.x, x.y, x.z, gv.x);
+    rec[1] = make_float4(gv.y, gv.z, gx1.x, gx1.y);
+    rec[2] = make_float4(gx1.z, gC.m[0], gC.m[1], gC.m[2]);
+    rec[3] = make_float4(gC.m[3], gC.m[4], gC.m[5], gC.m[6]);
+    rec[4] = make_float4(gC.m[7], gC.m[8], 0.f, 0.f);
+}
+__device__ __forceinline__ void unstage_g2p(const float4* rec, V3<float>& x, V3<float>& gv, V3<float>& gx1, M3<float>& gC) {
+    float4 a = rec[0], b = rec[1], c = rec[2], d = rec[3], e = rec[4];
+    x = {a.x, a.y, a.z};
+    gv = {a.w, b.x, b.y};
+    gx1 = {b.z, b.w, c.x};
+    gC.m[0] = c.y; gC.m[1] = c.z; gC.m[2] = c.w;
+    gC.m[3] = d.x; gC.m[4] = d.y; gC.m[5] = d.z; gC.m[6] = d.w;
+    gC.m[7] = e.x; gC.m[8] = e.y;
+}
+
+__global__ __launch_bounds__(DCP_WG, 2) void k_g2p_adj(SimDev sim, StateView S, StateView G1, StateView Gout) {
     __shared__ SortLDS L;
     __shared__ float4 tiles[DCP_WG / 64][DCP_STILE_CAP];
+    __shared__ float4 rec[DCP_STAGE_MAX * 5];
+    __shared__ int pbase[DCP_STAGE_MAX];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     for (int pi = blockIdx.x; pi < sim.n_patches; pi += gridDim.x) {
         const Patch P = sim.patches[pi];
         const InstDev I = sim.inst[P.inst];
-        PatchBox B;
-        patch_bbox(sim, S, P, I, L, B);
-        if (B.vol > DCP_STILE_CAP || P.count > DCP_SORT_MAX) {
+        if (P.count > DCP_STAGE_MAX) {
             g2p_adj_spill(sim, S, G1, Gout, P, I);
             __syncthreads();
             continue;
         }
-        patch_sort(sim, S, P, I, B, L);
+        if (tid < 3) { L.box[tid] = 1 << 30; L.box[3 + tid] = -(1 << 30); }
+        __syncthreads();
+        int lo[3] = {1 << 30, 1 << 30, 1 << 30}, hi[3] = {-(1 << 30), -(1 << 30), -(1 << 30)};
+        for (int q = tid; q < P.count; q += DCP_WG) {
+            V3<float> x = particle_x(sim, S, P, q), gv, gx1;
+            M3<float> gC;
+            g2p_adj_inputs(sim, I, G1, P, q, gv, gx1, gC);
+            int b0 = stencil_base(x.x, I.inv_dx, I.G), b1 = stencil_base(x.y, I.inv_dx, I.G),
+                b2 = stencil_base(x.z, I.inv_dx, I.G);
+            pbase[q] = pack_base(b0, b1, b2);
+            stage_g2p(rec + 5 * q, x, gv, gx1, gC);
+            lo[0] = min(lo[0], b0); lo[1] = min(lo[1], b1); lo[2] = min(lo[2], b2);
+            hi[0] = max(hi[0], b0); hi[1] = max(hi[1], b1); hi[2] = max(hi[2], b2);
+        }
+        PatchBox B;
+        bbox_reduce(lo, hi, L, B);
+        if (B.vol > DCP_STILE_CAP) {
+            g2p_adj_spill(sim, S, G1, Gout, P, I);
+            __syncthreads();
+            continue;
+        }
+        for (int q = tid; q < P.count; q += DCP_WG) {
+            int b0, b1, b2;
+            unpack_base(pbase[q], b0, b1, b2);
+            L.key[q] = (unsigned short)tile_cell(B, b0, b1, b2);
+        }
+        __syncthreads();
+        patch_sort(sim, S, P, I, B, L, true);
+        patch_chunks(P, B, L, sim.chunk);
         float4* T = tiles[wv];
         for (int c = lane; c < B.vol; c += 64) T[c] = make_float4(0.f, 0.f, 0.f, 0.f);
-        for (int sp0 = 0; sp0 < P.count; sp0 += DCP_WG) {
-            const int sp = sp0 + tid;
-            const bool act = sp < P.count;
-            SegInfo gi = segment_info(L, sp, P.count);
-            const int q = act ? L.perm[sp] : 0;
-            Stencil st;
-            V3<float> gv = {0.f, 0.f, 0.f}, gx1 = {0.f, 0.f, 0.f};
-            M3<float> gC = mzero<float>();
-            if (act) {
-                make_stencil(particle_x(sim, S, P, q), I.inv_dx, I.G, st, nullptr);
-                g2p_adj_inputs(sim, I, G1, P, q, gv, gx1, gC);
-            } else {
-                for (int d = 0; d < 3; ++d) {
-                    st.base[d] = 0; st.fx[d] = 1.0f;
-                    for (int n = 0; n < 3; ++n) { st.w[n][d] = 0.f; st.dw[n][d] = 0.f; }
-                }
-            }
-            const int c0 = act ? tile_cell(B, st.base[0], st.base[1], st.base[2]) : 0;
-            float gfx[3] = {0.f, 0.f, 0.f};
+        const int nch = L.n_chunks;
+        for (int ch0 = 0; ch0 < nch; ch0 += DCP_WG) {
+            const int ci = ch0 + tid;
+            const bool act = ci < nch;
+            const int key = act ? (int)L.ch_key[ci] : 0;
+            SegInfo gi = segment_info_key(key, act);
+            // dL/dx of each particle is accumulated over all 27 nodes in the
+            // first pass; the node scatter runs in three passes (one per x-node)
+            const int s0 = act ? (int)L.ch_start[ci] : 0, len = act ? (int)L.ch_len[ci] : 0;
 #pragma unroll
-            for (int i = 0; i < 3; ++i)
+            for (int i = 0; i < 3; ++i) {
+                float acc[9][3];
+#pragma unroll
+                for (int o = 0; o < 9; ++o) acc[o][0] = acc[o][1] = acc[o][2] = 0.f;
+                for (int u = 0; u < len; ++u) {
+                    const int q = L.perm[s0 + u];
+                    V3<float> x, gv, gx1;
+                    M3<float> gC;
+                    unstage_g2p(rec + 5 * q, x, gv, gx1, gC);
+                    Stencil st;
+                    make_stencil(x, I.inv_dx, I.G, st, nullptr);
+                    float gfx[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+                    for (int j = 0; j < 3; ++j)
+#pragma unroll
+                        for (int k = 0; k < 3; ++k) {
+                            float4 gvo = sim.vout[cell_index(I, st.base[0] + i, st.base[1] + j, st.base[2] + k)];
+                            float gg[3];
+                            g2p_adj_node(I, st, i, j, k, gvo, gv, gC, gg, gfx);
+                            const int o = j * 3 + k;
+                            acc[o][0] += gg[0]; acc[o][1] += gg[1]; acc[o][2] += gg[2];
+                        }
+                    if (i == 0) {
+                        // the remaining x-nodes' share of dL/dfx, then the store
+#pragma unroll
+                        for (int i2 = 1; i2 < 3; ++i2)
+#pragma unroll
+                            for (int j = 0; j < 3; ++j)
+#pragma unroll
+                                for (int k = 0; k < 3; ++k) {
+                                    float4 gvo = sim.vout[cell_index(I, st.base[0] + i2, st.base[1] + j, st.base[2] + k)];
+                                    float gg[3];
+                                    g2p_adj_node(I, st, i2, j, k, gvo, gv, gC, gg, gfx);
+                                }
+                        g2p_adj_store(sim, I, Gout, P, q, gx1, gfx);
+                    }
+                }
 #pragma unroll
                 for (int j = 0; j < 3; ++j)
 #pragma unroll
                     for (int k = 0; k < 3; ++k) {
-                        float4 gvo = act ? sim.vout[cell_index(I, st.base[0] + i, st.base[1] + j, st.base[2] + k)]
-                                         : make_float4(0.f, 0.f, 0.f, 0.f);
-                        float gg[3];
-                        g2p_adj_node(I, st, i, j, k, gvo, gv, gC, gg, gfx);
-                        gg[0] = seg_sum(gg[0], gi);
-                        gg[1] = seg_sum(gg[1], gi);
-                        gg[2] = seg_sum(gg[2], gi);
+                        const int o = j * 3 + k;
+                        float a0 = seg_sum(acc[o][0], gi), a1 = seg_sum(acc[o][1], gi), a2 = seg_sum(acc[o][2], gi);
                         if (gi.tail) {
-                            const int c = c0 + (i * B.e1 + j) * B.e2 + k;
+                            const int c = key + (i * B.e1 + j) * B.e2 + k;
                             float4 t = T[c];
-                            t.y += gg[0]; t.z += gg[1]; t.w += gg[2];
+                            t.y += a0; t.z += a1; t.w += a2;
                             T[c] = t;
                         }
                     }
-            if (act) g2p_adj_store(sim, I, Gout, P, q, gx1, gfx);
+            }
         }
         __syncthreads();
         for (int c = tid; c < B.vol; c += DCP_WG) {
@@ -177,15 +245,16 @@ __global__ __launch_bounds__(DCP_WG) void k_g2p_adj(SimDev sim, StateView S, Sta
 // grid update adjoint
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(DCP_WG) void k_gridop_adj(SimDev sim) {
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    int n = *sim.n_active;
-    if (n > sim.max_active) n = sim.max_active;
-    for (int it = blockIdx.x * (DCP_WG / 64) + wv; it < n; it += gridDim.x * (DCP_WG / 64)) {
-        const int2 a = sim.active[it];
-        const InstDev I = sim.inst[a.x];
-        const long long cell = I.grid_base + (long long)a.y * 64 + lane;
+    const int lane = threadIdx.x & 63;
+    ActiveWalk w;
+    if (!active_walk(sim, sim.inst, w)) return;
+    const InstDev I = sim.inst[w.b];
+    double gdamp_wave = 0.0;
+    for (int it = w.first; it < w.n; it += w.stride) {
+        const int blk = sim.active[I.flag_base + it];
+        const long long cell = I.grid_base + (long long)blk * 64 + lane;
         int ci, cj, ck;
-        block_cell_coords(I, a.y, lane, ci, cj, ck);
+        block_cell_coords(I, blk, lane, ci, cj, ck);
         float4 c = sim.acc[cell];
         float4 ga = sim.gacc[cell];
         float v[3], mask[3], vr[3];
@@ -199,12 +268,11 @@ __global__ __launch_bounds__(DCP_WG) void k_gridop_adj(SimDev sim) {
         sim.gmv[cell] = make_float4(gm, inv_m * g0, inv_m * g1, inv_m * g2);
         sim.acc[cell] = make_float4(0.f, 0.f, 0.f, 0.f);
         sim.gacc[cell] = make_float4(0.f, 0.f, 0.f, 0.f);
-        double s = wave_sum_d(gdamp);
-        if (lane == 0) {
-            sim.flags[I.flag_base + a.y] = 0;
-            if (s != 0.0) atomicAdd(&sim.pgrad_inst[a.x * DCP_N_PARAMS + 4], s);
-        }
+        gdamp_wave += gdamp;
+        if (lane == 0) sim.flags[I.flag_base + blk] = 0;
     }
+    double s = wave_sum_d(gdamp_wave);
+    if (lane == 0 && s != 0.0) atomicAdd(&sim.pgrad_inst[w.b * DCP_N_PARAMS + 4], s);
 }
 
 // ---------------------------------------------------------------------------

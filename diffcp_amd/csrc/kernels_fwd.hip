@@ -23,7 +23,8 @@
 // ---------------------------------------------------------------------------
 template <typename PT>
 __global__ __launch_bounds__(DCP_WG) void k_prep(SimDev sim, StateView S, StateView Snext, int fresh) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) *sim.n_active = 0;
+    if (blockIdx.x == 0)
+        for (int b = threadIdx.x; b < sim.B; b += blockDim.x) sim.n_active[b] = 0;
     const int NE = sim.NE;
     for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < NE; e += gridDim.x * blockDim.x) {
         const InstDev I = sim.inst[sim.elem_inst[e]];
@@ -57,16 +58,33 @@ __global__ __launch_bounds__(DCP_WG) void k_prep(SimDev sim, StateView S, StateV
 // ---------------------------------------------------------------------------
 // P2G
 // ---------------------------------------------------------------------------
+// Mark the 4^3 block holding cell (i, j, k) active and append it to the
+// instance's list the first time (a stale 0 only costs a failed CAS).
 __device__ __forceinline__ void mark_block(const SimDev& sim, const InstDev& I, int b, int i, int j, int k) {
     int blk = ((i >> 2) * I.nb + (j >> 2)) * I.nb + (k >> 2);
     int* f = sim.flags + I.flag_base + blk;
-    if (__atomic_load_n(f, __ATOMIC_RELAXED) == 0) {
+    if (*f == 0) {
         if (atomicCAS(f, 0, 1) == 0) {
-            int slot = atomicAdd(sim.n_active, 1);
-            if (slot < sim.max_active) sim.active[slot] = make_int2(b, blk);
+            int slot = atomicAdd(sim.n_active + b, 1);
+            if (slot < I.nb * I.nb * I.nb) sim.active[I.flag_base + slot] = blk;
         }
     }
 }
+
+// tile blocks of a patch box (the 4^3 grid blocks it overlaps)
+struct TileBlocks {
+    int bx0, by0, bz0, nby, nbz, n;
+};
+__device__ __forceinline__ TileBlocks tile_blocks(int l0, int l1, int l2, int e0, int e1, int e2) {
+    TileBlocks t;
+    t.bx0 = l0 >> 2; t.by0 = l1 >> 2; t.bz0 = l2 >> 2;
+    int nbx = ((l0 + e0 - 1) >> 2) - t.bx0 + 1;
+    t.nby = ((l1 + e1 - 1) >> 2) - t.by0 + 1;
+    t.nbz = ((l2 + e2 - 1) >> 2) - t.bz0 + 1;
+    t.n = nbx * t.nby * t.nbz;
+    return t;
+}
+#define DCP_TB_CAP 256
 
 // Particles of a patch that does not fit the sorted tile scatter straight into
 // the grid with float atomics (slow path, flagged in the status word).
@@ -97,61 +115,123 @@ __device__ __noinline__ void p2g_spill(const SimDev& sim, const StateView& S, co
 // (lanes of a run write nothing; runs of a wave have distinct base cells, so a
 // wave's writes never collide).  The four wave tiles are summed in wave order
 // and flushed once into the instance grid.
-__global__ __launch_bounds__(DCP_WG) void k_p2g(SimDev sim, StateView S, int s, int g) {
+// 24-float (6 x float4) LDS record of a particle's P2G inputs:
+// [x m] [b n0] [A00 A01 A02 A10] [A11 A12 A20 A21] [A22 n1 n2 k0] [k1 k2 - -]
+__device__ __forceinline__ void stage_p2g(float4* rec, const P2GRec& r) {
+    rec[0] = make_float4(r.x.x, r.x.y, r.x.z, r.m);
+    rec[1] = make_float4(r.b.x, r.b.y, r.b.z, r.n.x);
+    rec[2] = make_float4(r.A.m[0], r.A.m[1], r.A.m[2], r.A.m[3]);
+    rec[3] = make_float4(r.A.m[4], r.A.m[5], r.A.m[6], r.A.m[7]);
+    rec[4] = make_float4(r.A.m[8], r.n.y, r.n.z, r.k.x);
+    rec[5] = make_float4(r.k.y, r.k.z, 0.f, 0.f);
+}
+__device__ __forceinline__ void unstage_p2g(const float4* rec, bool elem, P2GRec& r) {
+    float4 a = rec[0], b = rec[1], c = rec[2], d = rec[3], e = rec[4], f = rec[5];
+    r.x = {a.x, a.y, a.z}; r.m = a.w;
+    r.b = {b.x, b.y, b.z};
+    r.A.m[0] = c.x; r.A.m[1] = c.y; r.A.m[2] = c.z; r.A.m[3] = c.w;
+    r.A.m[4] = d.x; r.A.m[5] = d.y; r.A.m[6] = d.z; r.A.m[7] = d.w;
+    r.A.m[8] = e.x;
+    r.n = {b.w, e.y, e.z};
+    r.k = {e.w, f.x, f.y};
+    r.elem = elem;
+}
+
+__global__ __launch_bounds__(DCP_WG, 2) void k_p2g(SimDev sim, StateView S, int s, int g) {
     __shared__ SortLDS L;
     __shared__ float4 tiles[DCP_WG / 64][DCP_STILE_CAP];
+    __shared__ float4 rec[DCP_STAGE_MAX * 6];
+    __shared__ int pbase[DCP_STAGE_MAX];
+    __shared__ int tb_touched[DCP_TB_CAP];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     for (int pi = blockIdx.x; pi < sim.n_patches; pi += gridDim.x) {
         const Patch P = sim.patches[pi];
         const InstDev I = sim.inst[P.inst];
-        PatchBox B;
-        patch_bbox(sim, S, P, I, L, B);
-        if (B.vol > DCP_STILE_CAP || P.count > DCP_SORT_MAX) {
+        const bool elem = P.cls == CLS_ELEMENT;
+        if (P.count > DCP_STAGE_MAX) {
             if (tid == 0) atomicOr(sim.status, (unsigned)DCP_STATUS_TILE_SPILL);
             p2g_spill(sim, S, P, I, s, g);
             __syncthreads();
             continue;
         }
-        patch_sort(sim, S, P, I, B, L);
+        // stage every particle's record (all loads of the patch in flight together)
+        if (tid < 3) { L.box[tid] = 1 << 30; L.box[3 + tid] = -(1 << 30); }
+        __syncthreads();
+        int lo[3] = {1 << 30, 1 << 30, 1 << 30}, hi[3] = {-(1 << 30), -(1 << 30), -(1 << 30)};
+        for (int q = tid; q < P.count; q += DCP_WG) {
+            P2GRec r;
+            p2g_record(sim, I, S, P, q, s, g, r);
+            int b0 = stencil_base(r.x.x, I.inv_dx, I.G), b1 = stencil_base(r.x.y, I.inv_dx, I.G),
+                b2 = stencil_base(r.x.z, I.inv_dx, I.G);
+            pbase[q] = pack_base(b0, b1, b2);
+            stage_p2g(rec + 6 * q, r);
+            lo[0] = min(lo[0], b0); lo[1] = min(lo[1], b1); lo[2] = min(lo[2], b2);
+            hi[0] = max(hi[0], b0); hi[1] = max(hi[1], b1); hi[2] = max(hi[2], b2);
+        }
+        PatchBox B;
+        bbox_reduce(lo, hi, L, B);
+        if (B.vol > DCP_STILE_CAP) {
+            if (tid == 0) atomicOr(sim.status, (unsigned)DCP_STATUS_TILE_SPILL);
+            p2g_spill(sim, S, P, I, s, g);
+            __syncthreads();
+            continue;
+        }
+        const TileBlocks TB = tile_blocks(B.l0, B.l1, B.l2, B.e0, B.e1, B.e2);
+        for (int t = tid; t < TB.n && t < DCP_TB_CAP; t += DCP_WG) tb_touched[t] = 0;
+        for (int q = tid; q < P.count; q += DCP_WG) {
+            int b0, b1, b2;
+            unpack_base(pbase[q], b0, b1, b2);
+            L.key[q] = (unsigned short)tile_cell(B, b0, b1, b2);
+        }
+        __syncthreads();
+        patch_sort(sim, S, P, I, B, L, true);
+        patch_chunks(P, B, L, sim.chunk);
         float4* T = tiles[wv];
         for (int c = lane; c < B.vol; c += 64) T[c] = make_float4(0.f, 0.f, 0.f, 0.f);
-        for (int sp0 = 0; sp0 < P.count; sp0 += DCP_WG) {
-            const int sp = sp0 + tid;
-            const bool act = sp < P.count;
-            SegInfo gi = segment_info(L, sp, P.count);
-            P2GRec r;
-            Stencil st;
-            if (act) {
-                p2g_record(sim, I, S, P, L.perm[sp], s, g, r);
-                make_stencil(r.x, I.inv_dx, I.G, st, sim.status);
-            } else {
-                r.m = 0.f; r.b = {0.f, 0.f, 0.f}; r.A = mzero<float>(); r.elem = false;
-                r.n = {0.f, 0.f, 0.f}; r.k = {0.f, 0.f, 0.f}; r.x = {0.f, 0.f, 0.f};
-                for (int d = 0; d < 3; ++d) {
-                    st.base[d] = 0; st.fx[d] = 1.0f;
-                    for (int n = 0; n < 3; ++n) { st.w[n][d] = 0.f; st.dw[n][d] = 0.f; }
-                }
-            }
-            const int c0 = act ? tile_cell(B, st.base[0], st.base[1], st.base[2]) : 0;
+        const int nch = L.n_chunks;
+        for (int ch0 = 0; ch0 < nch; ch0 += DCP_WG) {
+            const int ci = ch0 + tid;
+            const bool act = ci < nch;
+            const int key = act ? (int)L.ch_key[ci] : 0;
+            SegInfo gi = segment_info_key(key, act);
+            // the lane's chunk: particles of one base cell, accumulated in registers
+            // three passes over the chunk, one per x-node i, keep 9 x 4 sums live
+            const int s0 = act ? (int)L.ch_start[ci] : 0, len = act ? (int)L.ch_len[ci] : 0;
 #pragma unroll
-            for (int i = 0; i < 3; ++i)
+            for (int i = 0; i < 3; ++i) {
+                float acc[9][4];
+#pragma unroll
+                for (int o = 0; o < 9; ++o) acc[o][0] = acc[o][1] = acc[o][2] = acc[o][3] = 0.f;
+                for (int u = 0; u < len; ++u) {
+                    P2GRec r;
+                    unstage_p2g(rec + 6 * (int)L.perm[s0 + u], elem, r);
+                    Stencil st;
+                    make_stencil(r.x, I.inv_dx, I.G, st, nullptr);
+#pragma unroll
+                    for (int j = 0; j < 3; ++j)
+#pragma unroll
+                        for (int k = 0; k < 3; ++k) {
+                            float mm, m0, m1, m2;
+                            p2g_contrib(I, st, r, i, j, k, mm, m0, m1, m2);
+                            const int o = j * 3 + k;
+                            acc[o][0] += mm; acc[o][1] += m0; acc[o][2] += m1; acc[o][3] += m2;
+                        }
+                }
 #pragma unroll
                 for (int j = 0; j < 3; ++j)
 #pragma unroll
                     for (int k = 0; k < 3; ++k) {
-                        float mm, m0, m1, m2;
-                        p2g_contrib(I, st, r, i, j, k, mm, m0, m1, m2);
-                        mm = seg_sum(mm, gi);
-                        m0 = seg_sum(m0, gi);
-                        m1 = seg_sum(m1, gi);
-                        m2 = seg_sum(m2, gi);
+                        const int o = j * 3 + k;
+                        float mm = seg_sum(acc[o][0], gi), m0 = seg_sum(acc[o][1], gi);
+                        float m1 = seg_sum(acc[o][2], gi), m2 = seg_sum(acc[o][3], gi);
                         if (gi.tail) {
-                            const int c = c0 + (i * B.e1 + j) * B.e2 + k;
+                            const int c = key + (i * B.e1 + j) * B.e2 + k;
                             float4 t = T[c];
                             t.x += mm; t.y += m0; t.z += m1; t.w += m2;
                             T[c] = t;
                         }
                     }
+            }
         }
         __syncthreads();
         for (int c = tid; c < B.vol; c += DCP_WG) {
@@ -168,7 +248,21 @@ __global__ __launch_bounds__(DCP_WG) void k_p2g(SimDev sim, StateView S, int s, 
                 long long gi = cell_index(I, ci, cj, ck);
                 atomicAdd(&sim.acc[gi].x, a.x); atomicAdd(&sim.acc[gi].y, a.y);
                 atomicAdd(&sim.acc[gi].z, a.z); atomicAdd(&sim.acc[gi].w, a.w);
-                mark_block(sim, I, P.inst, ci, cj, ck);
+                if (TB.n <= DCP_TB_CAP)
+                    tb_touched[((ci >> 2) - TB.bx0) * TB.nby * TB.nbz + ((cj >> 2) - TB.by0) * TB.nbz + ((ck >> 2) - TB.bz0)] = 1;
+                else
+                    mark_block(sim, I, P.inst, ci, cj, ck);
+            }
+        }
+        __syncthreads();
+        // one activation per touched grid block of the tile
+        if (TB.n <= DCP_TB_CAP) {
+            for (int t = tid; t < TB.n; t += DCP_WG) {
+                if (tb_touched[t]) {
+                    int bz = t % TB.nbz, r = t / TB.nbz;
+                    int by = r % TB.nby, bx = r / TB.nby;
+                    mark_block(sim, I, P.inst, (TB.bx0 + bx) * 4, (TB.by0 + by) * 4, (TB.bz0 + bz) * 4);
+                }
             }
         }
         __syncthreads();
@@ -179,15 +273,15 @@ __global__ __launch_bounds__(DCP_WG) void k_p2g(SimDev sim, StateView S, int s, 
 // grid update over the active blocks
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(DCP_WG) void k_gridop(SimDev sim, int keep) {
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    int n = *sim.n_active;
-    if (n > sim.max_active) n = sim.max_active;
-    for (int it = blockIdx.x * (DCP_WG / 64) + wv; it < n; it += gridDim.x * (DCP_WG / 64)) {
-        const int2 a = sim.active[it];
-        const InstDev I = sim.inst[a.x];
-        const long long cell = I.grid_base + (long long)a.y * 64 + lane;
+    const int lane = threadIdx.x & 63;
+    ActiveWalk w;
+    if (!active_walk(sim, sim.inst, w)) return;
+    const InstDev I = sim.inst[w.b];
+    for (int it = w.first; it < w.n; it += w.stride) {
+        const int blk = sim.active[I.flag_base + it];
+        const long long cell = I.grid_base + (long long)blk * 64 + lane;
         int ci, cj, ck;
-        block_cell_coords(I, a.y, lane, ci, cj, ck);
+        block_cell_coords(I, blk, lane, ci, cj, ck);
         float4 c = sim.acc[cell];
         float v[3];
         grid_velocity(I, c, ci, cj, ck, v, nullptr);
@@ -195,7 +289,7 @@ __global__ __launch_bounds__(DCP_WG) void k_gridop(SimDev sim, int keep) {
         if (!(isfinite(v[0]) && isfinite(v[1]) && isfinite(v[2]))) atomicOr(sim.status, (unsigned)DCP_STATUS_NONFINITE);
         if (!keep) {
             sim.acc[cell] = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (lane == 0) sim.flags[I.flag_base + a.y] = 0;
+            if (lane == 0) sim.flags[I.flag_base + blk] = 0;
         }
     }
 }

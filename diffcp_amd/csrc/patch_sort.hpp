@@ -12,11 +12,21 @@
 #pragma once
 #include "kernels.hpp"
 
-#define DCP_SORT_MAX 1024          // particles per sorted patch (= DCP_PATCH_MAX)
-#define DCP_STILE_CAP 1024         // cells of a sorted-scatter tile (wave-private copies)
+#define DCP_SORT_MAX 256           // particles per sorted patch
+#define DCP_STILE_CAP 512          // cells of a sorted-scatter tile (wave-private copies)
+#define DCP_STAGE_MAX 256          // particles of a patch staged in LDS by the scatters
 
 struct PatchBox {
     int l0, l1, l2, e0, e1, e2, vol;
+};
+
+// Per-lane segment description of a wave of sorted keys: bit s of `same` says
+// that the lane 2^s below holds the same key; `tail` marks the last lane of a
+// run; `steps` is the wave-uniform number of scan steps needed.
+struct SegInfo {
+    unsigned same;
+    bool tail;
+    int steps;
 };
 
 struct SortLDS {
@@ -26,7 +36,11 @@ struct SortLDS {
     unsigned short rank[DCP_SORT_MAX];         // stable rank of q among its key
     unsigned short perm[DCP_SORT_MAX];         // sorted position -> q
     unsigned short skey[DCP_SORT_MAX];         // sorted position -> key
+    unsigned short ch_start[DCP_SORT_MAX];     // chunk -> first sorted position
+    unsigned short ch_len[DCP_SORT_MAX];       // chunk -> particles (<= K)
+    unsigned short ch_key[DCP_SORT_MAX];       // chunk -> key
     int wsum[DCP_WG / 64];
+    int n_chunks;
 };
 
 __device__ __forceinline__ int stencil_base(float x, float inv_dx, int G) {
@@ -60,15 +74,35 @@ __device__ __forceinline__ void patch_bbox(const SimDev& sim, const StateView& S
     __syncthreads();
 }
 
+// Bounding box from per-thread min/max base cells (block reduction).
+__device__ __forceinline__ void bbox_reduce(int lo[3], int hi[3], SortLDS& L, PatchBox& B) {
+    const int tid = threadIdx.x;
+    for (int d = 0; d < 3; ++d) {
+        int a = wave_min(lo[d]), b = wave_max(hi[d]);
+        if ((tid & 63) == 0) { atomicMin(&L.box[d], a); atomicMax(&L.box[3 + d], b); }
+    }
+    __syncthreads();
+    B.l0 = L.box[0]; B.l1 = L.box[1]; B.l2 = L.box[2];
+    B.e0 = L.box[3] - B.l0 + 3; B.e1 = L.box[4] - B.l1 + 3; B.e2 = L.box[5] - B.l2 + 3;
+    B.vol = B.e0 * B.e1 * B.e2;
+    __syncthreads();
+}
+
+__device__ __forceinline__ int pack_base(int b0, int b1, int b2) { return (b0 << 20) | (b1 << 10) | b2; }
+__device__ __forceinline__ void unpack_base(int p, int& b0, int& b1, int& b2) {
+    b0 = p >> 20; b1 = (p >> 10) & 1023; b2 = p & 1023;
+}
+
 // key of a base cell inside the patch tile (cells index the tile incl. the +2 halo)
 __device__ __forceinline__ int tile_cell(const PatchBox& B, int i, int j, int k) {
     return ((i - B.l0) * B.e1 + (j - B.l1)) * B.e2 + (k - B.l2);
 }
 
 // Stable counting sort of the patch's particles by tile cell of their base.
-// Requires B.vol <= DCP_STILE_CAP and P.count <= DCP_SORT_MAX.
+// Requires B.vol <= DCP_STILE_CAP and P.count <= DCP_SORT_MAX.  With
+// keys_ready the caller has already written L.key[q] for every particle.
 __device__ __forceinline__ void patch_sort(const SimDev& sim, const StateView& S, const Patch& P, const InstDev& I,
-                                           const PatchBox& B, SortLDS& L) {
+                                           const PatchBox& B, SortLDS& L, bool keys_ready = false) {
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     for (int c = tid; c < B.vol; c += DCP_WG) L.cnt[c] = 0;
     __syncthreads();
@@ -78,10 +112,14 @@ __device__ __forceinline__ void patch_sort(const SimDev& sim, const StateView& S
         const bool act = q < P.count;
         int key = -1;
         if (act) {
-            V3<float> x = particle_x(sim, S, P, q);
-            key = tile_cell(B, stencil_base(x.x, I.inv_dx, I.G), stencil_base(x.y, I.inv_dx, I.G),
-                            stencil_base(x.z, I.inv_dx, I.G));
-            L.key[q] = (unsigned short)key;
+            if (keys_ready) {
+                key = L.key[q];
+            } else {
+                V3<float> x = particle_x(sim, S, P, q);
+                key = tile_cell(B, stencil_base(x.x, I.inv_dx, I.G), stencil_base(x.y, I.inv_dx, I.G),
+                                stencil_base(x.z, I.inv_dx, I.G));
+                L.key[q] = (unsigned short)key;
+            }
         }
         // rank among lower lanes of this wave with the same key, and the group size
         unsigned long long todo = __ballot(act);
@@ -108,7 +146,7 @@ __device__ __forceinline__ void patch_sort(const SimDev& sim, const StateView& S
             __syncthreads();
         }
     }
-    // exclusive scan of the counts (vol <= DCP_STILE_CAP = 4 per thread)
+    // exclusive scan of the counts (vol <= DCP_STILE_CAP <= 4 per thread)
     int v[4], s = 0;
     for (int t = 0; t < 4; ++t) {
         int c = tid * 4 + t;
@@ -140,14 +178,76 @@ __device__ __forceinline__ void patch_sort(const SimDev& sim, const StateView& S
     __syncthreads();
 }
 
-// Per-lane segment description for sorted position sp of a wave: bit s of
-// `same` says that the lane 2^s below holds the same key; `tail` marks the last
-// lane of a run; `steps` is the wave-uniform number of scan steps needed.
-struct SegInfo {
-    unsigned same;
-    bool tail;
-    int steps;
-};
+// Block-wide exclusive scan of one int per key (vol <= DCP_STILE_CAP, 4 keys
+// per thread); returns the total.  `vals` are this thread's 4 values.
+__device__ __forceinline__ int block_exscan4(int vals[4], int out[4], SortLDS& L) {
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    int s = vals[0] + vals[1] + vals[2] + vals[3];
+    int incl = s;
+    for (int o = 1; o < 64; o <<= 1) {
+        int y = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += y;
+    }
+    __syncthreads();
+    if (lane == 63) L.wsum[wv] = incl;
+    __syncthreads();
+    int wbase = 0, total = 0;
+    for (int w = 0; w < DCP_WG / 64; ++w) {
+        if (w < wv) wbase += L.wsum[w];
+        total += L.wsum[w];
+    }
+    int run = wbase + incl - s;
+    for (int t = 0; t < 4; ++t) { out[t] = run; run += vals[t]; }
+    return total;
+}
+
+// Cut every key's run of sorted particles into chunks of at most K; chunks of
+// one key are consecutive.  Uses the exclusive offsets left in L.cnt.
+__device__ __forceinline__ void patch_chunks(const Patch& P, const PatchBox& B, SortLDS& L, int K) {
+    const int tid = threadIdx.x;
+    int nch[4], off[4], cnt[4], choff[4];
+    for (int t = 0; t < 4; ++t) {
+        int c = tid * 4 + t;
+        off[t] = c < B.vol ? L.cnt[c] : 0;
+        int nxt = (c + 1 < B.vol) ? L.cnt[c + 1] : P.count;
+        cnt[t] = c < B.vol ? nxt - off[t] : 0;
+        nch[t] = (cnt[t] + K - 1) / K;
+    }
+    int total = block_exscan4(nch, choff, L);
+    for (int t = 0; t < 4; ++t) {
+        int c = tid * 4 + t;
+        for (int j = 0; j < nch[t]; ++j) {
+            int ci = choff[t] + j;
+            L.ch_start[ci] = (unsigned short)(off[t] + j * K);
+            L.ch_len[ci] = (unsigned short)min(K, cnt[t] - j * K);
+            L.ch_key[ci] = (unsigned short)c;
+        }
+    }
+    if (tid == 0) L.n_chunks = total;
+    __syncthreads();
+}
+
+// segment description over chunk keys (same as segment_info, keyed by chunk)
+__device__ __forceinline__ SegInfo segment_info_key(int key, bool act) {
+    const int lane = threadIdx.x & 63;
+    const int k = act ? key : -1 - lane;
+    SegInfo g;
+    g.same = 0;
+    unsigned any = 0;
+#pragma unroll
+    for (int s = 0; s < 6; ++s) {
+        int d = 1 << s;
+        int kd = __shfl_up(k, d, 64);
+        bool same = act && lane >= d && kd == k;
+        if (same) g.same |= 1u << s;
+        if (__ballot(same)) any |= 1u << s;
+    }
+    int nxt = __shfl_down(k, 1, 64);
+    g.tail = act && (lane == 63 || nxt != k);
+    g.steps = 0;
+    while (g.steps < 6 && ((any >> g.steps) & 1u)) ++g.steps;
+    return g;
+}
 
 __device__ __forceinline__ SegInfo segment_info(const SortLDS& L, int sp, int count) {
     const int lane = threadIdx.x & 63;

@@ -38,6 +38,7 @@ struct Patch {                     // a chunk of one particle class of one insta
 struct SimDev {
     int B, NP, NE, n_patches;
     int precision, svd_grad, per_elem, max_active;
+    int chunk;                     // particles of one base cell accumulated per lane in the scatters
     const InstDev* inst;
     const Patch* patches;          // point patches then element patches
     const int* tri;                // [3][NE] point indices
@@ -56,9 +57,9 @@ struct SimDev {
     float4* vout;                  // grid_v_out
     float4* gacc;                  // d/d grid_v_out accumulators
     float4* gmv;                   // (d/dm, d/dmv)
-    int* flags;                    // active block flags
-    int2* active;                  // (instance, block) list
-    int* n_active;
+    int* flags;                    // active block flags (per instance at flag_base)
+    int* active;                   // active block ids, per instance at flag_base
+    int* n_active;                 // [B] active blocks per instance
     unsigned* status;
     // adjoint temporaries
     float* g_xe; float* g_ve; float* g_ne; float* g_ke;  // [3][NE]
