@@ -171,9 +171,13 @@ static int launch_prep(dcp_ctx* c, StateView S, StateView Sn, int fresh) {
     return 0;
 }
 
+static void launch_p2g(dcp_ctx* c, StateView S, int s, int g) {
+    hipLaunchKernelGGL(k_p2g, dim3(c->grid_patches()), dim3(DCP_WG), 0, c->stream, c->sim, S, s, g);
+}
+
 static int launch_forward(dcp_ctx* c, int s, int g, StateView S, StateView Sn) {
     if (launch_prep(c, S, Sn, s == 0)) return -1;
-    hipLaunchKernelGGL(k_p2g, dim3(c->grid_patches()), dim3(DCP_WG), 0, c->stream, c->sim, S, s, g);
+    launch_p2g(c, S, s, g);
     LAUNCH_OK();
     hipLaunchKernelGGL(k_gridop, dim3(c->grid_gridop()), dim3(DCP_WG), 0, c->stream, c->sim, 0);
     LAUNCH_OK();
@@ -185,7 +189,7 @@ static int launch_forward(dcp_ctx* c, int s, int g, StateView S, StateView Sn) {
 static int launch_adjoint(dcp_ctx* c, int s, int g, StateView S, StateView G1, StateView Gout) {
     const int fresh = (s == 0);
     if (launch_prep(c, S, c->scratch_view(), fresh)) return -1;
-    hipLaunchKernelGGL(k_p2g, dim3(c->grid_patches()), dim3(DCP_WG), 0, c->stream, c->sim, S, s, g);
+    launch_p2g(c, S, s, g);
     LAUNCH_OK();
     hipLaunchKernelGGL(k_gridop, dim3(c->grid_gridop()), dim3(DCP_WG), 0, c->stream, c->sim, 1);
     LAUNCH_OK();

@@ -73,7 +73,7 @@ __device__ __forceinline__ void g2p_adj_store(const SimDev& sim, const InstDev& 
     else store_v3(Gout.px(0), sim.NP, P.start + q, gx1 + gxt);
 }
 
-__device__ __noinline__ void g2p_adj_spill(const SimDev& sim, const StateView& S, const StateView& G1,
+__device__ __forceinline__ void g2p_adj_spill(const SimDev& sim, const StateView& S, const StateView& G1,
                                            const StateView& Gout, const Patch& P, const InstDev& I) {
     for (int q = threadIdx.x; q < P.count; q += DCP_WG) {
         Stencil st;
@@ -165,15 +165,15 @@ __global__ __launch_bounds__(DCP_WG, 2) void k_g2p_adj(SimDev sim, StateView S, 
             const int ci = ch0 + tid;
             const bool act = ci < nch;
             const int key = act ? (int)L.ch_key[ci] : 0;
-            SegInfo gi = segment_info_key(key, act);
+            const SegInfo gi = segment_info_key(key, act);
             // dL/dx of each particle is accumulated over all 27 nodes in the
             // first pass; the node scatter runs in three passes (one per x-node)
             const int s0 = act ? (int)L.ch_start[ci] : 0, len = act ? (int)L.ch_len[ci] : 0;
 #pragma unroll
             for (int i = 0; i < 3; ++i) {
-                float acc[9][3];
+                float acc[27];
 #pragma unroll
-                for (int o = 0; o < 9; ++o) acc[o][0] = acc[o][1] = acc[o][2] = 0.f;
+                for (int o = 0; o < 27; ++o) acc[o] = 0.f;
                 for (int u = 0; u < len; ++u) {
                     const int q = L.perm[s0 + u];
                     V3<float> x, gv, gx1;
@@ -189,8 +189,8 @@ __global__ __launch_bounds__(DCP_WG, 2) void k_g2p_adj(SimDev sim, StateView S, 
                             float4 gvo = sim.vout[cell_index(I, st.base[0] + i, st.base[1] + j, st.base[2] + k)];
                             float gg[3];
                             g2p_adj_node(I, st, i, j, k, gvo, gv, gC, gg, gfx);
-                            const int o = j * 3 + k;
-                            acc[o][0] += gg[0]; acc[o][1] += gg[1]; acc[o][2] += gg[2];
+                            const int o = 3 * (j * 3 + k);
+                            acc[o] += gg[0]; acc[o + 1] += gg[1]; acc[o + 2] += gg[2];
                         }
                     if (i == 0) {
                         // the remaining x-nodes' share of dL/dfx, then the store
@@ -207,18 +207,19 @@ __global__ __launch_bounds__(DCP_WG, 2) void k_g2p_adj(SimDev sim, StateView S, 
                         g2p_adj_store(sim, I, Gout, P, q, gx1, gfx);
                     }
                 }
+                seg_sum_n(acc, gi);
 #pragma unroll
                 for (int j = 0; j < 3; ++j)
 #pragma unroll
                     for (int k = 0; k < 3; ++k) {
-                        const int o = j * 3 + k;
-                        float a0 = seg_sum(acc[o][0], gi), a1 = seg_sum(acc[o][1], gi), a2 = seg_sum(acc[o][2], gi);
+                        const int o = 3 * (j * 3 + k);
                         if (gi.tail) {
                             const int c = key + (i * B.e1 + j) * B.e2 + k;
                             float4 t = T[c];
-                            t.y += a0; t.z += a1; t.w += a2;
+                            t.y += acc[o]; t.z += acc[o + 1]; t.w += acc[o + 2];
                             T[c] = t;
                         }
+                        tile_order();
                     }
             }
         }

@@ -88,7 +88,7 @@ __device__ __forceinline__ TileBlocks tile_blocks(int l0, int l1, int l2, int e0
 
 // Particles of a patch that does not fit the sorted tile scatter straight into
 // the grid with float atomics (slow path, flagged in the status word).
-__device__ __noinline__ void p2g_spill(const SimDev& sim, const StateView& S, const Patch& P, const InstDev& I, int s,
+__device__ __forceinline__ void p2g_spill(const SimDev& sim, const StateView& S, const Patch& P, const InstDev& I, int s,
                                        int g) {
     for (int q = threadIdx.x; q < P.count; q += DCP_WG) {
         P2GRec r;
@@ -109,12 +109,13 @@ __device__ __noinline__ void p2g_spill(const SimDev& sim, const StateView& S, co
     }
 }
 
-// P2G of one patch: stable counting sort by base cell, per stencil node a
-// wave-level segmented sum over lanes sharing the base cell, and a plain LDS
-// read-modify-write by the last lane of each run into the wave's private tile
-// (lanes of a run write nothing; runs of a wave have distinct base cells, so a
-// wave's writes never collide).  The four wave tiles are summed in wave order
-// and flushed once into the instance grid.
+// P2G of one patch: stable counting sort by base cell; the run of each cell is
+// cut into chunks of at most sim.chunk particles and a lane sums its chunk's
+// stencil contributions in registers.  A wave-level segmented sum over lanes
+// of one cell follows, and the run's last lane does a plain LDS
+// read-modify-write into the wave's private tile (tail lanes of a wave hold
+// distinct cells, so for one stencil offset their writes never collide).  The
+// four wave tiles are summed in wave order and flushed once into the grid.
 // 24-float (6 x float4) LDS record of a particle's P2G inputs:
 // [x m] [b n0] [A00 A01 A02 A10] [A11 A12 A20 A21] [A22 n1 n2 k0] [k1 k2 - -]
 __device__ __forceinline__ void stage_p2g(float4* rec, const P2GRec& r) {
@@ -193,15 +194,15 @@ __global__ __launch_bounds__(DCP_WG, 2) void k_p2g(SimDev sim, StateView S, int 
             const int ci = ch0 + tid;
             const bool act = ci < nch;
             const int key = act ? (int)L.ch_key[ci] : 0;
-            SegInfo gi = segment_info_key(key, act);
+            const SegInfo gi = segment_info_key(key, act);
             // the lane's chunk: particles of one base cell, accumulated in registers
             // three passes over the chunk, one per x-node i, keep 9 x 4 sums live
             const int s0 = act ? (int)L.ch_start[ci] : 0, len = act ? (int)L.ch_len[ci] : 0;
 #pragma unroll
             for (int i = 0; i < 3; ++i) {
-                float acc[9][4];
+                float acc[36];
 #pragma unroll
-                for (int o = 0; o < 9; ++o) acc[o][0] = acc[o][1] = acc[o][2] = acc[o][3] = 0.f;
+                for (int o = 0; o < 36; ++o) acc[o] = 0.f;
                 for (int u = 0; u < len; ++u) {
                     P2GRec r;
                     unstage_p2g(rec + 6 * (int)L.perm[s0 + u], elem, r);
@@ -213,23 +214,23 @@ __global__ __launch_bounds__(DCP_WG, 2) void k_p2g(SimDev sim, StateView S, int 
                         for (int k = 0; k < 3; ++k) {
                             float mm, m0, m1, m2;
                             p2g_contrib(I, st, r, i, j, k, mm, m0, m1, m2);
-                            const int o = j * 3 + k;
-                            acc[o][0] += mm; acc[o][1] += m0; acc[o][2] += m1; acc[o][3] += m2;
+                            const int o = 4 * (j * 3 + k);
+                            acc[o] += mm; acc[o + 1] += m0; acc[o + 2] += m1; acc[o + 3] += m2;
                         }
                 }
+                seg_sum_n(acc, gi);
 #pragma unroll
                 for (int j = 0; j < 3; ++j)
 #pragma unroll
                     for (int k = 0; k < 3; ++k) {
-                        const int o = j * 3 + k;
-                        float mm = seg_sum(acc[o][0], gi), m0 = seg_sum(acc[o][1], gi);
-                        float m1 = seg_sum(acc[o][2], gi), m2 = seg_sum(acc[o][3], gi);
+                        const int o = 4 * (j * 3 + k);
                         if (gi.tail) {
                             const int c = key + (i * B.e1 + j) * B.e2 + k;
                             float4 t = T[c];
-                            t.x += mm; t.y += m0; t.z += m1; t.w += m2;
+                            t.x += acc[o]; t.y += acc[o + 1]; t.z += acc[o + 2]; t.w += acc[o + 3];
                             T[c] = t;
                         }
+                        tile_order();
                     }
             }
         }

@@ -77,6 +77,7 @@ __device__ __forceinline__ void patch_bbox(const SimDev& sim, const StateView& S
 // Bounding box from per-thread min/max base cells (block reduction).
 __device__ __forceinline__ void bbox_reduce(int lo[3], int hi[3], SortLDS& L, PatchBox& B) {
     const int tid = threadIdx.x;
+#pragma unroll
     for (int d = 0; d < 3; ++d) {
         int a = wave_min(lo[d]), b = wave_max(hi[d]);
         if ((tid & 63) == 0) { atomicMin(&L.box[d], a); atomicMax(&L.box[3 + d], b); }
@@ -272,6 +273,32 @@ __device__ __forceinline__ SegInfo segment_info(const SortLDS& L, int sp, int co
 }
 
 // inclusive segmented scan; after it the tail lane of each run holds the run sum
+// Segmented inclusive sum of N values at once: every shuffle of one step is
+// issued before any result is used, so the LDS-crossbar latencies overlap
+// instead of serialising per value.  The step count is wave-uniform.
+template <int N>
+__device__ __forceinline__ void seg_sum_n(float (&v)[N], const SegInfo& g) {
+    const int steps = __builtin_amdgcn_readfirstlane(g.steps);
+#pragma unroll
+    for (int s = 0; s < 6; ++s) {
+        if (s < steps) {
+            float t[N];
+#pragma unroll
+            for (int o = 0; o < N; ++o) t[o] = __shfl_up(v[o], 1 << s, 64);
+            const bool take = (g.same >> s) & 1u;
+#pragma unroll
+            for (int o = 0; o < N; ++o) v[o] = take ? v[o] + t[o] : v[o];
+        }
+    }
+}
+
+// Compiler-only ordering point between read-modify-writes of a wave-private
+// LDS tile.  Lanes of a wave update distinct cells for one stencil offset, but
+// a cell written at one offset may be read by another lane at the next; the
+// LDS executes a wave's instructions in issue order, so keeping the compiler
+// from hoisting the next read above this write is all that is needed.
+__device__ __forceinline__ void tile_order() { asm volatile("" ::: "memory"); }
+
 __device__ __forceinline__ float seg_sum(float v, const SegInfo& g) {
 #pragma unroll
     for (int s = 0; s < 6; ++s) {
