@@ -1,0 +1,271 @@
+#!/usr/bin/env python3
+"""Throughput of the batched differentiable cloth substep on one MI355X.
+
+One step = one forward + adjoint pass over a whole window of the batched
+parameter-identification workload: 64 cloth instances (sides 32..96
+vertices, own 128^3 grids, seeded E/nu/damping/friction and grasp
+trajectories), 300 substeps, point-cloud loss on 1024 targets per instance,
+gradients of the loss w.r.t. every instance's parameters.  The substep loop
+runs as captured hipGraphs.  Counted particles are cloth vertices plus
+element (quadrature) particles; grasper and cube particles are processed but
+not counted.
+
+Also reported on the same JSON line:
+  * the plain path (one 64x64 cloth on a 128^3 grid, 400 substeps, checkpoints
+    every 50, one launch per kernel, no graph) and the batched/plain ratio;
+  * the gradient of the plain workload vs central finite differences
+    (h = 1e-3 |theta|) for E and nu;
+  * ``roofline``: the P2G kernel's algorithmic bytes per launch over its
+    average launch duration, measured with HIP events recorded on the
+    library's stream around every P2G launch of the timed region, against the
+    8 TB/s HBM peak; ``traffic`` comes from the committed rocprofv3 counter
+    summary when present;
+  * ``cpu_baseline``: the FP64 CPU restatement in ``oracle/`` timed on the
+    host on a bounded sample of the same workload.
+
+Usage: python bench.py [--gpus 1] [--steps 5] [--warmup 2]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+GT_PARAMS = (8e3, 0.25)
+N_TARGETS = 1024
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def set_ground_truth_targets(eng, w, n_loss_state: int) -> None:
+    """Targets = seeded samples of the vertex positions at the loss state of a
+    forward run with E = 8e3, nu = 0.25 (the other parameters unchanged)."""
+    from diffcp_amd import scene
+    base = np.stack([i.params() for i in w.instances])
+    gt = base.copy()
+    gt[:, 0], gt[:, 1] = GT_PARAMS
+    eng.set_params(gt)
+    eng.reset(0)
+    slot, _ = eng.rollout(n_loss_state)
+    vx = eng.split(eng.download_state(slot)["vertex_x"], "n_vertices")
+    eng.set_targets([scene.sample_targets(v, N_TARGETS, w.seed * 100 + b) for b, v in enumerate(vx)])
+    eng.set_params(base)
+    eng.reset(0)
+
+
+def distinct_active_cells(eng, slot: int) -> int:
+    """Distinct grid cells the quadratic stencils of all particles touch."""
+    b = eng.binning_keys(slot).astype(np.int64)     # [n][3] base cells, instances concatenated
+    counts = [i.n_vertices + i.n_grasper + i.n_cube + i.n_elements for i in eng.instances]
+    inst = np.repeat(np.arange(len(counts), dtype=np.int64), counts)
+    cells = []
+    for di in range(3):
+        for dj in range(3):
+            for dk in range(3):
+                c = ((inst * 1024 + b[:, 0] + di) * 1024 + b[:, 1] + dj) * 1024 + b[:, 2] + dk
+                cells.append(c)
+    return int(np.unique(np.concatenate(cells)).size)
+
+
+def p2g_algorithmic_bytes(w, active_cells: int) -> float:
+    """Compulsory HBM bytes of one P2G launch: vertex x, v, C (60 B), element
+    C, normal column, Dinv and triangle (96 B), and (m, m v) written once per
+    distinct active cell (16 B)."""
+    nv = sum(i.n_vertices for i in w.instances)
+    ne = sum(i.n_elements for i in w.instances)
+    return 60.0 * nv + 96.0 * ne + 16.0 * active_cells
+
+
+def load_traffic():
+    """HBM bytes per P2G launch from the committed rocprofv3 counter summary."""
+    path = os.path.join(ROOT, "profiles", "k_p2g_c3_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        d = json.load(f)
+    return d.get("bytes_per_dispatch")
+
+
+def bench_batched(steps: int, warmup: int, use_graph: bool = True):
+    from diffcp_amd import scene
+    from diffcp_amd.engine import Engine
+
+    w = scene.config_c3()
+    S, K = w.substeps, w.checkpoint_every
+    eng = Engine(w.instances, max_states=Engine.slots_for(S, K))
+    set_ground_truth_targets(eng, w, S - 2)
+    # active cells at the initial state and at the loss state (mean)
+    slot, _ = eng.rollout(S - 2)
+    cells = 0.5 * (distinct_active_cells(eng, 0) + distinct_active_cells(eng, slot))
+    eng.reset(0)
+    # per-launch HIP events around P2G, captured into the rollout graphs.  torch
+    # is kept out of this process until the GPU legs are done: with torch's
+    # HIP runtime initialised first, capturing an event-record node fails
+    # (hipErrorInvalidValue), while the same capture succeeds without it.
+    eng.set_kernel_timing("p2g", max_launches=4 * S + 16)
+    for _ in range(warmup):
+        eng.rollout_grad(S, 0, K, use_graph=use_graph)
+    eng.status(clear=True)
+    eng.synchronize()
+    t0 = time.perf_counter()
+    dev_ms = 0.0
+    for _ in range(steps):
+        loss, grads, ms = eng.rollout_grad(S, 0, K, use_graph=use_graph)
+        dev_ms += ms
+    eng.synchronize()
+    wall = time.perf_counter() - t0
+    p2g_ms, p2g_n = eng.kernel_timing()
+    status = eng.status()
+    finite = bool(np.isfinite(loss).all() and np.isfinite(grads).all())
+    eng.close()
+    return dict(w=w, S=S, K=K, wall=wall, dev_ms=dev_ms, cells=cells, p2g_ms=p2g_ms, p2g_n=p2g_n,
+                status=status, finite=finite)
+
+
+def bench_plain(reps: int = 2):
+    from diffcp_amd import scene
+    from diffcp_amd.engine import Engine
+
+    w = scene.config_c2()
+    S, K = w.substeps, w.checkpoint_every
+    eng = Engine(w.instances, max_states=Engine.slots_for(S, K))
+    set_ground_truth_targets(eng, w, S - 2)
+    eng.rollout_grad(S, 0, K, use_graph=False)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        loss, grads, _ = eng.rollout_grad(S, 0, K, use_graph=False)
+    wall = (time.perf_counter() - t0) / reps
+    rate = w.particles * S / wall
+    # central finite differences of the loss at the loss state, h = 1e-3 |theta|
+    base = np.stack([i.params() for i in w.instances])
+    fd_err = {}
+    for pi, name in ((0, "E"), (1, "nu")):
+        h = 1e-3 * abs(base[0, pi])
+        vals = []
+        for sgn in (1.0, -1.0):
+            p = base.copy()
+            p[0, pi] += sgn * h
+            eng.set_params(p)
+            eng.reset(0)
+            slot, _ = eng.rollout(S - 2)
+            vals.append(float(eng.loss(slot)[0]))
+        fd = (vals[0] - vals[1]) / (2 * h)
+        fd_err[name] = abs(grads[0, pi] - fd) / max(abs(fd), 1e-300)
+    eng.set_params(base)
+    eng.close()
+    return dict(rate=rate, ms=wall * 1e3, fd_err=fd_err, particles=w.particles, S=S)
+
+
+def cpu_baseline(target_seconds: float = 15.0):
+    """FP64 torch restatement (oracle/), forward + autograd adjoint, on one C3
+    instance for as many substeps as fit ~target_seconds."""
+    import torch
+    from diffcp_amd import scene
+    from oracle import sim_torch as O
+
+    w = scene.config_c3()
+    inst = w.instances[0]
+    inst.targets = scene.sample_targets(inst.vertex_x, N_TARGETS, 0)
+
+    def run(n):
+        params = O.make_params(inst, requires_grad=True)
+        t = time.perf_counter()
+        loss, _ = O.tape_loss(inst, params, n, inst.targets)
+        loss.backward()
+        return time.perf_counter() - t
+
+    t3 = run(3)
+    n = int(min(w.substeps, max(3, round(target_seconds / max(t3 / 3, 1e-6)))))
+    dt = run(n)
+    return {"value": inst.n_cloth_particles * n / dt, "unit": "particle-substeps/s",
+            "cores": torch.get_num_threads(), "kind": "port",
+            "sample": (f"instance 0 of the batched workload ({inst.n_vertices} vertices, "
+                       f"{inst.n_elements} elements, 128^3 grid), {n} substeps forward + autograd "
+                       f"adjoint, FP64 torch restatement, {dt:.1f} s")}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--eager", action="store_true",
+                    help="launch kernels one by one instead of replaying graphs (for counter profiling)")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a 512 MiB device-to-device copy 5 times (scale for HBM byte counters)")
+    args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+        x = torch.ones(128 << 20, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(5):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": x.numel() * 4, "copy_bytes_written": y.numel() * 4}))
+        return 0
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (the batched workload fills one MI355X; "
+              f"--gpus {args.gpus} is not supported)", file=sys.stderr)
+        return 2
+
+    from diffcp_amd import _abi
+    if not os.path.exists(_abi.LIB_PATH):
+        import __graft_entry__
+        __graft_entry__.build()
+
+    log("batched workload: building 64 instances")
+    b = bench_batched(args.steps, args.warmup, use_graph=not args.eager)
+    w, S = b["w"], b["S"]
+    value = w.particles * S * args.steps / b["wall"]
+    log(f"batched: {value / 1e9:.4f} G particle-substeps/s, {b['wall'] / args.steps * 1e3:.1f} ms/step")
+
+    avg_ms = b["p2g_ms"] / max(b["p2g_n"], 1)
+    alg = p2g_algorithmic_bytes(w, int(b["cells"]))
+    achieved = alg / (avg_ms * 1e-3) / 1e9 if b["p2g_n"] else None
+    traffic = load_traffic()
+    roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": traffic,
+                "kernel": "k_p2g", "launches_timed": b["p2g_n"], "avg_launch_us": avg_ms * 1e3,
+                "algorithmic_bytes_per_launch": alg}
+
+    out = {
+        "metric": "particle-substeps/s fwd+adjoint, batched vs plain, 1 GPU; grad rel err vs FD",
+        "value": value, "unit": "particle-substeps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": b["wall"] / args.steps * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded cloths, trajectories and ground-truth targets)",
+        "config": {"workload": "batched parameter identification: 64 instances, sides 32..96 vertices, "
+                               "128^3 grids, 300 substeps fwd+adj, hipGraph loop",
+                   "instances": len(w.instances), "particles": w.particles, "substeps": S,
+                   "checkpoint_every": b["K"], "precision": "fp32 bulk, fp64 2x2 polar + accumulation"},
+        "roofline": roofline,
+        "status": b["status"], "finite": b["finite"],
+    }
+    if not args.no_plain:
+        log("plain workload (single instance, per-kernel launches)")
+        p = bench_plain()
+        out["plain"] = {"value": p["rate"], "ms_per_rollout": p["ms"], "particles": p["particles"],
+                        "substeps": p["S"]}
+        out["batched_over_plain"] = value / p["rate"]
+        out["grad_rel_err_fd"] = p["fd_err"]
+    if not args.no_cpu_baseline:
+        log("cpu baseline (oracle restatement on the host)")
+        out["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -123,6 +123,16 @@ struct dcp_ctx {
     std::map<std::tuple<int, int, int, int, int>, hipGraphExec_t> graphs;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_ms = 0.0;
+    // per-launch kernel timing
+    int time_kid = 0;
+    std::vector<hipEvent_t> tev;
+    int tev_used = 0;
+    bool capturing = false;
+    void drop_timing_events() {
+        for (hipEvent_t e : tev) (void)hipEventDestroy(e);
+        tev.clear();
+        tev_used = 0;
+    }
 
     StateView state(int slot) const { return StateView{states + (size_t)slot * slot_floats, NP, NE}; }
     StateView adjv(int g) const { return StateView{adj + (size_t)g * slot_floats, NP, NE}; }
@@ -162,50 +172,53 @@ extern "C" const char* dcp_last_error(void) { return g_err.c_str(); }
 // ---------------------------------------------------------------------------
 // launch sequences
 // ---------------------------------------------------------------------------
+// Optional per-launch HIP events around one kernel kind (dcp_set_kernel_timing).
+// Inside a stream capture the events become event-record nodes of the graph,
+// so every replay re-records them.
+static hipError_t time_mark(dcp_ctx* c) {
+    hipEvent_t e = c->tev[c->tev_used++];
+    return c->capturing ? hipEventRecordWithFlags(e, c->stream, hipEventRecordExternal) : hipEventRecord(e, c->stream);
+}
+#define TIMED_LAUNCH(kid, ...)                                                          \
+    do {                                                                                \
+        const bool on_ = (kid) == c->time_kid && c->tev_used + 2 <= (int)c->tev.size(); \
+        if (on_) HIP_OK(time_mark(c));                                                  \
+        hipLaunchKernelGGL(__VA_ARGS__);                                                \
+        LAUNCH_OK();                                                                    \
+        if (on_) HIP_OK(time_mark(c));                                                  \
+    } while (0)
+
 static int launch_prep(dcp_ctx* c, StateView S, StateView Sn, int fresh) {
     if (c->opt.precision == 1)
-        hipLaunchKernelGGL(k_prep<float>, dim3(c->grid_elems()), dim3(DCP_WG), 0, c->stream, c->sim, S, Sn, fresh);
+        TIMED_LAUNCH(DCP_KERNEL_PREP, k_prep<float>, dim3(c->grid_elems()), dim3(DCP_WG), 0, c->stream, c->sim, S, Sn, fresh);
     else
-        hipLaunchKernelGGL(k_prep<double>, dim3(c->grid_elems()), dim3(DCP_WG), 0, c->stream, c->sim, S, Sn, fresh);
-    LAUNCH_OK();
+        TIMED_LAUNCH(DCP_KERNEL_PREP, k_prep<double>, dim3(c->grid_elems()), dim3(DCP_WG), 0, c->stream, c->sim, S, Sn, fresh);
     return 0;
-}
-
-static void launch_p2g(dcp_ctx* c, StateView S, int s, int g) {
-    hipLaunchKernelGGL(k_p2g, dim3(c->grid_patches()), dim3(DCP_WG), 0, c->stream, c->sim, S, s, g);
 }
 
 static int launch_forward(dcp_ctx* c, int s, int g, StateView S, StateView Sn) {
     if (launch_prep(c, S, Sn, s == 0)) return -1;
-    launch_p2g(c, S, s, g);
-    LAUNCH_OK();
-    hipLaunchKernelGGL(k_gridop, dim3(c->grid_gridop()), dim3(DCP_WG), 0, c->stream, c->sim, 0);
-    LAUNCH_OK();
-    hipLaunchKernelGGL(k_g2p, dim3(c->grid_patches()), dim3(DCP_WG), 0, c->stream, c->sim, S, Sn);
-    LAUNCH_OK();
+    TIMED_LAUNCH(DCP_KERNEL_P2G, k_p2g, dim3(c->grid_patches()), dim3(DCP_WG), 0, c->stream, c->sim, S, s, g);
+    TIMED_LAUNCH(DCP_KERNEL_GRIDOP, k_gridop, dim3(c->grid_gridop()), dim3(DCP_WG), 0, c->stream, c->sim, 0);
+    TIMED_LAUNCH(DCP_KERNEL_G2P, k_g2p, dim3(c->grid_patches()), dim3(DCP_WG), 0, c->stream, c->sim, S, Sn);
     return 0;
 }
 
 static int launch_adjoint(dcp_ctx* c, int s, int g, StateView S, StateView G1, StateView Gout) {
     const int fresh = (s == 0);
     if (launch_prep(c, S, c->scratch_view(), fresh)) return -1;
-    launch_p2g(c, S, s, g);
-    LAUNCH_OK();
-    hipLaunchKernelGGL(k_gridop, dim3(c->grid_gridop()), dim3(DCP_WG), 0, c->stream, c->sim, 1);
-    LAUNCH_OK();
-    hipLaunchKernelGGL(k_g2p_adj, dim3(c->grid_patches()), dim3(DCP_WG), 0, c->stream, c->sim, S, G1, Gout);
-    LAUNCH_OK();
-    hipLaunchKernelGGL(k_gridop_adj, dim3(c->grid_gridop()), dim3(DCP_WG), 0, c->stream, c->sim);
-    LAUNCH_OK();
-    hipLaunchKernelGGL(k_p2g_adj, dim3(c->grid_patches()), dim3(DCP_WG), 0, c->stream, c->sim, S, Gout, s, g);
-    LAUNCH_OK();
+    TIMED_LAUNCH(DCP_KERNEL_P2G, k_p2g, dim3(c->grid_patches()), dim3(DCP_WG), 0, c->stream, c->sim, S, s, g);
+    TIMED_LAUNCH(DCP_KERNEL_GRIDOP, k_gridop, dim3(c->grid_gridop()), dim3(DCP_WG), 0, c->stream, c->sim, 1);
+    TIMED_LAUNCH(DCP_KERNEL_G2P_ADJ, k_g2p_adj, dim3(c->grid_patches()), dim3(DCP_WG), 0, c->stream, c->sim, S, G1, Gout);
+    TIMED_LAUNCH(DCP_KERNEL_GRIDOP_ADJ, k_gridop_adj, dim3(c->grid_gridop()), dim3(DCP_WG), 0, c->stream, c->sim);
+    TIMED_LAUNCH(DCP_KERNEL_P2G_ADJ, k_p2g_adj, dim3(c->grid_patches()), dim3(DCP_WG), 0, c->stream, c->sim, S, Gout, s, g);
     if (c->opt.precision == 1)
-        hipLaunchKernelGGL(k_prep_adj<float>, dim3(c->grid_patches()), dim3(DCP_WG), 0, c->stream, c->sim, S, G1, Gout, fresh);
+        TIMED_LAUNCH(DCP_KERNEL_PREP_ADJ, k_prep_adj<float>, dim3(c->grid_patches()), dim3(DCP_WG), 0, c->stream, c->sim, S, G1,
+                     Gout, fresh);
     else
-        hipLaunchKernelGGL(k_prep_adj<double>, dim3(c->grid_patches()), dim3(DCP_WG), 0, c->stream, c->sim, S, G1, Gout, fresh);
-    LAUNCH_OK();
-    hipLaunchKernelGGL(k_vertex_final, dim3(c->grid_patches()), dim3(DCP_WG), 0, c->stream, c->sim, Gout);
-    LAUNCH_OK();
+        TIMED_LAUNCH(DCP_KERNEL_PREP_ADJ, k_prep_adj<double>, dim3(c->grid_patches()), dim3(DCP_WG), 0, c->stream, c->sim, S, G1,
+                     Gout, fresh);
+    TIMED_LAUNCH(DCP_KERNEL_VERTEX_FINAL, k_vertex_final, dim3(c->grid_patches()), dim3(DCP_WG), 0, c->stream, c->sim, Gout);
     return 0;
 }
 
@@ -460,6 +473,7 @@ extern "C" int dcp_destroy(dcp_ctx* c) {
     for (void* p : c->allocs) hipFree(p);
     if (c->ev0) hipEventDestroy(c->ev0);
     if (c->ev1) hipEventDestroy(c->ev1);
+    c->drop_timing_events();
     if (c->own_stream) hipStreamDestroy(c->stream);
     delete c;
     return 0;
@@ -877,7 +891,9 @@ static int run_recorded(dcp_ctx* c, int kind, int cseg, int g, const Plan& p, in
     if (it == c->graphs.end()) {
         hipGraph_t graph;
         HIP_OK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        c->capturing = true;
         int r = rec();
+        c->capturing = false;
         hipError_t e = hipStreamEndCapture(c->stream, &graph);
         if (r) return r;
         if (e != hipSuccess) return set_err(std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
@@ -897,6 +913,8 @@ extern "C" int dcp_rollout_grad(dcp_ctx* c, int32_t S, int32_t g, int32_t K, int
     if (plan_rollout(c, S, K, p)) return -1;
     if (c->n_targets_total == 0) return set_err("no targets set (dcp_set_targets)");
     // the reset state must already be in slot 0 (dcp_reset)
+    // timing events restart with an eager rollout or with a plan's first capture
+    if (!use_graph || !c->graphs.count(std::make_tuple(0, 0, g, p.S, p.K))) c->tev_used = 0;
     HIP_OK(hipEventRecord(c->ev0, c->stream));
     if (dcp_zero_param_grads(c)) return -1;
     for (int cs = 0; cs < p.nseg; ++cs)
@@ -923,6 +941,7 @@ extern "C" int dcp_rollout(dcp_ctx* c, int32_t S, int32_t g, int32_t use_graph, 
     // forward-only: ping-pong between slots 0 and 1 would lose the reset state;
     // use slots 1 and 2 after the reset slot 0
     if (c->max_states < 3) return set_err("dcp_rollout needs max_states >= 3");
+    c->tev_used = 0;
     HIP_OK(hipEventRecord(c->ev0, c->stream));
     int cur = 0;
     for (int f = 0; f < S; ++f) {
@@ -937,6 +956,44 @@ extern "C" int dcp_rollout(dcp_ctx* c, int32_t S, int32_t g, int32_t use_graph, 
     HIP_OK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
     c->last_ms = ms;
     if (final_slot) *final_slot = cur;
+    return 0;
+}
+
+extern "C" int dcp_synchronize(dcp_ctx* c) {
+    if (!c) return set_err("bad args");
+    HIP_OK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+extern "C" int dcp_set_kernel_timing(dcp_ctx* c, int32_t kernel, int32_t max_launches) {
+    if (!c || kernel < 0 || kernel > DCP_KERNEL_VERTEX_FINAL || max_launches < 0) return set_err("bad args");
+    HIP_OK(hipStreamSynchronize(c->stream));
+    // cached graphs hold the previous event nodes (or none): rebuild them
+    for (auto& kv : c->graphs) hipGraphExecDestroy(kv.second);
+    c->graphs.clear();
+    c->drop_timing_events();
+    c->time_kid = kernel;
+    if (kernel == 0) return 0;
+    c->tev.resize((size_t)2 * max_launches, nullptr);
+    for (auto& e : c->tev) HIP_OK(hipEventCreate(&e));
+    // a capture rejects an event that was never recorded: record each once
+    for (auto& e : c->tev) HIP_OK(hipEventRecord(e, c->stream));
+    HIP_OK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+extern "C" int dcp_kernel_timing(dcp_ctx* c, double* total_ms, int32_t* launches) {
+    if (!c || !total_ms || !launches) return set_err("bad args");
+    HIP_OK(hipStreamSynchronize(c->stream));
+    double tot = 0.0;
+    int n = c->tev_used / 2;
+    for (int i = 0; i < n; ++i) {
+        float ms = 0.f;
+        HIP_OK(hipEventElapsedTime(&ms, c->tev[2 * i], c->tev[2 * i + 1]));
+        tot += ms;
+    }
+    *total_ms = tot;
+    *launches = n;
     return 0;
 }
 

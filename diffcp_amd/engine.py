@@ -239,6 +239,25 @@ class Engine:
         _abi.check(self.lib.dcp_last_rollout_ms(self.ctx, ctypes.byref(ms)))
         return loss, grads, ms.value
 
+    def synchronize(self) -> None:
+        """Wait for all work queued on the engine's stream."""
+        _abi.check(self.lib.dcp_synchronize(self.ctx))
+
+    KERNELS = {"prep": 1, "p2g": 2, "gridop": 3, "g2p": 4, "g2p_adj": 5, "gridop_adj": 6,
+               "p2g_adj": 7, "prep_adj": 8, "vertex_final": 9}
+
+    def set_kernel_timing(self, kernel: Optional[str], max_launches: int = 4096) -> None:
+        """Time every launch of one kernel kind with HIP events (None: off)."""
+        kid = 0 if kernel is None else self.KERNELS[kernel]
+        _abi.check(self.lib.dcp_set_kernel_timing(self.ctx, kid, max_launches if kid else 0))
+
+    def kernel_timing(self):
+        """(total ms, launches) recorded by set_kernel_timing."""
+        ms = ctypes.c_double()
+        n = ctypes.c_int32()
+        _abi.check(self.lib.dcp_kernel_timing(self.ctx, ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value
+
     def rollout(self, n_substeps: int, g: int = 0, use_graph: bool = False):
         slot = ctypes.c_int32()
         _abi.check(self.lib.dcp_rollout(self.ctx, n_substeps, g, int(use_graph), ctypes.byref(slot)))

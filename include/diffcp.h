@@ -71,7 +71,7 @@ typedef struct dcp_options {
     int32_t svd_grad;            /* 0: reference clamped SVD backward; 1: exact polar derivative */
     int32_t per_element_moduli;  /* 1: per-triangle (E, shear, contact) arrays     */
     int32_t patch_size;          /* particles per P2G/G2P patch (<= 1024); 0 = default */
-    int32_t p2g_mode;            /* 0: LDS float atomics; 1: LDS counting sort + wave shuffles */
+    int32_t p2g_mode;            /* particles per lane chunk in the sorted P2G / G2P-adjoint scatter (1..64); 0 = default 4 */
     int32_t grid_wgs;            /* persistent grid-update workgroups; 0 = default */
     int32_t reserved;
 } dcp_options;
@@ -198,6 +198,31 @@ int dcp_rollout(dcp_ctx* ctx, int32_t n_substeps, int32_t g, int32_t use_graph,
 
 /* Device time of the last dcp_rollout_grad / dcp_rollout call in ms. */
 int dcp_last_rollout_ms(dcp_ctx* ctx, double* ms);
+
+/* Block until all work queued on the context's stream has finished. */
+int dcp_synchronize(dcp_ctx* ctx);
+
+/* Kernel kinds for dcp_set_kernel_timing. */
+#define DCP_KERNEL_NONE         0
+#define DCP_KERNEL_PREP         1
+#define DCP_KERNEL_P2G          2
+#define DCP_KERNEL_GRIDOP       3
+#define DCP_KERNEL_G2P          4
+#define DCP_KERNEL_G2P_ADJ      5
+#define DCP_KERNEL_GRIDOP_ADJ   6
+#define DCP_KERNEL_P2G_ADJ      7
+#define DCP_KERNEL_PREP_ADJ     8
+#define DCP_KERNEL_VERTEX_FINAL 9
+
+/* Record a pair of HIP events on the context's stream around each launch of
+ * one kernel kind, for at most max_launches launches (DCP_KERNEL_NONE turns it
+ * off).  Cached rollout graphs are dropped and recaptured with the events as
+ * event-record nodes.  Profiling aid with no counterpart in the reference. */
+int dcp_set_kernel_timing(dcp_ctx* ctx, int32_t kernel, int32_t max_launches);
+
+/* Sum of the recorded launch durations (ms) and their number, for the most
+ * recent eager rollout or, with graphs, the most recent replay of each graph. */
+int dcp_kernel_timing(dcp_ctx* ctx, double* total_ms, int32_t* launches);
 
 /* Sizes: [B, total points, total elements, slot floats, max_states]. */
 int dcp_sizes(dcp_ctx* ctx, int64_t* out5);
