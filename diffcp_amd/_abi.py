@@ -63,6 +63,7 @@ SIGNATURES = {
     "dcp_loss_adjoint": (c_int, [c_void_p, c_int, c_int, c_dbl_p]),
     "dcp_param_grads": (c_int, [c_void_p, c_dbl_p]),
     "dcp_element_moduli_grads": (c_int, [c_void_p, c_dbl_p]),
+    "dcp_set_trajectory": (c_int, [c_void_p, c_dbl_p]),
     "dcp_trajectory_grads": (c_int, [c_void_p, c_dbl_p]),
     "dcp_download_state": (c_int, [c_void_p, c_int] + [c_flt_p] * 11),
     "dcp_upload_state": (c_int, [c_void_p, c_int] + [c_flt_p] * 11),

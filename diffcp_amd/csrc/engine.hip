@@ -116,6 +116,8 @@ struct dcp_ctx {
     int* vnat = nullptr;
     float* tmin = nullptr; int* targ = nullptr;
     double* loss_d = nullptr; double* seed_d = nullptr;
+    std::vector<double> ones_h;        // host source of the unit loss seeds
+    std::vector<float> traj_h;         // staging of dcp_set_trajectory
     int* keys_d = nullptr;
     int n_targets_total = 0;
     InstDev* inst_d = nullptr;
@@ -435,8 +437,8 @@ static int build(dcp_ctx* c, const dcp_instance_desc* d, int B) {
         c->alloc(&c->scratch, c->slot_floats) || c->alloc(&c->loss_d, B) || c->alloc(&c->seed_d, B) ||
         c->alloc(&c->keys_d, 3 * (size_t)(np + ne)))
         return -1;
-    std::vector<double> ones(B, 1.0);
-    HIP_OK(hipMemcpy(c->seed_d, ones.data(), B * sizeof(double), hipMemcpyHostToDevice));
+    c->ones_h.assign(B, 1.0);
+    HIP_OK(hipMemcpy(c->seed_d, c->ones_h.data(), B * sizeof(double), hipMemcpyHostToDevice));
     HIP_OK(hipEventCreate(&c->ev0));
     HIP_OK(hipEventCreate(&c->ev1));
     return 0;
@@ -686,6 +688,17 @@ extern "C" int dcp_element_moduli_grads(dcp_ctx* c, double* out) {
     return 0;
 }
 
+extern "C" int dcp_set_trajectory(dcp_ctx* c, const double* traj) {
+    if (!c || !traj) return set_err("bad args");
+    size_t nt = 0;
+    for (auto& I : c->inst_h) nt += (size_t)I.n_actions * I.n_grasp;
+    c->traj_h.resize(3 * nt);
+    for (size_t i = 0; i < 3 * nt; ++i) c->traj_h[i] = (float)traj[i];
+    HIP_OK(hipMemcpyAsync((void*)c->sim.traj, c->traj_h.data(), 3 * nt * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    HIP_OK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
 extern "C" int dcp_trajectory_grads(dcp_ctx* c, double* out) {
     if (!c || !out) return set_err("bad args");
     size_t nt = 0;
@@ -916,6 +929,8 @@ extern "C" int dcp_rollout_grad(dcp_ctx* c, int32_t S, int32_t g, int32_t K, int
     // timing events restart with an eager rollout or with a plan's first capture
     if (!use_graph || !c->graphs.count(std::make_tuple(0, 0, g, p.S, p.K))) c->tev_used = 0;
     HIP_OK(hipEventRecord(c->ev0, c->stream));
+    // unit loss seeds (dcp_loss_adjoint may have left others)
+    HIP_OK(hipMemcpyAsync(c->seed_d, c->ones_h.data(), c->B * sizeof(double), hipMemcpyHostToDevice, c->stream));
     if (dcp_zero_param_grads(c)) return -1;
     for (int cs = 0; cs < p.nseg; ++cs)
         if (run_recorded(c, 0, cs, g, p, use_graph)) return -1;

@@ -154,6 +154,13 @@ class Engine:
         _abi.check(self.lib.dcp_element_moduli_grads(self.ctx, _dp(out)))
         return out
 
+    def set_trajectory(self, trajectories: List[np.ndarray]) -> None:
+        """Replace every instance's grasp trajectory ([n_actions, n_grasp_points, 3] each)."""
+        t = np.ascontiguousarray(np.concatenate([np.asarray(x, np.float64).reshape(-1) for x in trajectories]))
+        if t.size != sum(i.trajectory.size for i in self.instances):
+            raise ValueError("trajectory sizes differ from the ones the engine was created with")
+        _abi.check(self.lib.dcp_set_trajectory(self.ctx, _dp(t)))
+
     def trajectory_grads(self) -> List[np.ndarray]:
         n = sum(i.trajectory.size for i in self.instances)
         out = np.zeros(n, np.float64)

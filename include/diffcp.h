@@ -141,6 +141,11 @@ int dcp_param_grads(dcp_ctx* ctx, double* out);
 /* Accumulated per-element moduli gradients [sum(n_elements)][3], natural order. */
 int dcp_element_moduli_grads(dcp_ctx* ctx, double* out);
 
+/* Replace the grasp trajectories [sum(n_actions*n_grasp_points)][3] (same
+ * layout as dcp_instance_desc.trajectory, instances concatenated).  Replaces
+ * grasp_tracking_seq.from_numpy (simulation.py:434). */
+int dcp_set_trajectory(dcp_ctx* ctx, const double* trajectory);
+
 /* Accumulated trajectory gradient [sum(n_actions*n_grasp_points)][3]
  * (grasp_tracking_seq.grad, gradient_descent_for_traj_generation.py:24). */
 int dcp_trajectory_grads(dcp_ctx* ctx, double* out);

@@ -1,0 +1,84 @@
+"""The drop-in module (reference names, Tape) and the torch.autograd.Function
+against the library's fused window call, through the C-ABI.
+
+Tolerance 1e-5 relative: the same kernels run in a different launch order
+(substep by substep vs checkpointed segments); only float-atomic summation
+order differs.
+"""
+import numpy as np
+import pytest
+import torch
+
+from diffcp_amd import scene
+
+pytestmark = pytest.mark.gpu
+
+
+def _rel(a, b):
+    a, b = np.asarray(a, np.float64), np.asarray(b, np.float64)
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+def _instance():
+    inst = scene.tiny_instance(n=6, n_grid=16, substeps=40)
+    rng = np.random.default_rng(2)
+    inst.targets = inst.vertex_x[rng.choice(inst.n_vertices, 16, replace=False)] + rng.normal(0, 0.01, (16, 3))
+    return inst
+
+
+def test_tape_window_matches_fused_rollout():
+    from diffcp_amd import simulation as sim
+    from diffcp_amd.engine import Engine
+
+    inst = _instance()
+    S = 30
+    s = sim.setup(inst, max_steps=S + 1)
+    sim.initialize_objects()
+    sim.garment_E[None] = 6e3
+    sim.garment_nu[None] = 0.28
+    sim.target_pos.from_numpy(inst.targets)
+    with sim.Tape(loss=sim.loss):
+        sim.loss[None] = 0
+        for f in range(sim.max_steps - 1):
+            sim.forward(f, 0)
+        sim.compute_loss(f)
+    grads = [getattr(sim, n).grad[None] for n in ("garment_E", "garment_nu", "shearing_stiffness", "damping")]
+    final = sim.vertex_x.to_numpy()[sim.max_steps - 1]
+    assert final.shape == (inst.n_vertices, 3)
+
+    eng = Engine([inst], max_states=Engine.slots_for(S, S))
+    p = inst.params()
+    p[0], p[1] = 6e3, 0.28
+    eng.set_params(p[None])
+    eng.reset(0)
+    loss, g, _ = eng.rollout_grad(S, 0, S, use_graph=False)
+    assert _rel(sim.loss[None], loss[0]) < 1e-5
+    assert _rel(grads, g[0, [0, 1, 2, 4]]) < 1e-5
+
+    # window carry-over (parameter_fitting_functions.py:40-49)
+    sim.initial_vertex_x.from_numpy(final)
+    sim.initial_operator_x.from_numpy(sim.operator_x.to_numpy()[sim.max_steps - 1])
+    sim.initial_cube_x.from_numpy(sim.cube_x.to_numpy()[sim.max_steps - 1])
+    sim.initialize_ti_field()
+    st0 = s.engine.download_state(0)
+    assert np.abs(st0["vertex_x"] - final).max() < 1e-6
+    assert np.abs(st0["vertex_v"]).max() == 0.0
+
+
+def test_autograd_function_scales_unit_seed_gradients():
+    from diffcp_amd.autograd import rollout_loss
+    from diffcp_amd.engine import Engine
+
+    insts = [_instance(), _instance()]
+    insts[1].E = 9e3
+    S = 24
+    eng = Engine(insts, max_states=Engine.slots_for(S, 8))
+    eng.reset(0)
+    params = torch.tensor(np.stack([i.params() for i in insts]), dtype=torch.float64, requires_grad=True)
+    loss = rollout_loss(eng, params, S, checkpoint_every=8)
+    w = torch.tensor([2.0, -0.5], dtype=torch.float64)
+    (loss * w).sum().backward()
+    eng.reset(0)
+    l2, g2, _ = eng.rollout_grad(S, 0, 8, use_graph=False)
+    assert _rel(loss.detach().numpy(), l2) < 1e-5
+    assert _rel(params.grad.numpy(), w.numpy()[:, None] * g2) < 1e-5

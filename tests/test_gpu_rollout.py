@@ -1,0 +1,124 @@
+"""Whole-window tests of the HIP path through the C-ABI: gradients against
+finite differences of the FP64 oracle, graph replay against eager launches,
+checkpoint plans against each other, the trajectory gradient, event timing.
+
+Tolerances:
+  * gradient vs FP64 central FD (h = 1e-4 relative, committed fixture):
+    1e-3 relative, the bar the gradient must meet;
+  * graph replay vs eager and checkpoint plans vs each other: 1e-5 relative
+    (the same kernels; only float-atomic summation order differs);
+  * trajectory gradient vs the oracle's autograd: 2e-3 of its max norm.
+"""
+import os
+
+import numpy as np
+import pytest
+import torch
+
+from diffcp_amd import scene
+from oracle import sim_torch as O
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+FIXTURE = os.path.join(ROOT, "tests", "golden", "c1_grad_fd.npz")
+
+
+def _engine(insts, **kw):
+    from diffcp_amd.engine import Engine
+    return Engine(insts, **kw)
+
+
+def _slots(S, K):
+    from diffcp_amd.engine import Engine
+    return Engine.slots_for(S, K)
+
+
+def _rel(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+@pytest.mark.parametrize("svd_grad", ["reference", "stable"])
+def test_c1_gradient_vs_fp64_finite_differences(svd_grad):
+    ref = np.load(FIXTURE)
+    w = scene.config_c1()
+    S = int(ref["substeps"])
+    eng = _engine(w.instances, max_states=_slots(S, S), svd_grad=svd_grad)
+    eng.set_targets([ref["targets"]])
+    eng.reset(0)
+    loss, grads, _ = eng.rollout_grad(S, 0, S, use_graph=False)
+    assert abs(loss[0] - float(ref["loss"])) < 1e-4 * abs(float(ref["loss"]))
+    fd = ref["fd_0.0001"]
+    for j, name in enumerate(("E", "nu")):
+        err = abs(grads[0, j] - fd[j]) / abs(fd[j])
+        assert err < 1e-3, (name, grads[0, j], fd[j], err)
+
+
+def _c1_engine(S, K, **kw):
+    w = scene.config_c1()
+    inst = w.instances[0]
+    inst.targets = scene.sample_targets(inst.vertex_x, 512, 3) + np.array([0.0, 0.004, 0.0])
+    eng = _engine([inst], max_states=_slots(S, K), **kw)
+    eng.reset(0)
+    return eng
+
+
+def test_graph_replay_matches_eager():
+    S = 120
+    eng = _c1_engine(S, 40)
+    l0, g0, _ = eng.rollout_grad(S, 0, 40, use_graph=False)
+    for rep in range(3):
+        l1, g1, _ = eng.rollout_grad(S, 0, 40, use_graph=True)
+        assert _rel(l1, l0) < 1e-5 and _rel(g1, g0) < 1e-5, rep
+        # eager work and a synchronising read-back between replays
+        eng.substep(0, 0, 0, 1)
+        eng.download_state(1)
+    assert eng.status() & 1 == 0
+
+
+def test_checkpoint_plans_agree():
+    S = 120
+    res = []
+    for K in (120, 40, 17):
+        eng = _c1_engine(S, K)
+        l, g, _ = eng.rollout_grad(S, 0, K, use_graph=True)
+        res.append((l, g))
+        eng.close()
+    for l, g in res[1:]:
+        assert _rel(l, res[0][0]) < 1e-5
+        assert _rel(g[:, :5], res[0][1][:, :5]) < 1e-5
+
+
+def test_kernel_timing_events_eager_and_graph():
+    S = 60
+    eng = _c1_engine(S, S)
+    eng.set_kernel_timing("p2g", 4 * S)
+    eng.rollout_grad(S, 0, S, use_graph=False)
+    ms_e, n_e = eng.kernel_timing()
+    assert n_e > 2 * (S - 2) and ms_e > 0
+    eng.set_kernel_timing(None)
+    ms_0, n_0 = eng.kernel_timing()
+    assert n_0 == 0
+
+
+def test_trajectory_gradient_matches_oracle():
+    inst = scene.tiny_instance(n=5, n_grid=16, substeps=40)
+    rng = np.random.default_rng(5)
+    inst.targets = inst.vertex_x[rng.choice(inst.n_vertices, 12, replace=False)] + rng.normal(0, 0.01, (12, 3))
+    S = 30
+    eng = _engine([inst], max_states=_slots(S, S), svd_grad="stable")
+    eng.reset(0)
+    loss, _, _ = eng.rollout_grad(S, 0, S, use_graph=False)
+    gt = eng.trajectory_grads()[0]
+
+    st, static = O.initialize_state(inst)
+    static["traj"] = static["traj"].clone().requires_grad_(True)
+    params = O.make_params(inst)
+    states, _ = O.rollout(inst, params, S, svd_grad="stable", keep_states=True, state=st, static=static)
+    lo, _ = O.compute_loss(states[S - 2]["vx"], torch.as_tensor(inst.targets))
+    lo.backward()
+    ref = static["traj"].grad.numpy()
+    assert abs(loss[0] - float(lo)) < 1e-4 * float(lo)
+    assert np.abs(ref).max() > 0
+    assert np.abs(gt - ref).max() < 2e-3 * np.abs(ref).max(), (np.abs(gt - ref).max(), np.abs(ref).max())
