@@ -20,6 +20,11 @@ from oracle import sim_torch as O
 pytestmark = pytest.mark.gpu
 
 
+# status bits that mean a wrong result (NaN/Inf, stencil clamped at the grid
+# edge); bit 4 only records that a patch took the slower global-atomic path
+BAD_STATUS = 1 | 2
+
+
 def _engine(insts, **kw):
     from diffcp_amd.engine import Engine
     return Engine(insts, **kw)
@@ -44,7 +49,7 @@ def test_forward_tiny_matches_oracle(precision):
     inst = scene.tiny_instance(n=5, n_grid=16, substeps=10)
     N = 8
     eng = _gpu_rollout(inst, N, precision=precision)
-    assert eng.status() == 0
+    assert eng.status() & BAD_STATUS == 0
     states, _ = O.rollout(inst, n_substeps=N, keep_states=True)
     for s in (1, N):
         gpu = eng.download_state(s)
@@ -67,7 +72,7 @@ def test_forward_c1_full_rollout_positions():
         st, _ = O.rollout(inst, n_substeps=S)
     err = np.abs(gpu["vertex_x"] - st["vx"].numpy()).max()
     assert err / np.abs(st["vx"].numpy()).max() < 1e-5, err
-    assert eng.status() == 0
+    assert eng.status() & BAD_STATUS == 0
 
 
 @pytest.mark.parametrize("svd_grad", ["stable", "reference"])
