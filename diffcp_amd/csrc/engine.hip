@@ -190,6 +190,14 @@ static hipError_t time_mark(dcp_ctx* c) {
         if (on_) HIP_OK(time_mark(c));                                                  \
     } while (0)
 
+static int launch_p2g(dcp_ctx* c, StateView S, int s, int g) {
+    if (c->sim.chunk > 0)
+        TIMED_LAUNCH(DCP_KERNEL_P2G, k_p2g_chunked, dim3(c->grid_patches()), dim3(DCP_WG), 0, c->stream, c->sim, S, s, g);
+    else
+        TIMED_LAUNCH(DCP_KERNEL_P2G, k_p2g, dim3(c->grid_patches()), dim3(DCP_WG), 0, c->stream, c->sim, S, s, g);
+    return 0;
+}
+
 static int launch_prep(dcp_ctx* c, StateView S, StateView Sn, int fresh) {
     if (c->opt.precision == 1)
         TIMED_LAUNCH(DCP_KERNEL_PREP, k_prep<float>, dim3(c->grid_elems()), dim3(DCP_WG), 0, c->stream, c->sim, S, Sn, fresh);
@@ -200,7 +208,7 @@ static int launch_prep(dcp_ctx* c, StateView S, StateView Sn, int fresh) {
 
 static int launch_forward(dcp_ctx* c, int s, int g, StateView S, StateView Sn) {
     if (launch_prep(c, S, Sn, s == 0)) return -1;
-    TIMED_LAUNCH(DCP_KERNEL_P2G, k_p2g, dim3(c->grid_patches()), dim3(DCP_WG), 0, c->stream, c->sim, S, s, g);
+    if (launch_p2g(c, S, s, g)) return -1;
     TIMED_LAUNCH(DCP_KERNEL_GRIDOP, k_gridop, dim3(c->grid_gridop()), dim3(DCP_WG), 0, c->stream, c->sim, 0);
     TIMED_LAUNCH(DCP_KERNEL_G2P, k_g2p, dim3(c->grid_patches()), dim3(DCP_WG), 0, c->stream, c->sim, S, Sn);
     return 0;
@@ -209,9 +217,13 @@ static int launch_forward(dcp_ctx* c, int s, int g, StateView S, StateView Sn) {
 static int launch_adjoint(dcp_ctx* c, int s, int g, StateView S, StateView G1, StateView Gout) {
     const int fresh = (s == 0);
     if (launch_prep(c, S, c->scratch_view(), fresh)) return -1;
-    TIMED_LAUNCH(DCP_KERNEL_P2G, k_p2g, dim3(c->grid_patches()), dim3(DCP_WG), 0, c->stream, c->sim, S, s, g);
+    if (launch_p2g(c, S, s, g)) return -1;
     TIMED_LAUNCH(DCP_KERNEL_GRIDOP, k_gridop, dim3(c->grid_gridop()), dim3(DCP_WG), 0, c->stream, c->sim, 1);
-    TIMED_LAUNCH(DCP_KERNEL_G2P_ADJ, k_g2p_adj, dim3(c->grid_patches()), dim3(DCP_WG), 0, c->stream, c->sim, S, G1, Gout);
+    if (c->sim.chunk > 0)
+        TIMED_LAUNCH(DCP_KERNEL_G2P_ADJ, k_g2p_adj_chunked, dim3(c->grid_patches()), dim3(DCP_WG), 0, c->stream, c->sim, S,
+                     G1, Gout);
+    else
+        TIMED_LAUNCH(DCP_KERNEL_G2P_ADJ, k_g2p_adj, dim3(c->grid_patches()), dim3(DCP_WG), 0, c->stream, c->sim, S, G1, Gout);
     TIMED_LAUNCH(DCP_KERNEL_GRIDOP_ADJ, k_gridop_adj, dim3(c->grid_gridop()), dim3(DCP_WG), 0, c->stream, c->sim);
     TIMED_LAUNCH(DCP_KERNEL_P2G_ADJ, k_p2g_adj, dim3(c->grid_patches()), dim3(DCP_WG), 0, c->stream, c->sim, S, Gout, s, g);
     if (c->opt.precision == 1)
@@ -407,7 +419,8 @@ static int build(dcp_ctx* c, const dcp_instance_desc* d, int B) {
     S.B = B; S.NP = np; S.NE = ne; S.n_patches = c->n_patches;
     S.precision = c->opt.precision; S.svd_grad = c->opt.svd_grad; S.per_elem = c->opt.per_element_moduli;
     S.max_active = flags;
-    S.chunk = c->opt.p2g_mode > 0 ? std::min(c->opt.p2g_mode, 64) : 4;
+    // 0: compute-then-gather P2G; K > 0: chunks of K particles with a segmented sum
+    S.chunk = c->opt.p2g_mode > 0 ? std::min(c->opt.p2g_mode, 64) : 0;
     c->slot_floats = 15 * (size_t)np + 12 * (size_t)ne;
     c->max_states = std::max(2, c->opt.max_states);
     Patch* pd; int *trid, *eid, *iod, *ild; float* trd;
@@ -887,7 +900,10 @@ static int record_backward_segment(dcp_ctx* c, const Plan& p, int cseg, int g) {
     int base = cseg * p.K;
     int end = std::min(base + p.K, p.L);   // adjoint substeps base .. end-1
     if (end <= base) return 0;
-    for (int f = base; f + 1 < end; ++f)   // recompute states base+1 .. end-1
+    // recompute states base+1 .. end-1, except in the last forward segment: no
+    // later segment reused its slots, so they still hold those states
+    const bool resident = cseg == p.nseg - 1;
+    for (int f = base; !resident && f + 1 < end; ++f)
         if (launch_forward(c, f, g, c->state(seg_slot(p, cseg, f)), c->state(seg_slot(p, cseg, f + 1)))) return -1;
     for (int f = end - 1; f >= base; --f) {
         int gin = (p.L - (f + 1)) & 1, gout = (p.L - f) & 1;

@@ -107,42 +107,47 @@ struct P2GRec {
 // everything a particle scatters: per cell w*(b + A dpos) + (grad w . n) k, mass w*m
 __device__ __forceinline__ void p2g_record(const SimDev& sim, const InstDev& I, const StateView& S, const Patch& P,
                                            int q, int s, int g, P2GRec& r) {
-    r.elem = false;
-    if (P.cls == CLS_ELEMENT) {
+    // locals assigned once at the end: a struct filled field by field in
+    // branches with an early return ends up in scratch memory
+    V3<float> x, b, n = {0.f, 0.f, 0.f}, k = {0.f, 0.f, 0.f};
+    M3<float> C;
+    float m;
+    const bool elem = P.cls == CLS_ELEMENT;
+    if (elem) {
         const int e = P.start + q;
-        r.elem = true;
-        r.x = load_v3(sim.xe, sim.NE, e);
-        V3<float> v = load_v3(sim.ve, sim.NE, e);
-        r.m = I.m_quad;
-        r.b = r.m * v;
-        r.A = mscale(r.m, S.Ce(e));
-        r.n = load_v3(sim.nd, sim.NE, e);
-        r.k = load_v3(sim.ke, sim.NE, e);
-        return;
-    }
-    const int p = P.start + q;
-    r.x = S.x(p);
-    r.n = {0.f, 0.f, 0.f};
-    r.k = {0.f, 0.f, 0.f};
-    if (P.cls == CLS_VERTEX) {
-        r.m = I.m_vertex;
-        V3<float> v = S.v(p);
-        v.y += I.gdt;                                       // simulation.py:181-182
-        V3<float> f = {0.f, 0.f, 0.f};
-        for (int t = sim.inc_off[p]; t < sim.inc_off[p + 1]; ++t) {
-            int ec = sim.inc_list[t];
-            int e = ec / 3, c = ec - 3 * e;
-            f = f + load_v3(sim.fc + (size_t)3 * c * sim.NE, sim.NE, e);
-        }
-        r.b = r.m * v + f;
-    } else if (P.cls == CLS_GRASPER) {
-        r.m = I.m_op;
-        r.b = r.m * grasper_velocity(sim, I, p - I.o_start, s, g);
+        x = load_v3(sim.xe, sim.NE, e);
+        m = I.m_quad;
+        b = m * load_v3(sim.ve, sim.NE, e);
+        C = S.Ce(e);
+        n = load_v3(sim.nd, sim.NE, e);
+        k = load_v3(sim.ke, sim.NE, e);
     } else {
-        r.m = I.m_cube;
-        r.b = {0.f, 0.f, 0.f};                              // simulation.py:150-151 (no velocity term)
+        const int p = P.start + q;
+        x = S.x(p);
+        C = S.C(p);
+        if (P.cls == CLS_VERTEX) {
+            m = I.m_vertex;
+            V3<float> v = S.v(p);
+            v.y += I.gdt;                                   // simulation.py:181-182
+            V3<float> f = {0.f, 0.f, 0.f};
+            for (int t = sim.inc_off[p]; t < sim.inc_off[p + 1]; ++t) {
+                int ec = sim.inc_list[t];
+                int e = ec / 3, c = ec - 3 * e;
+                f = f + load_v3(sim.fc + (size_t)3 * c * sim.NE, sim.NE, e);
+            }
+            b = m * v + f;
+        } else if (P.cls == CLS_GRASPER) {
+            m = I.m_op;
+            b = m * grasper_velocity(sim, I, p - I.o_start, s, g);
+        } else {
+            m = I.m_cube;
+            b = {0.f, 0.f, 0.f};                            // simulation.py:150-151 (no velocity term)
+        }
     }
-    r.A = mscale(r.m, S.C(p));
+    r.x = x; r.b = b; r.n = n; r.k = k;
+    r.A = mscale(m, C);
+    r.m = m;
+    r.elem = elem;
 }
 
 __device__ __forceinline__ void p2g_contrib(const InstDev& I, const Stencil& st, const P2GRec& r, int i, int j, int k,
@@ -264,6 +269,7 @@ __device__ __forceinline__ void grid_velocity(const InstDev& I, float4 c, int i,
 // kernels -------------------------------------------------------------------
 template <typename PT> __global__ void k_prep(SimDev sim, StateView S, StateView Snext, int fresh);
 __global__ void k_p2g(SimDev sim, StateView S, int s, int g);
+__global__ void k_p2g_chunked(SimDev sim, StateView S, int s, int g);
 __global__ void k_gridop(SimDev sim, int keep);
 __global__ void k_g2p(SimDev sim, StateView S, StateView Sn);
 __global__ void k_loss_min(SimDev sim, StateView S, const float* targets, const int4* tchunks, int n_chunks,
@@ -272,6 +278,7 @@ __global__ void k_loss_sum(const float* tmin, const int* toff, const int* tcount
 __global__ void k_binning_keys(SimDev sim, StateView S, int* keys);
 
 __global__ void k_g2p_adj(SimDev sim, StateView S, StateView G1, StateView Gout);
+__global__ void k_g2p_adj_chunked(SimDev sim, StateView S, StateView G1, StateView Gout);
 __global__ void k_gridop_adj(SimDev sim);
 __global__ void k_p2g_adj(SimDev sim, StateView S, StateView Gout, int s, int g);
 template <typename PT> __global__ void k_prep_adj(SimDev sim, StateView S, StateView G1, StateView Gout, int fresh);

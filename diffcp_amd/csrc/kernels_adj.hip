@@ -115,7 +115,150 @@ __device__ __forceinline__ void unstage_g2p(const float4* rec, V3<float>& x, V3<
     gC.m[7] = e.x; gC.m[8] = e.y;
 }
 
+// Stage a patch's G2P-reverse inputs into LDS, reduce its stencil box and sort
+// it by base cell.  Returns false after the global-atomic path when the patch
+// does not fit the LDS tile.
+__device__ __forceinline__ bool g2p_adj_stage_sort(const SimDev& sim, const StateView& S, const StateView& G1,
+                                                   const StateView& Gout, const Patch& P, const InstDev& I,
+                                                   SortLDS& L, float4* rec, int* pbase, PatchBox& B) {
+    const int tid = threadIdx.x;
+    if (P.count > DCP_STAGE_MAX) {
+        g2p_adj_spill(sim, S, G1, Gout, P, I);
+        __syncthreads();
+        return false;
+    }
+    if (tid < 3) { L.box[tid] = 1 << 30; L.box[3 + tid] = -(1 << 30); }
+    __syncthreads();
+    int lo0 = 1 << 30, lo1 = 1 << 30, lo2 = 1 << 30, hi0 = -(1 << 30), hi1 = -(1 << 30), hi2 = -(1 << 30);
+    for (int q = tid; q < P.count; q += DCP_WG) {
+        V3<float> x = particle_x(sim, S, P, q), gv, gx1;
+        M3<float> gC;
+        g2p_adj_inputs(sim, I, G1, P, q, gv, gx1, gC);
+        int b0 = stencil_base(x.x, I.inv_dx, I.G), b1 = stencil_base(x.y, I.inv_dx, I.G),
+            b2 = stencil_base(x.z, I.inv_dx, I.G);
+        pbase[q] = pack_base(b0, b1, b2);
+        stage_g2p(rec + 5 * q, x, gv, gx1, gC);
+        lo0 = min(lo0, b0); lo1 = min(lo1, b1); lo2 = min(lo2, b2);
+        hi0 = max(hi0, b0); hi1 = max(hi1, b1); hi2 = max(hi2, b2);
+    }
+    bbox_reduce(lo0, lo1, lo2, hi0, hi1, hi2, L, B);
+    if (B.vol > DCP_STILE_CAP) {
+        g2p_adj_spill(sim, S, G1, Gout, P, I);
+        __syncthreads();
+        return false;
+    }
+    for (int q = tid; q < P.count; q += DCP_WG) {
+        int b0, b1, b2;
+        unpack_base(pbase[q], b0, b1, b2);
+        L.key[q] = (unsigned short)tile_cell(B, b0, b1, b2);
+    }
+    __syncthreads();
+    patch_sort(sim, S, P, I, B, L, true);
+    return true;
+}
+
+// Compute-then-gather form (the structure of k_p2g): one particle per lane with
+// its G2P-reverse inputs in registers; per x-layer i each lane computes its
+// particle's dL/d grid_v_out contributions to the layer's nine nodes into LDS
+// slots, the runs of each occupied base cell are summed, and the lane owning a
+// tile node adds its nine run sums.  dL/dx of each particle (all 27 nodes) is
+// completed in the first pass and stored.
 __global__ __launch_bounds__(DCP_WG, 2) void k_g2p_adj(SimDev sim, StateView S, StateView G1, StateView Gout) {
+    __shared__ BinLDS L;
+    __shared__ float4 contrib[9 * DCP_SORT_MAX];
+    const int tid = threadIdx.x;
+    bin_clear(L);
+    if (tid < 3) { L.box[tid] = 1 << 30; L.box[3 + tid] = -(1 << 30); }
+    __syncthreads();
+    for (int pi = blockIdx.x; pi < sim.n_patches; pi += gridDim.x) {
+        const Patch P = sim.patches[pi];
+        const InstDev I = sim.inst[P.inst];
+        bool tiled = P.count <= DCP_WG;
+        if (tiled) {
+            const bool act = tid < P.count;
+            V3<float> x = {0.f, 0.f, 0.f}, gv, gx1;
+            M3<float> gC;
+            int b0 = 0, b1 = 0, b2 = 0;
+            if (act) {
+                x = particle_x(sim, S, P, tid);
+                g2p_adj_inputs(sim, I, G1, P, tid, gv, gx1, gC);
+                b0 = stencil_base(x.x, I.inv_dx, I.G);
+                b1 = stencil_base(x.y, I.inv_dx, I.G);
+                b2 = stencil_base(x.z, I.inv_dx, I.G);
+            }
+            PatchBox B;
+            box_reduce(b0, b1, b2, act, L.box, B);
+            tiled = B.vol <= DCP_STILE_CAP;
+            if (tiled) {
+                const int pos = patch_bin(act ? tile_cell(B, b0, b1, b2) : -1, B, L);
+                int nc[2][3];
+                bool has[2];
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const int c = tid + h * DCP_WG;
+                    has[h] = c < B.vol;
+                    const int ck = c % B.e2, t = c / B.e2;
+                    nc[h][0] = t / B.e1; nc[h][1] = t % B.e1; nc[h][2] = ck;
+                }
+                float4 a[2] = {make_float4(0.f, 0.f, 0.f, 0.f), make_float4(0.f, 0.f, 0.f, 0.f)};
+#pragma unroll
+                for (int i = 0; i < 3; ++i) {
+                    if (act) {
+                        Stencil st;
+                        make_stencil(x, I.inv_dx, I.G, st, nullptr);
+                        float gfx[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+                        for (int j = 0; j < 3; ++j)
+#pragma unroll
+                            for (int k = 0; k < 3; ++k) {
+                                float4 gvo = sim.vout[cell_index(I, st.base[0] + i, st.base[1] + j, st.base[2] + k)];
+                                float gg[3];
+                                g2p_adj_node(I, st, i, j, k, gvo, gv, gC, gg, gfx);
+                                contrib[(j * 3 + k) * DCP_SORT_MAX + pos] = make_float4(0.f, gg[0], gg[1], gg[2]);
+                            }
+                        if (i == 0) {
+                            // the remaining x-layers' share of dL/dfx, then the store
+#pragma unroll
+                            for (int i2 = 1; i2 < 3; ++i2)
+#pragma unroll
+                                for (int j = 0; j < 3; ++j)
+#pragma unroll
+                                    for (int k = 0; k < 3; ++k) {
+                                        float4 gvo = sim.vout[cell_index(I, st.base[0] + i2, st.base[1] + j, st.base[2] + k)];
+                                        float gg[3];
+                                        g2p_adj_node(I, st, i2, j, k, gvo, gv, gC, gg, gfx);
+                                    }
+                            g2p_adj_store(sim, I, Gout, P, tid, gx1, gfx);
+                        }
+                    }
+                    __syncthreads();
+                    run_sums(L, contrib);
+                    __syncthreads();
+#pragma unroll
+                    for (int h = 0; h < 2; ++h) {
+                        const float4 v = node_sum(L, B, P.count, contrib, nc[h][0], nc[h][1], nc[h][2], i, has[h]);
+                        a[h].y += v.y; a[h].z += v.z; a[h].w += v.w;
+                    }
+                    __syncthreads();
+                }
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    if (has[h] && (a[h].y != 0.f || a[h].z != 0.f || a[h].w != 0.f)) {
+                        long long gidx = cell_index(I, nc[h][0] + B.l0, nc[h][1] + B.l1, nc[h][2] + B.l2);
+                        atomicAdd(&sim.gacc[gidx].y, a[h].y); atomicAdd(&sim.gacc[gidx].z, a[h].z);
+                        atomicAdd(&sim.gacc[gidx].w, a[h].w);
+                    }
+                }
+            }
+        }
+        if (!tiled) g2p_adj_spill(sim, S, G1, Gout, P, I);
+        bin_clear(L);
+        if (tid < 3) { L.box[tid] = 1 << 30; L.box[3 + tid] = -(1 << 30); }
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(DCP_WG, 2) void k_g2p_adj_chunked(SimDev sim, StateView S, StateView G1, StateView Gout) {
     __shared__ SortLDS L;
     __shared__ float4 tiles[DCP_WG / 64][DCP_STILE_CAP];
     __shared__ float4 rec[DCP_STAGE_MAX * 5];
@@ -124,40 +267,9 @@ __global__ __launch_bounds__(DCP_WG, 2) void k_g2p_adj(SimDev sim, StateView S, 
     for (int pi = blockIdx.x; pi < sim.n_patches; pi += gridDim.x) {
         const Patch P = sim.patches[pi];
         const InstDev I = sim.inst[P.inst];
-        if (P.count > DCP_STAGE_MAX) {
-            g2p_adj_spill(sim, S, G1, Gout, P, I);
-            __syncthreads();
-            continue;
-        }
-        if (tid < 3) { L.box[tid] = 1 << 30; L.box[3 + tid] = -(1 << 30); }
-        __syncthreads();
-        int lo[3] = {1 << 30, 1 << 30, 1 << 30}, hi[3] = {-(1 << 30), -(1 << 30), -(1 << 30)};
-        for (int q = tid; q < P.count; q += DCP_WG) {
-            V3<float> x = particle_x(sim, S, P, q), gv, gx1;
-            M3<float> gC;
-            g2p_adj_inputs(sim, I, G1, P, q, gv, gx1, gC);
-            int b0 = stencil_base(x.x, I.inv_dx, I.G), b1 = stencil_base(x.y, I.inv_dx, I.G),
-                b2 = stencil_base(x.z, I.inv_dx, I.G);
-            pbase[q] = pack_base(b0, b1, b2);
-            stage_g2p(rec + 5 * q, x, gv, gx1, gC);
-            lo[0] = min(lo[0], b0); lo[1] = min(lo[1], b1); lo[2] = min(lo[2], b2);
-            hi[0] = max(hi[0], b0); hi[1] = max(hi[1], b1); hi[2] = max(hi[2], b2);
-        }
         PatchBox B;
-        bbox_reduce(lo, hi, L, B);
-        if (B.vol > DCP_STILE_CAP) {
-            g2p_adj_spill(sim, S, G1, Gout, P, I);
-            __syncthreads();
-            continue;
-        }
-        for (int q = tid; q < P.count; q += DCP_WG) {
-            int b0, b1, b2;
-            unpack_base(pbase[q], b0, b1, b2);
-            L.key[q] = (unsigned short)tile_cell(B, b0, b1, b2);
-        }
-        __syncthreads();
-        patch_sort(sim, S, P, I, B, L, true);
-        patch_chunks(P, B, L, sim.chunk);
+        if (!g2p_adj_stage_sort(sim, S, G1, Gout, P, I, L, rec, pbase, B)) continue;
+        patch_chunks(P, B, L, sim.chunk > 0 ? sim.chunk : 4);
         float4* T = tiles[wv];
         for (int c = lane; c < B.vol; c += 64) T[c] = make_float4(0.f, 0.f, 0.f, 0.f);
         const int nch = L.n_chunks;

@@ -138,7 +138,171 @@ __device__ __forceinline__ void unstage_p2g(const float4* rec, bool elem, P2GRec
     r.elem = elem;
 }
 
+// Stage a patch's P2G records into LDS (all loads of the patch in flight
+// together), reduce its stencil box and sort it by base cell.  Returns false
+// after scattering the patch with global atomics when it does not fit the
+// LDS tile (status bit 4).
+__device__ __forceinline__ bool p2g_stage_sort(const SimDev& sim, const StateView& S, const Patch& P,
+                                               const InstDev& I, int s, int g, SortLDS& L, float4* rec, int* pbase,
+                                               int* tb_touched, PatchBox& B, TileBlocks& TB) {
+    const int tid = threadIdx.x;
+    if (P.count > DCP_STAGE_MAX) {
+        if (tid == 0) atomicOr(sim.status, (unsigned)DCP_STATUS_TILE_SPILL);
+        p2g_spill(sim, S, P, I, s, g);
+        __syncthreads();
+        return false;
+    }
+    if (tid < 3) { L.box[tid] = 1 << 30; L.box[3 + tid] = -(1 << 30); }
+    __syncthreads();
+    int lo0 = 1 << 30, lo1 = 1 << 30, lo2 = 1 << 30, hi0 = -(1 << 30), hi1 = -(1 << 30), hi2 = -(1 << 30);
+    for (int q = tid; q < P.count; q += DCP_WG) {
+        P2GRec r;
+        p2g_record(sim, I, S, P, q, s, g, r);
+        int b0 = stencil_base(r.x.x, I.inv_dx, I.G), b1 = stencil_base(r.x.y, I.inv_dx, I.G),
+            b2 = stencil_base(r.x.z, I.inv_dx, I.G);
+        pbase[q] = pack_base(b0, b1, b2);
+        stage_p2g(rec + 6 * q, r);
+        lo0 = min(lo0, b0); lo1 = min(lo1, b1); lo2 = min(lo2, b2);
+        hi0 = max(hi0, b0); hi1 = max(hi1, b1); hi2 = max(hi2, b2);
+    }
+    bbox_reduce(lo0, lo1, lo2, hi0, hi1, hi2, L, B);
+    if (B.vol > DCP_STILE_CAP) {
+        if (tid == 0) atomicOr(sim.status, (unsigned)DCP_STATUS_TILE_SPILL);
+        p2g_spill(sim, S, P, I, s, g);
+        __syncthreads();
+        return false;
+    }
+    TB = tile_blocks(B.l0, B.l1, B.l2, B.e0, B.e1, B.e2);
+    for (int t = tid; t < TB.n && t < DCP_TB_CAP; t += DCP_WG) tb_touched[t] = 0;
+    for (int q = tid; q < P.count; q += DCP_WG) {
+        int b0, b1, b2;
+        unpack_base(pbase[q], b0, b1, b2);
+        L.key[q] = (unsigned short)tile_cell(B, b0, b1, b2);
+    }
+    __syncthreads();
+    patch_sort(sim, S, P, I, B, L, true);
+    return true;
+}
+
+// Add one tile node's (m, m v) into the instance grid and note its 4^3 block.
+__device__ __forceinline__ void p2g_flush_node(const SimDev& sim, const InstDev& I, const Patch& P, const PatchBox& B,
+                                               const TileBlocks& TB, int* tb_touched, int ci, int cj, int ck, float4 a) {
+    if (a.x == 0.f && a.y == 0.f && a.z == 0.f && a.w == 0.f) return;
+    ci += B.l0; cj += B.l1; ck += B.l2;
+    long long gi = cell_index(I, ci, cj, ck);
+    atomicAdd(&sim.acc[gi].x, a.x); atomicAdd(&sim.acc[gi].y, a.y);
+    atomicAdd(&sim.acc[gi].z, a.z); atomicAdd(&sim.acc[gi].w, a.w);
+    if (TB.n <= DCP_TB_CAP)
+        tb_touched[((ci >> 2) - TB.bx0) * TB.nby * TB.nbz + ((cj >> 2) - TB.by0) * TB.nbz + ((ck >> 2) - TB.bz0)] = 1;
+    else
+        mark_block(sim, I, P.inst, ci, cj, ck);
+}
+
+// one activation per touched grid block of the tile
+__device__ __forceinline__ void p2g_mark_blocks(const SimDev& sim, const InstDev& I, const Patch& P,
+                                                const TileBlocks& TB, const int* tb_touched) {
+    if (TB.n > DCP_TB_CAP) return;
+    for (int t = threadIdx.x; t < TB.n; t += DCP_WG) {
+        if (tb_touched[t]) {
+            int bz = t % TB.nbz, r = t / TB.nbz;
+            int by = r % TB.nby, bx = r / TB.nby;
+            mark_block(sim, I, P.inst, (TB.bx0 + bx) * 4, (TB.by0 + by) * 4, (TB.bz0 + bz) * 4);
+        }
+    }
+}
+
+// P2G of one patch, compute-then-gather.  One particle per lane: the lane
+// builds its particle's record in registers, the patch is binned by base cell
+// (patch_bin: LDS-atomic ranks, one packed scan), then three passes over the
+// stencil's x-layer i: (1) each lane computes its particle's nine
+// contributions to layer i and writes them to LDS slots [(j, k)][position];
+// (2) one task per (occupied cell, slot) sums the cell's run; (3) the lane
+// owning tile node n adds the nine run sums of the base cells n - (i, j, k).
+// Node sums stay in registers across the passes and are flushed once into the
+// instance grid with float atomics (tiles of neighbouring patches overlap).
 __global__ __launch_bounds__(DCP_WG, 2) void k_p2g(SimDev sim, StateView S, int s, int g) {
+    __shared__ BinLDS L;
+    __shared__ float4 contrib[9 * DCP_SORT_MAX];
+    __shared__ int tb_touched[DCP_TB_CAP];
+    const int tid = threadIdx.x;
+    bin_clear(L);
+    if (tid < 3) { L.box[tid] = 1 << 30; L.box[3 + tid] = -(1 << 30); }
+    __syncthreads();
+    for (int pi = blockIdx.x; pi < sim.n_patches; pi += gridDim.x) {
+        const Patch P = sim.patches[pi];
+        const InstDev I = sim.inst[P.inst];
+        bool tiled = P.count <= DCP_WG;
+        if (tiled) {
+            const bool act = tid < P.count;
+            P2GRec r;
+            int b0 = 0, b1 = 0, b2 = 0;
+            if (act) {
+                p2g_record(sim, I, S, P, tid, s, g, r);
+                b0 = stencil_base(r.x.x, I.inv_dx, I.G);
+                b1 = stencil_base(r.x.y, I.inv_dx, I.G);
+                b2 = stencil_base(r.x.z, I.inv_dx, I.G);
+            }
+            PatchBox B;
+            box_reduce(b0, b1, b2, act, L.box, B);
+            tiled = B.vol <= DCP_STILE_CAP;
+            if (tiled) {
+                const TileBlocks TB = tile_blocks(B.l0, B.l1, B.l2, B.e0, B.e1, B.e2);
+                for (int t = tid; t < TB.n && t < DCP_TB_CAP; t += DCP_WG) tb_touched[t] = 0;
+                const int pos = patch_bin(act ? tile_cell(B, b0, b1, b2) : -1, B, L);
+                // this lane's tile nodes: tid and tid + DCP_WG (vol <= 2 * DCP_WG)
+                int nc[2][3];
+                bool has[2];
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const int c = tid + h * DCP_WG;
+                    has[h] = c < B.vol;
+                    const int ck = c % B.e2, t = c / B.e2;
+                    nc[h][0] = t / B.e1; nc[h][1] = t % B.e1; nc[h][2] = ck;
+                }
+                float4 a[2] = {make_float4(0.f, 0.f, 0.f, 0.f), make_float4(0.f, 0.f, 0.f, 0.f)};
+#pragma unroll
+                for (int i = 0; i < 3; ++i) {
+                    if (act) {
+                        Stencil st;
+                        make_stencil(r.x, I.inv_dx, I.G, st, nullptr);
+#pragma unroll
+                        for (int j = 0; j < 3; ++j)
+#pragma unroll
+                            for (int k = 0; k < 3; ++k) {
+                                float mm, m0, m1, m2;
+                                p2g_contrib(I, st, r, i, j, k, mm, m0, m1, m2);
+                                contrib[(j * 3 + k) * DCP_SORT_MAX + pos] = make_float4(mm, m0, m1, m2);
+                            }
+                    }
+                    __syncthreads();
+                    run_sums(L, contrib);
+                    __syncthreads();
+#pragma unroll
+                    for (int h = 0; h < 2; ++h) {
+                        const float4 v = node_sum(L, B, P.count, contrib, nc[h][0], nc[h][1], nc[h][2], i, has[h]);
+                        a[h].x += v.x; a[h].y += v.y; a[h].z += v.z; a[h].w += v.w;
+                    }
+                    __syncthreads();
+                }
+#pragma unroll
+                for (int h = 0; h < 2; ++h)
+                    if (has[h]) p2g_flush_node(sim, I, P, B, TB, tb_touched, nc[h][0], nc[h][1], nc[h][2], a[h]);
+                __syncthreads();
+                p2g_mark_blocks(sim, I, P, TB, tb_touched);
+            }
+        }
+        if (!tiled) {
+            if (tid == 0) atomicOr(sim.status, (unsigned)DCP_STATUS_TILE_SPILL);
+            p2g_spill(sim, S, P, I, s, g);
+        }
+        // next patch: empty counts and box
+        bin_clear(L);
+        if (tid < 3) { L.box[tid] = 1 << 30; L.box[3 + tid] = -(1 << 30); }
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(DCP_WG, 2) void k_p2g_chunked(SimDev sim, StateView S, int s, int g) {
     __shared__ SortLDS L;
     __shared__ float4 tiles[DCP_WG / 64][DCP_STILE_CAP];
     __shared__ float4 rec[DCP_STAGE_MAX * 6];
@@ -149,43 +313,9 @@ __global__ __launch_bounds__(DCP_WG, 2) void k_p2g(SimDev sim, StateView S, int 
         const Patch P = sim.patches[pi];
         const InstDev I = sim.inst[P.inst];
         const bool elem = P.cls == CLS_ELEMENT;
-        if (P.count > DCP_STAGE_MAX) {
-            if (tid == 0) atomicOr(sim.status, (unsigned)DCP_STATUS_TILE_SPILL);
-            p2g_spill(sim, S, P, I, s, g);
-            __syncthreads();
-            continue;
-        }
-        // stage every particle's record (all loads of the patch in flight together)
-        if (tid < 3) { L.box[tid] = 1 << 30; L.box[3 + tid] = -(1 << 30); }
-        __syncthreads();
-        int lo[3] = {1 << 30, 1 << 30, 1 << 30}, hi[3] = {-(1 << 30), -(1 << 30), -(1 << 30)};
-        for (int q = tid; q < P.count; q += DCP_WG) {
-            P2GRec r;
-            p2g_record(sim, I, S, P, q, s, g, r);
-            int b0 = stencil_base(r.x.x, I.inv_dx, I.G), b1 = stencil_base(r.x.y, I.inv_dx, I.G),
-                b2 = stencil_base(r.x.z, I.inv_dx, I.G);
-            pbase[q] = pack_base(b0, b1, b2);
-            stage_p2g(rec + 6 * q, r);
-            lo[0] = min(lo[0], b0); lo[1] = min(lo[1], b1); lo[2] = min(lo[2], b2);
-            hi[0] = max(hi[0], b0); hi[1] = max(hi[1], b1); hi[2] = max(hi[2], b2);
-        }
         PatchBox B;
-        bbox_reduce(lo, hi, L, B);
-        if (B.vol > DCP_STILE_CAP) {
-            if (tid == 0) atomicOr(sim.status, (unsigned)DCP_STATUS_TILE_SPILL);
-            p2g_spill(sim, S, P, I, s, g);
-            __syncthreads();
-            continue;
-        }
-        const TileBlocks TB = tile_blocks(B.l0, B.l1, B.l2, B.e0, B.e1, B.e2);
-        for (int t = tid; t < TB.n && t < DCP_TB_CAP; t += DCP_WG) tb_touched[t] = 0;
-        for (int q = tid; q < P.count; q += DCP_WG) {
-            int b0, b1, b2;
-            unpack_base(pbase[q], b0, b1, b2);
-            L.key[q] = (unsigned short)tile_cell(B, b0, b1, b2);
-        }
-        __syncthreads();
-        patch_sort(sim, S, P, I, B, L, true);
+        TileBlocks TB;
+        if (!p2g_stage_sort(sim, S, P, I, s, g, L, rec, pbase, tb_touched, B, TB)) continue;
         patch_chunks(P, B, L, sim.chunk);
         float4* T = tiles[wv];
         for (int c = lane; c < B.vol; c += 64) T[c] = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -242,30 +372,11 @@ __global__ __launch_bounds__(DCP_WG, 2) void k_p2g(SimDev sim, StateView S, int 
                 float4 b = tiles[w][c];
                 a.x += b.x; a.y += b.y; a.z += b.z; a.w += b.w;
             }
-            if (a.x != 0.f || a.y != 0.f || a.z != 0.f || a.w != 0.f) {
-                int ck = c % B.e2, t = c / B.e2;
-                int cj = t % B.e1, ci = t / B.e1;
-                ci += B.l0; cj += B.l1; ck += B.l2;
-                long long gi = cell_index(I, ci, cj, ck);
-                atomicAdd(&sim.acc[gi].x, a.x); atomicAdd(&sim.acc[gi].y, a.y);
-                atomicAdd(&sim.acc[gi].z, a.z); atomicAdd(&sim.acc[gi].w, a.w);
-                if (TB.n <= DCP_TB_CAP)
-                    tb_touched[((ci >> 2) - TB.bx0) * TB.nby * TB.nbz + ((cj >> 2) - TB.by0) * TB.nbz + ((ck >> 2) - TB.bz0)] = 1;
-                else
-                    mark_block(sim, I, P.inst, ci, cj, ck);
-            }
+            int ck = c % B.e2, t = c / B.e2;
+            p2g_flush_node(sim, I, P, B, TB, tb_touched, t / B.e1, t % B.e1, ck, a);
         }
         __syncthreads();
-        // one activation per touched grid block of the tile
-        if (TB.n <= DCP_TB_CAP) {
-            for (int t = tid; t < TB.n; t += DCP_WG) {
-                if (tb_touched[t]) {
-                    int bz = t % TB.nbz, r = t / TB.nbz;
-                    int by = r % TB.nby, bx = r / TB.nby;
-                    mark_block(sim, I, P.inst, (TB.bx0 + bx) * 4, (TB.by0 + by) * 4, (TB.bz0 + bz) * 4);
-                }
-            }
-        }
+        p2g_mark_blocks(sim, I, P, TB, tb_touched);
         __syncthreads();
     }
 }

@@ -75,12 +75,14 @@ __device__ __forceinline__ void patch_bbox(const SimDev& sim, const StateView& S
 }
 
 // Bounding box from per-thread min/max base cells (block reduction).
-__device__ __forceinline__ void bbox_reduce(int lo[3], int hi[3], SortLDS& L, PatchBox& B) {
+__device__ __forceinline__ void bbox_reduce(int lo0, int lo1, int lo2, int hi0, int hi1, int hi2, SortLDS& L,
+                                            PatchBox& B) {
     const int tid = threadIdx.x;
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-        int a = wave_min(lo[d]), b = wave_max(hi[d]);
-        if ((tid & 63) == 0) { atomicMin(&L.box[d], a); atomicMax(&L.box[3 + d], b); }
+    lo0 = wave_min(lo0); lo1 = wave_min(lo1); lo2 = wave_min(lo2);
+    hi0 = wave_max(hi0); hi1 = wave_max(hi1); hi2 = wave_max(hi2);
+    if ((tid & 63) == 0) {
+        atomicMin(&L.box[0], lo0); atomicMin(&L.box[1], lo1); atomicMin(&L.box[2], lo2);
+        atomicMax(&L.box[3], hi0); atomicMax(&L.box[4], hi1); atomicMax(&L.box[5], hi2);
     }
     __syncthreads();
     B.l0 = L.box[0]; B.l1 = L.box[1]; B.l2 = L.box[2];
@@ -138,6 +140,7 @@ __device__ __forceinline__ void patch_sort(const SimDev& sim, const StateView& S
             todo &= ~grp;
         }
         // waves of the round add their counts in wave order
+#pragma unroll
         for (int w = 0; w < DCP_WG / 64; ++w) {
             if (wv == w && act) {
                 int old = L.cnt[key];
@@ -149,12 +152,14 @@ __device__ __forceinline__ void patch_sort(const SimDev& sim, const StateView& S
     }
     // exclusive scan of the counts (vol <= DCP_STILE_CAP <= 4 per thread)
     int v[4], s = 0;
+#pragma unroll
     for (int t = 0; t < 4; ++t) {
         int c = tid * 4 + t;
         v[t] = c < B.vol ? L.cnt[c] : 0;
         s += v[t];
     }
     int incl = s;
+#pragma unroll
     for (int o = 1; o < 64; o <<= 1) {
         int y = __shfl_up(incl, o, 64);
         if (lane >= o) incl += y;
@@ -164,6 +169,7 @@ __device__ __forceinline__ void patch_sort(const SimDev& sim, const StateView& S
     int wbase = 0;
     for (int w = 0; w < wv; ++w) wbase += L.wsum[w];
     int run = wbase + incl - s;
+#pragma unroll
     for (int t = 0; t < 4; ++t) {
         int c = tid * 4 + t;
         if (c < B.vol) L.cnt[c] = run;
@@ -179,12 +185,148 @@ __device__ __forceinline__ void patch_sort(const SimDev& sim, const StateView& S
     __syncthreads();
 }
 
+// ---------------------------------------------------------------------------
+// Binning of a patch for the compute-then-gather scatters (one particle per
+// lane, P.count <= DCP_WG): per-cell counts with LDS atomics (the returned old
+// count is the particle's rank inside its cell; the order among particles of
+// one cell follows the LDS atomic order), then one block scan of packed
+// (count, occupied) words giving every cell's offset and every occupied
+// cell's run index, and the run table.  No permutation is stored: a particle's
+// sorted position is offset(cell) + rank.
+// ---------------------------------------------------------------------------
+struct BinLDS {
+    int box[6];
+    int wsum[DCP_WG / 64];
+    int n_runs;
+    int off[DCP_STILE_CAP];                    // counts, then exclusive offsets
+    unsigned short run_start[DCP_SORT_MAX];    // run (occupied cell) -> first position
+    unsigned short run_len[DCP_SORT_MAX];
+};
+
+// Zero the cell counts; the box barrier that follows orders this before use.
+__device__ __forceinline__ void bin_clear(BinLDS& L) {
+    for (int c = threadIdx.x; c < DCP_STILE_CAP; c += DCP_WG) L.off[c] = 0;
+}
+
+// `key` < 0 for a lane without a particle.  Returns the sorted position.
+__device__ __forceinline__ int patch_bin(int key, const PatchBox& B, BinLDS& L) {
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int rank = key >= 0 ? atomicAdd(&L.off[key], 1) : 0;
+    __syncthreads();
+    int v[4], sum = 0;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        const int c = tid * 4 + t;
+        const int n = c < B.vol ? L.off[c] : 0;
+        v[t] = n | (n > 0 ? 1 << 16 : 0);
+        sum += v[t];
+    }
+    int incl = sum;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += y;
+    }
+    if (lane == 63) L.wsum[wv] = incl;
+    __syncthreads();
+    int run = incl - sum, total = 0;
+#pragma unroll
+    for (int w = 0; w < DCP_WG / 64; ++w) {
+        if (w < wv) run += L.wsum[w];
+        total += L.wsum[w];
+    }
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        const int c = tid * 4 + t;
+        if (c < B.vol) {
+            const int n = v[t] & 0xffff;
+            L.off[c] = run & 0xffff;
+            if (n) {
+                L.run_start[run >> 16] = (unsigned short)(run & 0xffff);
+                L.run_len[run >> 16] = (unsigned short)n;
+            }
+        }
+        run += v[t];
+    }
+    if (tid == 0) L.n_runs = total >> 16;
+    __syncthreads();
+    return key >= 0 ? L.off[key] + rank : -1;
+}
+
+// Two-stage reduction of one stencil x-layer's contributions.  `contrib`
+// holds one float4 per (slot, sorted position) at [slot * DCP_SORT_MAX +
+// position], slot = (j, k) of the node relative to the particle's base cell.
+//
+// Stage 1 (run_sums): one task per (run, slot) sums the run's entries of that
+// slot and leaves the sum in the run's first entry.  Loads go four at a time.
+__device__ __forceinline__ void run_sums(const BinLDS& L, float4* contrib) {
+    const int ntask = 9 * L.n_runs;
+    for (int t = threadIdx.x; t < ntask; t += DCP_WG) {
+        const int r = t / 9, slot = t - 9 * r;
+        float4* c = contrib + slot * DCP_SORT_MAX + L.run_start[r];
+        const int len = L.run_len[r];
+        float4 a = c[0];
+        int u = 1;
+        for (; u + 3 < len; u += 4) {
+            const float4 b0 = c[u], b1 = c[u + 1], b2 = c[u + 2], b3 = c[u + 3];
+            a.x += (b0.x + b1.x) + (b2.x + b3.x); a.y += (b0.y + b1.y) + (b2.y + b3.y);
+            a.z += (b0.z + b1.z) + (b2.z + b3.z); a.w += (b0.w + b1.w) + (b2.w + b3.w);
+        }
+        for (; u < len; ++u) {
+            const float4 b = c[u];
+            a.x += b.x; a.y += b.y; a.z += b.z; a.w += b.w;
+        }
+        c[0] = a;
+    }
+}
+
+// Stage 2 (node_sum): the tile node (ci, cj, ck) adds, for its nine base cells
+// node - (i, dj, dk), the run sum of slot (dj, dk) — nine independent loads,
+// the same count for every node.  `has` = false contributes nothing.
+__device__ __forceinline__ float4 node_sum(const BinLDS& L, const PatchBox& B, int count, const float4* contrib,
+                                           int ci, int cj, int ck, int i, bool has) {
+    float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+    const int bi = ci - i;
+    if (!has || bi < 0 || bi > B.e0 - 3) return acc;
+#pragma unroll
+    for (int o = 0; o < 9; ++o) {
+        const int dj = o / 3, dk = o % 3;
+        const int bj = cj - dj, bk = ck - dk;
+        if (bj >= 0 && bj <= B.e1 - 3 && bk >= 0 && bk <= B.e2 - 3) {
+            const int kb = (bi * B.e1 + bj) * B.e2 + bk;
+            const int a = L.off[kb];
+            const int e = kb + 1 < B.vol ? L.off[kb + 1] : count;
+            if (e > a) {
+                const float4 v = contrib[o * DCP_SORT_MAX + a];
+                acc.x += v.x; acc.y += v.y; acc.z += v.z; acc.w += v.w;
+            }
+        }
+    }
+    return acc;
+}
+
+// Box of the patch's base cells from one candidate per lane.
+__device__ __forceinline__ void box_reduce(int b0, int b1, int b2, bool act, int* box, PatchBox& B) {
+    const int big = 1 << 30;
+    const int lo0 = wave_min(act ? b0 : big), lo1 = wave_min(act ? b1 : big), lo2 = wave_min(act ? b2 : big);
+    const int hi0 = wave_max(act ? b0 : -big), hi1 = wave_max(act ? b1 : -big), hi2 = wave_max(act ? b2 : -big);
+    if ((threadIdx.x & 63) == 0) {
+        atomicMin(&box[0], lo0); atomicMin(&box[1], lo1); atomicMin(&box[2], lo2);
+        atomicMax(&box[3], hi0); atomicMax(&box[4], hi1); atomicMax(&box[5], hi2);
+    }
+    __syncthreads();
+    B.l0 = box[0]; B.l1 = box[1]; B.l2 = box[2];
+    B.e0 = box[3] - B.l0 + 3; B.e1 = box[4] - B.l1 + 3; B.e2 = box[5] - B.l2 + 3;
+    B.vol = B.e0 * B.e1 * B.e2;
+}
+
 // Block-wide exclusive scan of one int per key (vol <= DCP_STILE_CAP, 4 keys
 // per thread); returns the total.  `vals` are this thread's 4 values.
 __device__ __forceinline__ int block_exscan4(int vals[4], int out[4], SortLDS& L) {
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     int s = vals[0] + vals[1] + vals[2] + vals[3];
     int incl = s;
+#pragma unroll
     for (int o = 1; o < 64; o <<= 1) {
         int y = __shfl_up(incl, o, 64);
         if (lane >= o) incl += y;
@@ -193,11 +335,13 @@ __device__ __forceinline__ int block_exscan4(int vals[4], int out[4], SortLDS& L
     if (lane == 63) L.wsum[wv] = incl;
     __syncthreads();
     int wbase = 0, total = 0;
+#pragma unroll
     for (int w = 0; w < DCP_WG / 64; ++w) {
         if (w < wv) wbase += L.wsum[w];
         total += L.wsum[w];
     }
     int run = wbase + incl - s;
+#pragma unroll
     for (int t = 0; t < 4; ++t) { out[t] = run; run += vals[t]; }
     return total;
 }
@@ -207,6 +351,7 @@ __device__ __forceinline__ int block_exscan4(int vals[4], int out[4], SortLDS& L
 __device__ __forceinline__ void patch_chunks(const Patch& P, const PatchBox& B, SortLDS& L, int K) {
     const int tid = threadIdx.x;
     int nch[4], off[4], cnt[4], choff[4];
+#pragma unroll
     for (int t = 0; t < 4; ++t) {
         int c = tid * 4 + t;
         off[t] = c < B.vol ? L.cnt[c] : 0;
@@ -215,6 +360,7 @@ __device__ __forceinline__ void patch_chunks(const Patch& P, const PatchBox& B, 
         nch[t] = (cnt[t] + K - 1) / K;
     }
     int total = block_exscan4(nch, choff, L);
+#pragma unroll
     for (int t = 0; t < 4; ++t) {
         int c = tid * 4 + t;
         for (int j = 0; j < nch[t]; ++j) {
