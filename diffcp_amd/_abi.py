@@ -40,8 +40,8 @@ class InstanceDesc(ctypes.Structure):
 class Options(ctypes.Structure):
     _fields_ = [
         ("max_states", c_int), ("precision", c_int), ("svd_grad", c_int),
-        ("per_element_moduli", c_int), ("patch_size", c_int), ("p2g_mode", c_int),
-        ("grid_wgs", c_int), ("reserved", c_int),
+        ("per_element_moduli", c_int), ("patch_size", c_int), ("scatter_wgs", c_int),
+        ("grid_wgs", c_int), ("bin_cells", c_int),
     ]
 
 

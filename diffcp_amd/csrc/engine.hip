@@ -51,16 +51,37 @@ uint32_t morton(float x, float y, float z) {
     return spread10(q(x)) | (spread10(q(y)) << 1) | (spread10(q(z)) << 2);
 }
 
-// Spatial bin (cubes of kBinCells grid cells) in the high bits, Morton code of
+// Spatial bin (cubes of `cells` grid cells) in the high bits, Morton code of
 // the position in the low bits: sorting by this key groups particles by bin,
 // and patches never straddle a bin, so a patch's tile stays compact.
-constexpr int kBinCells = 6;
-uint64_t bin_of(float x, float y, float z, int G) {
-    auto b = [&](float a) { int t = (int)std::floor(a * G / kBinCells); return (uint64_t)std::min(63, std::max(0, t)); };
-    return (b(x) * 64 + b(y)) * 64 + b(z);
+uint64_t bin_of(float x, float y, float z, int G, int cells) {
+    auto b = [&](float a) { int t = (int)std::floor(a * G / cells); return (uint64_t)std::min(127, std::max(0, t)); };
+    return (b(x) * 128 + b(y)) * 128 + b(z);
 }
-uint64_t spatial_key(float x, float y, float z, int G) {
-    return (bin_of(x, y, z, G) << 32) | morton(x, y, z);
+uint64_t spatial_key(float x, float y, float z, int G, int cells) {
+    return (bin_of(x, y, z, G, cells) << 32) | morton(x, y, z);
+}
+
+// Bin side (grid cells) for which one bin of a cloth sheet holds about one
+// patch: particles per occupied cell rho, a bin of s x s occupied cells holds
+// s^2 rho.  Small bins keep a patch's box small when the sheet bends.
+int bin_cells(const std::vector<float>& xv, const std::vector<float>& xe, int G, int patch) {
+    std::vector<uint64_t> cells;
+    auto add = [&](const std::vector<float>& x) {
+        for (size_t i = 0; i + 2 < x.size(); i += 3) {
+            uint64_t a = (uint64_t)std::min(G - 1, std::max(0, (int)(x[i] * G)));
+            uint64_t b = (uint64_t)std::min(G - 1, std::max(0, (int)(x[i + 1] * G)));
+            uint64_t c = (uint64_t)std::min(G - 1, std::max(0, (int)(x[i + 2] * G)));
+            cells.push_back((a * G + b) * G + c);
+        }
+    };
+    add(xv);
+    add(xe);
+    if (cells.empty()) return 6;
+    std::sort(cells.begin(), cells.end());
+    const double occupied = (double)(std::unique(cells.begin(), cells.end()) - cells.begin());
+    const double rho = (xv.size() + xe.size()) / 3.0 / occupied;
+    return std::max(2, std::min(8, (int)std::floor(std::sqrt(patch / rho))));
 }
 
 // host FP32 cofactor inverse (same formula as the device minv)
@@ -157,7 +178,9 @@ struct dcp_ctx {
         if (!h.empty()) HIP_OK(hipMemcpy(*p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
         return 0;
     }
-    int grid_patches() const { return std::max(1, std::min(n_patches, n_cu * 8)); }
+    int grid_patches() const {
+        return std::max(1, std::min(n_patches, opt.scatter_wgs > 0 ? opt.scatter_wgs : n_cu * 8));
+    }
     int grid_elems() const { return std::max(1, std::min((NE + DCP_WG - 1) / DCP_WG, n_cu * 16)); }
     // B x per workgroups walking the per-instance active block lists
     int grid_gridop() const {
@@ -191,10 +214,7 @@ static hipError_t time_mark(dcp_ctx* c) {
     } while (0)
 
 static int launch_p2g(dcp_ctx* c, StateView S, int s, int g) {
-    if (c->sim.chunk > 0)
-        TIMED_LAUNCH(DCP_KERNEL_P2G, k_p2g_chunked, dim3(c->grid_patches()), dim3(DCP_WG), 0, c->stream, c->sim, S, s, g);
-    else
-        TIMED_LAUNCH(DCP_KERNEL_P2G, k_p2g, dim3(c->grid_patches()), dim3(DCP_WG), 0, c->stream, c->sim, S, s, g);
+    TIMED_LAUNCH(DCP_KERNEL_P2G, k_p2g, dim3(c->grid_patches()), dim3(DCP_WG), 0, c->stream, c->sim, S, s, g);
     return 0;
 }
 
@@ -219,11 +239,7 @@ static int launch_adjoint(dcp_ctx* c, int s, int g, StateView S, StateView G1, S
     if (launch_prep(c, S, c->scratch_view(), fresh)) return -1;
     if (launch_p2g(c, S, s, g)) return -1;
     TIMED_LAUNCH(DCP_KERNEL_GRIDOP, k_gridop, dim3(c->grid_gridop()), dim3(DCP_WG), 0, c->stream, c->sim, 1);
-    if (c->sim.chunk > 0)
-        TIMED_LAUNCH(DCP_KERNEL_G2P_ADJ, k_g2p_adj_chunked, dim3(c->grid_patches()), dim3(DCP_WG), 0, c->stream, c->sim, S,
-                     G1, Gout);
-    else
-        TIMED_LAUNCH(DCP_KERNEL_G2P_ADJ, k_g2p_adj, dim3(c->grid_patches()), dim3(DCP_WG), 0, c->stream, c->sim, S, G1, Gout);
+    TIMED_LAUNCH(DCP_KERNEL_G2P_ADJ, k_g2p_adj, dim3(c->grid_patches()), dim3(DCP_WG), 0, c->stream, c->sim, S, G1, Gout);
     TIMED_LAUNCH(DCP_KERNEL_GRIDOP_ADJ, k_gridop_adj, dim3(c->grid_gridop()), dim3(DCP_WG), 0, c->stream, c->sim);
     TIMED_LAUNCH(DCP_KERNEL_P2G_ADJ, k_p2g_adj, dim3(c->grid_patches()), dim3(DCP_WG), 0, c->stream, c->sim, S, Gout, s, g);
     if (c->opt.precision == 1)
@@ -266,6 +282,7 @@ static int launch_loss_adj(dcp_ctx* c, StateView S, StateView G) {
 // creation
 // ---------------------------------------------------------------------------
 static int build(dcp_ctx* c, const dcp_instance_desc* d, int B) {
+    const int patch_size = c->opt.patch_size > 0 ? std::min(c->opt.patch_size, 256) : 256;
     c->B = B;
     c->hi.resize(B);
     c->inst_h.resize(B);
@@ -289,8 +306,15 @@ static int build(dcp_ctx* c, const dcp_instance_desc* d, int B) {
         for (int t : H.tri)
             if (t < 0 || t >= H.nv) return set_err("triangle index out of range");
         // spatial (Morton) order of vertices and element centroids
+        std::vector<float> xc(3 * (size_t)H.ne);
+        for (int e = 0; e < H.ne; ++e)
+            for (int d = 0; d < 3; ++d)
+                xc[3 * e + d] = (H.x0v[3 * H.tri[3 * e] + d] + H.x0v[3 * H.tri[3 * e + 1] + d] +
+                                 H.x0v[3 * H.tri[3 * e + 2] + d]) / 3;
+        const int bcells = c->opt.bin_cells > 0 ? std::max(2, std::min(16, (int)c->opt.bin_cells))
+                           : c->opt.bin_cells < 0 ? bin_cells(H.x0v, xc, H.G, patch_size) : 6;
         std::vector<uint64_t> code(H.nv);
-        for (int i = 0; i < H.nv; ++i) code[i] = spatial_key(H.x0v[3 * i], H.x0v[3 * i + 1], H.x0v[3 * i + 2], H.G);
+        for (int i = 0; i < H.nv; ++i) code[i] = spatial_key(H.x0v[3 * i], H.x0v[3 * i + 1], H.x0v[3 * i + 2], H.G, bcells);
         H.vperm.resize(H.nv);
         std::iota(H.vperm.begin(), H.vperm.end(), 0);
         std::stable_sort(H.vperm.begin(), H.vperm.end(), [&](int a, int bb) { return code[a] < code[bb]; });
@@ -299,14 +323,7 @@ static int build(dcp_ctx* c, const dcp_instance_desc* d, int B) {
         H.vbin.resize(H.nv);
         for (int i = 0; i < H.nv; ++i) H.vbin[i] = code[H.vperm[i]] >> 32;
         std::vector<uint64_t> ecode(H.ne);
-        for (int e = 0; e < H.ne; ++e) {
-            float cx = 0, cy = 0, cz = 0;
-            for (int k = 0; k < 3; ++k) {
-                int v = H.tri[3 * e + k];
-                cx += H.x0v[3 * v]; cy += H.x0v[3 * v + 1]; cz += H.x0v[3 * v + 2];
-            }
-            ecode[e] = spatial_key(cx / 3, cy / 3, cz / 3, H.G);
-        }
+        for (int e = 0; e < H.ne; ++e) ecode[e] = spatial_key(xc[3 * e], xc[3 * e + 1], xc[3 * e + 2], H.G, bcells);
         H.eperm.resize(H.ne);
         std::iota(H.eperm.begin(), H.eperm.end(), 0);
         std::stable_sort(H.eperm.begin(), H.eperm.end(), [&](int a, int bb) { return ecode[a] < ecode[bb]; });
@@ -381,7 +398,7 @@ static int build(dcp_ctx* c, const dcp_instance_desc* d, int B) {
     }
     // patches: point patches (vertex, grasper, cube) then element patches
     // particles per patch: the scatters stage up to 256 particle records in LDS
-    int ps = c->opt.patch_size > 0 ? std::min(c->opt.patch_size, 256) : 256;
+    const int ps = patch_size;
     auto chunk = [&](int b, int cls, int start, int count) {
         for (int o = 0; o < count; o += ps) c->patches_h.push_back(Patch{b, cls, start + o, std::min(ps, count - o)});
     };
@@ -419,8 +436,6 @@ static int build(dcp_ctx* c, const dcp_instance_desc* d, int B) {
     S.B = B; S.NP = np; S.NE = ne; S.n_patches = c->n_patches;
     S.precision = c->opt.precision; S.svd_grad = c->opt.svd_grad; S.per_elem = c->opt.per_element_moduli;
     S.max_active = flags;
-    // 0: compute-then-gather P2G; K > 0: chunks of K particles with a segmented sum
-    S.chunk = c->opt.p2g_mode > 0 ? std::min(c->opt.p2g_mode, 64) : 0;
     c->slot_floats = 15 * (size_t)np + 12 * (size_t)ne;
     c->max_states = std::max(2, c->opt.max_states);
     Patch* pd; int *trid, *eid, *iod, *ild; float* trd;

@@ -269,7 +269,6 @@ __device__ __forceinline__ void grid_velocity(const InstDev& I, float4 c, int i,
 // kernels -------------------------------------------------------------------
 template <typename PT> __global__ void k_prep(SimDev sim, StateView S, StateView Snext, int fresh);
 __global__ void k_p2g(SimDev sim, StateView S, int s, int g);
-__global__ void k_p2g_chunked(SimDev sim, StateView S, int s, int g);
 __global__ void k_gridop(SimDev sim, int keep);
 __global__ void k_g2p(SimDev sim, StateView S, StateView Sn);
 __global__ void k_loss_min(SimDev sim, StateView S, const float* targets, const int4* tchunks, int n_chunks,
@@ -278,7 +277,6 @@ __global__ void k_loss_sum(const float* tmin, const int* toff, const int* tcount
 __global__ void k_binning_keys(SimDev sim, StateView S, int* keys);
 
 __global__ void k_g2p_adj(SimDev sim, StateView S, StateView G1, StateView Gout);
-__global__ void k_g2p_adj_chunked(SimDev sim, StateView S, StateView G1, StateView Gout);
 __global__ void k_gridop_adj(SimDev sim);
 __global__ void k_p2g_adj(SimDev sim, StateView S, StateView Gout, int s, int g);
 template <typename PT> __global__ void k_prep_adj(SimDev sim, StateView S, StateView G1, StateView Gout, int fresh);

@@ -94,69 +94,6 @@ __device__ __forceinline__ void g2p_adj_spill(const SimDev& sim, const StateView
     }
 }
 
-// G2P reverse of one patch: the per-particle dL/dx is a gather; the scatter of
-// dL/d grid_v_out uses the sorted segmented reduction of k_p2g.
-// 20-float (5 x float4) LDS record of a particle's G2P-reverse inputs:
-// [x gv0] [gv1 gv2 gx0 gx1] [gx2 C00 C01 C02] [C10 C11 C12 C20] [C21 C22 - -]
-__device__ __forceinline__ void stage_g2p(float4* rec, V3<float> x, V3<float> gv, V3<float> gx1, const M3<float>& gC) {
-    rec[0] = make_float4(x.x, x.y, x.z, gv.x);
-    rec[1] = make_float4(gv.y, gv.z, gx1.x, gx1.y);
-    rec[2] = make_float4(gx1.z, gC.m[0], gC.m[1], gC.m[2]);
-    rec[3] = make_float4(gC.m[3], gC.m[4], gC.m[5], gC.m[6]);
-    rec[4] = make_float4(gC.m[7], gC.m[8], 0.f, 0.f);
-}
-__device__ __forceinline__ void unstage_g2p(const float4* rec, V3<float>& x, V3<float>& gv, V3<float>& gx1, M3<float>& gC) {
-    float4 a = rec[0], b = rec[1], c = rec[2], d = rec[3], e = rec[4];
-    x = {a.x, a.y, a.z};
-    gv = {a.w, b.x, b.y};
-    gx1 = {b.z, b.w, c.x};
-    gC.m[0] = c.y; gC.m[1] = c.z; gC.m[2] = c.w;
-    gC.m[3] = d.x; gC.m[4] = d.y; gC.m[5] = d.z; gC.m[6] = d.w;
-    gC.m[7] = e.x; gC.m[8] = e.y;
-}
-
-// Stage a patch's G2P-reverse inputs into LDS, reduce its stencil box and sort
-// it by base cell.  Returns false after the global-atomic path when the patch
-// does not fit the LDS tile.
-__device__ __forceinline__ bool g2p_adj_stage_sort(const SimDev& sim, const StateView& S, const StateView& G1,
-                                                   const StateView& Gout, const Patch& P, const InstDev& I,
-                                                   SortLDS& L, float4* rec, int* pbase, PatchBox& B) {
-    const int tid = threadIdx.x;
-    if (P.count > DCP_STAGE_MAX) {
-        g2p_adj_spill(sim, S, G1, Gout, P, I);
-        __syncthreads();
-        return false;
-    }
-    if (tid < 3) { L.box[tid] = 1 << 30; L.box[3 + tid] = -(1 << 30); }
-    __syncthreads();
-    int lo0 = 1 << 30, lo1 = 1 << 30, lo2 = 1 << 30, hi0 = -(1 << 30), hi1 = -(1 << 30), hi2 = -(1 << 30);
-    for (int q = tid; q < P.count; q += DCP_WG) {
-        V3<float> x = particle_x(sim, S, P, q), gv, gx1;
-        M3<float> gC;
-        g2p_adj_inputs(sim, I, G1, P, q, gv, gx1, gC);
-        int b0 = stencil_base(x.x, I.inv_dx, I.G), b1 = stencil_base(x.y, I.inv_dx, I.G),
-            b2 = stencil_base(x.z, I.inv_dx, I.G);
-        pbase[q] = pack_base(b0, b1, b2);
-        stage_g2p(rec + 5 * q, x, gv, gx1, gC);
-        lo0 = min(lo0, b0); lo1 = min(lo1, b1); lo2 = min(lo2, b2);
-        hi0 = max(hi0, b0); hi1 = max(hi1, b1); hi2 = max(hi2, b2);
-    }
-    bbox_reduce(lo0, lo1, lo2, hi0, hi1, hi2, L, B);
-    if (B.vol > DCP_STILE_CAP) {
-        g2p_adj_spill(sim, S, G1, Gout, P, I);
-        __syncthreads();
-        return false;
-    }
-    for (int q = tid; q < P.count; q += DCP_WG) {
-        int b0, b1, b2;
-        unpack_base(pbase[q], b0, b1, b2);
-        L.key[q] = (unsigned short)tile_cell(B, b0, b1, b2);
-    }
-    __syncthreads();
-    patch_sort(sim, S, P, I, B, L, true);
-    return true;
-}
-
 // Compute-then-gather form (the structure of k_p2g): one particle per lane with
 // its G2P-reverse inputs in registers; per x-layer i each lane computes its
 // particle's dL/d grid_v_out contributions to the layer's nine nodes into LDS
@@ -191,16 +128,18 @@ __global__ __launch_bounds__(DCP_WG, 2) void k_g2p_adj(SimDev sim, StateView S, 
             tiled = B.vol <= DCP_STILE_CAP;
             if (tiled) {
                 const int pos = patch_bin(act ? tile_cell(B, b0, b1, b2) : -1, B, L);
-                int nc[2][3];
-                bool has[2];
+                int nc[DCP_NODES][3];
+                bool has[DCP_NODES];
 #pragma unroll
-                for (int h = 0; h < 2; ++h) {
+                for (int h = 0; h < DCP_NODES; ++h) {
                     const int c = tid + h * DCP_WG;
                     has[h] = c < B.vol;
                     const int ck = c % B.e2, t = c / B.e2;
                     nc[h][0] = t / B.e1; nc[h][1] = t % B.e1; nc[h][2] = ck;
                 }
-                float4 a[2] = {make_float4(0.f, 0.f, 0.f, 0.f), make_float4(0.f, 0.f, 0.f, 0.f)};
+                float4 a[DCP_NODES];
+#pragma unroll
+                for (int h = 0; h < DCP_NODES; ++h) a[h] = make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
                 for (int i = 0; i < 3; ++i) {
                     if (act) {
@@ -235,14 +174,14 @@ __global__ __launch_bounds__(DCP_WG, 2) void k_g2p_adj(SimDev sim, StateView S, 
                     run_sums(L, contrib);
                     __syncthreads();
 #pragma unroll
-                    for (int h = 0; h < 2; ++h) {
+                    for (int h = 0; h < DCP_NODES; ++h) {
                         const float4 v = node_sum(L, B, P.count, contrib, nc[h][0], nc[h][1], nc[h][2], i, has[h]);
                         a[h].y += v.y; a[h].z += v.z; a[h].w += v.w;
                     }
                     __syncthreads();
                 }
 #pragma unroll
-                for (int h = 0; h < 2; ++h) {
+                for (int h = 0; h < DCP_NODES; ++h) {
                     if (has[h] && (a[h].y != 0.f || a[h].z != 0.f || a[h].w != 0.f)) {
                         long long gidx = cell_index(I, nc[h][0] + B.l0, nc[h][1] + B.l1, nc[h][2] + B.l2);
                         atomicAdd(&sim.gacc[gidx].y, a[h].y); atomicAdd(&sim.gacc[gidx].z, a[h].z);
@@ -254,102 +193,6 @@ __global__ __launch_bounds__(DCP_WG, 2) void k_g2p_adj(SimDev sim, StateView S, 
         if (!tiled) g2p_adj_spill(sim, S, G1, Gout, P, I);
         bin_clear(L);
         if (tid < 3) { L.box[tid] = 1 << 30; L.box[3 + tid] = -(1 << 30); }
-        __syncthreads();
-    }
-}
-
-__global__ __launch_bounds__(DCP_WG, 2) void k_g2p_adj_chunked(SimDev sim, StateView S, StateView G1, StateView Gout) {
-    __shared__ SortLDS L;
-    __shared__ float4 tiles[DCP_WG / 64][DCP_STILE_CAP];
-    __shared__ float4 rec[DCP_STAGE_MAX * 5];
-    __shared__ int pbase[DCP_STAGE_MAX];
-    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    for (int pi = blockIdx.x; pi < sim.n_patches; pi += gridDim.x) {
-        const Patch P = sim.patches[pi];
-        const InstDev I = sim.inst[P.inst];
-        PatchBox B;
-        if (!g2p_adj_stage_sort(sim, S, G1, Gout, P, I, L, rec, pbase, B)) continue;
-        patch_chunks(P, B, L, sim.chunk > 0 ? sim.chunk : 4);
-        float4* T = tiles[wv];
-        for (int c = lane; c < B.vol; c += 64) T[c] = make_float4(0.f, 0.f, 0.f, 0.f);
-        const int nch = L.n_chunks;
-        for (int ch0 = 0; ch0 < nch; ch0 += DCP_WG) {
-            const int ci = ch0 + tid;
-            const bool act = ci < nch;
-            const int key = act ? (int)L.ch_key[ci] : 0;
-            const SegInfo gi = segment_info_key(key, act);
-            // dL/dx of each particle is accumulated over all 27 nodes in the
-            // first pass; the node scatter runs in three passes (one per x-node)
-            const int s0 = act ? (int)L.ch_start[ci] : 0, len = act ? (int)L.ch_len[ci] : 0;
-#pragma unroll
-            for (int i = 0; i < 3; ++i) {
-                float acc[27];
-#pragma unroll
-                for (int o = 0; o < 27; ++o) acc[o] = 0.f;
-                for (int u = 0; u < len; ++u) {
-                    const int q = L.perm[s0 + u];
-                    V3<float> x, gv, gx1;
-                    M3<float> gC;
-                    unstage_g2p(rec + 5 * q, x, gv, gx1, gC);
-                    Stencil st;
-                    make_stencil(x, I.inv_dx, I.G, st, nullptr);
-                    float gfx[3] = {0.f, 0.f, 0.f};
-#pragma unroll
-                    for (int j = 0; j < 3; ++j)
-#pragma unroll
-                        for (int k = 0; k < 3; ++k) {
-                            float4 gvo = sim.vout[cell_index(I, st.base[0] + i, st.base[1] + j, st.base[2] + k)];
-                            float gg[3];
-                            g2p_adj_node(I, st, i, j, k, gvo, gv, gC, gg, gfx);
-                            const int o = 3 * (j * 3 + k);
-                            acc[o] += gg[0]; acc[o + 1] += gg[1]; acc[o + 2] += gg[2];
-                        }
-                    if (i == 0) {
-                        // the remaining x-nodes' share of dL/dfx, then the store
-#pragma unroll
-                        for (int i2 = 1; i2 < 3; ++i2)
-#pragma unroll
-                            for (int j = 0; j < 3; ++j)
-#pragma unroll
-                                for (int k = 0; k < 3; ++k) {
-                                    float4 gvo = sim.vout[cell_index(I, st.base[0] + i2, st.base[1] + j, st.base[2] + k)];
-                                    float gg[3];
-                                    g2p_adj_node(I, st, i2, j, k, gvo, gv, gC, gg, gfx);
-                                }
-                        g2p_adj_store(sim, I, Gout, P, q, gx1, gfx);
-                    }
-                }
-                seg_sum_n(acc, gi);
-#pragma unroll
-                for (int j = 0; j < 3; ++j)
-#pragma unroll
-                    for (int k = 0; k < 3; ++k) {
-                        const int o = 3 * (j * 3 + k);
-                        if (gi.tail) {
-                            const int c = key + (i * B.e1 + j) * B.e2 + k;
-                            float4 t = T[c];
-                            t.y += acc[o]; t.z += acc[o + 1]; t.w += acc[o + 2];
-                            T[c] = t;
-                        }
-                        tile_order();
-                    }
-            }
-        }
-        __syncthreads();
-        for (int c = tid; c < B.vol; c += DCP_WG) {
-            float4 a = tiles[0][c];
-#pragma unroll
-            for (int w = 1; w < DCP_WG / 64; ++w) {
-                float4 b = tiles[w][c];
-                a.y += b.y; a.z += b.z; a.w += b.w;
-            }
-            if (a.y != 0.f || a.z != 0.f || a.w != 0.f) {
-                int ck = c % B.e2, t = c / B.e2;
-                int cj = t % B.e1, ci = t / B.e1;
-                long long gidx = cell_index(I, ci + B.l0, cj + B.l1, ck + B.l2);
-                atomicAdd(&sim.gacc[gidx].y, a.y); atomicAdd(&sim.gacc[gidx].z, a.z); atomicAdd(&sim.gacc[gidx].w, a.w);
-            }
-        }
         __syncthreads();
     }
 }

@@ -109,81 +109,6 @@ __device__ __forceinline__ void p2g_spill(const SimDev& sim, const StateView& S,
     }
 }
 
-// P2G of one patch: stable counting sort by base cell; the run of each cell is
-// cut into chunks of at most sim.chunk particles and a lane sums its chunk's
-// stencil contributions in registers.  A wave-level segmented sum over lanes
-// of one cell follows, and the run's last lane does a plain LDS
-// read-modify-write into the wave's private tile (tail lanes of a wave hold
-// distinct cells, so for one stencil offset their writes never collide).  The
-// four wave tiles are summed in wave order and flushed once into the grid.
-// 24-float (6 x float4) LDS record of a particle's P2G inputs:
-// [x m] [b n0] [A00 A01 A02 A10] [A11 A12 A20 A21] [A22 n1 n2 k0] [k1 k2 - -]
-__device__ __forceinline__ void stage_p2g(float4* rec, const P2GRec& r) {
-    rec[0] = make_float4(r.x.x, r.x.y, r.x.z, r.m);
-    rec[1] = make_float4(r.b.x, r.b.y, r.b.z, r.n.x);
-    rec[2] = make_float4(r.A.m[0], r.A.m[1], r.A.m[2], r.A.m[3]);
-    rec[3] = make_float4(r.A.m[4], r.A.m[5], r.A.m[6], r.A.m[7]);
-    rec[4] = make_float4(r.A.m[8], r.n.y, r.n.z, r.k.x);
-    rec[5] = make_float4(r.k.y, r.k.z, 0.f, 0.f);
-}
-__device__ __forceinline__ void unstage_p2g(const float4* rec, bool elem, P2GRec& r) {
-    float4 a = rec[0], b = rec[1], c = rec[2], d = rec[3], e = rec[4], f = rec[5];
-    r.x = {a.x, a.y, a.z}; r.m = a.w;
-    r.b = {b.x, b.y, b.z};
-    r.A.m[0] = c.x; r.A.m[1] = c.y; r.A.m[2] = c.z; r.A.m[3] = c.w;
-    r.A.m[4] = d.x; r.A.m[5] = d.y; r.A.m[6] = d.z; r.A.m[7] = d.w;
-    r.A.m[8] = e.x;
-    r.n = {b.w, e.y, e.z};
-    r.k = {e.w, f.x, f.y};
-    r.elem = elem;
-}
-
-// Stage a patch's P2G records into LDS (all loads of the patch in flight
-// together), reduce its stencil box and sort it by base cell.  Returns false
-// after scattering the patch with global atomics when it does not fit the
-// LDS tile (status bit 4).
-__device__ __forceinline__ bool p2g_stage_sort(const SimDev& sim, const StateView& S, const Patch& P,
-                                               const InstDev& I, int s, int g, SortLDS& L, float4* rec, int* pbase,
-                                               int* tb_touched, PatchBox& B, TileBlocks& TB) {
-    const int tid = threadIdx.x;
-    if (P.count > DCP_STAGE_MAX) {
-        if (tid == 0) atomicOr(sim.status, (unsigned)DCP_STATUS_TILE_SPILL);
-        p2g_spill(sim, S, P, I, s, g);
-        __syncthreads();
-        return false;
-    }
-    if (tid < 3) { L.box[tid] = 1 << 30; L.box[3 + tid] = -(1 << 30); }
-    __syncthreads();
-    int lo0 = 1 << 30, lo1 = 1 << 30, lo2 = 1 << 30, hi0 = -(1 << 30), hi1 = -(1 << 30), hi2 = -(1 << 30);
-    for (int q = tid; q < P.count; q += DCP_WG) {
-        P2GRec r;
-        p2g_record(sim, I, S, P, q, s, g, r);
-        int b0 = stencil_base(r.x.x, I.inv_dx, I.G), b1 = stencil_base(r.x.y, I.inv_dx, I.G),
-            b2 = stencil_base(r.x.z, I.inv_dx, I.G);
-        pbase[q] = pack_base(b0, b1, b2);
-        stage_p2g(rec + 6 * q, r);
-        lo0 = min(lo0, b0); lo1 = min(lo1, b1); lo2 = min(lo2, b2);
-        hi0 = max(hi0, b0); hi1 = max(hi1, b1); hi2 = max(hi2, b2);
-    }
-    bbox_reduce(lo0, lo1, lo2, hi0, hi1, hi2, L, B);
-    if (B.vol > DCP_STILE_CAP) {
-        if (tid == 0) atomicOr(sim.status, (unsigned)DCP_STATUS_TILE_SPILL);
-        p2g_spill(sim, S, P, I, s, g);
-        __syncthreads();
-        return false;
-    }
-    TB = tile_blocks(B.l0, B.l1, B.l2, B.e0, B.e1, B.e2);
-    for (int t = tid; t < TB.n && t < DCP_TB_CAP; t += DCP_WG) tb_touched[t] = 0;
-    for (int q = tid; q < P.count; q += DCP_WG) {
-        int b0, b1, b2;
-        unpack_base(pbase[q], b0, b1, b2);
-        L.key[q] = (unsigned short)tile_cell(B, b0, b1, b2);
-    }
-    __syncthreads();
-    patch_sort(sim, S, P, I, B, L, true);
-    return true;
-}
-
 // Add one tile node's (m, m v) into the instance grid and note its 4^3 block.
 __device__ __forceinline__ void p2g_flush_node(const SimDev& sim, const InstDev& I, const Patch& P, const PatchBox& B,
                                                const TileBlocks& TB, int* tb_touched, int ci, int cj, int ck, float4 a) {
@@ -249,17 +174,19 @@ __global__ __launch_bounds__(DCP_WG, 2) void k_p2g(SimDev sim, StateView S, int 
                 const TileBlocks TB = tile_blocks(B.l0, B.l1, B.l2, B.e0, B.e1, B.e2);
                 for (int t = tid; t < TB.n && t < DCP_TB_CAP; t += DCP_WG) tb_touched[t] = 0;
                 const int pos = patch_bin(act ? tile_cell(B, b0, b1, b2) : -1, B, L);
-                // this lane's tile nodes: tid and tid + DCP_WG (vol <= 2 * DCP_WG)
-                int nc[2][3];
-                bool has[2];
+                // this lane's tile nodes: tid + h * DCP_WG (vol <= DCP_NODES * DCP_WG)
+                int nc[DCP_NODES][3];
+                bool has[DCP_NODES];
 #pragma unroll
-                for (int h = 0; h < 2; ++h) {
+                for (int h = 0; h < DCP_NODES; ++h) {
                     const int c = tid + h * DCP_WG;
                     has[h] = c < B.vol;
                     const int ck = c % B.e2, t = c / B.e2;
                     nc[h][0] = t / B.e1; nc[h][1] = t % B.e1; nc[h][2] = ck;
                 }
-                float4 a[2] = {make_float4(0.f, 0.f, 0.f, 0.f), make_float4(0.f, 0.f, 0.f, 0.f)};
+                float4 a[DCP_NODES];
+#pragma unroll
+                for (int h = 0; h < DCP_NODES; ++h) a[h] = make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
                 for (int i = 0; i < 3; ++i) {
                     if (act) {
@@ -278,14 +205,14 @@ __global__ __launch_bounds__(DCP_WG, 2) void k_p2g(SimDev sim, StateView S, int 
                     run_sums(L, contrib);
                     __syncthreads();
 #pragma unroll
-                    for (int h = 0; h < 2; ++h) {
+                    for (int h = 0; h < DCP_NODES; ++h) {
                         const float4 v = node_sum(L, B, P.count, contrib, nc[h][0], nc[h][1], nc[h][2], i, has[h]);
                         a[h].x += v.x; a[h].y += v.y; a[h].z += v.z; a[h].w += v.w;
                     }
                     __syncthreads();
                 }
 #pragma unroll
-                for (int h = 0; h < 2; ++h)
+                for (int h = 0; h < DCP_NODES; ++h)
                     if (has[h]) p2g_flush_node(sim, I, P, B, TB, tb_touched, nc[h][0], nc[h][1], nc[h][2], a[h]);
                 __syncthreads();
                 p2g_mark_blocks(sim, I, P, TB, tb_touched);
@@ -298,85 +225,6 @@ __global__ __launch_bounds__(DCP_WG, 2) void k_p2g(SimDev sim, StateView S, int 
         // next patch: empty counts and box
         bin_clear(L);
         if (tid < 3) { L.box[tid] = 1 << 30; L.box[3 + tid] = -(1 << 30); }
-        __syncthreads();
-    }
-}
-
-__global__ __launch_bounds__(DCP_WG, 2) void k_p2g_chunked(SimDev sim, StateView S, int s, int g) {
-    __shared__ SortLDS L;
-    __shared__ float4 tiles[DCP_WG / 64][DCP_STILE_CAP];
-    __shared__ float4 rec[DCP_STAGE_MAX * 6];
-    __shared__ int pbase[DCP_STAGE_MAX];
-    __shared__ int tb_touched[DCP_TB_CAP];
-    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    for (int pi = blockIdx.x; pi < sim.n_patches; pi += gridDim.x) {
-        const Patch P = sim.patches[pi];
-        const InstDev I = sim.inst[P.inst];
-        const bool elem = P.cls == CLS_ELEMENT;
-        PatchBox B;
-        TileBlocks TB;
-        if (!p2g_stage_sort(sim, S, P, I, s, g, L, rec, pbase, tb_touched, B, TB)) continue;
-        patch_chunks(P, B, L, sim.chunk);
-        float4* T = tiles[wv];
-        for (int c = lane; c < B.vol; c += 64) T[c] = make_float4(0.f, 0.f, 0.f, 0.f);
-        const int nch = L.n_chunks;
-        for (int ch0 = 0; ch0 < nch; ch0 += DCP_WG) {
-            const int ci = ch0 + tid;
-            const bool act = ci < nch;
-            const int key = act ? (int)L.ch_key[ci] : 0;
-            const SegInfo gi = segment_info_key(key, act);
-            // the lane's chunk: particles of one base cell, accumulated in registers
-            // three passes over the chunk, one per x-node i, keep 9 x 4 sums live
-            const int s0 = act ? (int)L.ch_start[ci] : 0, len = act ? (int)L.ch_len[ci] : 0;
-#pragma unroll
-            for (int i = 0; i < 3; ++i) {
-                float acc[36];
-#pragma unroll
-                for (int o = 0; o < 36; ++o) acc[o] = 0.f;
-                for (int u = 0; u < len; ++u) {
-                    P2GRec r;
-                    unstage_p2g(rec + 6 * (int)L.perm[s0 + u], elem, r);
-                    Stencil st;
-                    make_stencil(r.x, I.inv_dx, I.G, st, nullptr);
-#pragma unroll
-                    for (int j = 0; j < 3; ++j)
-#pragma unroll
-                        for (int k = 0; k < 3; ++k) {
-                            float mm, m0, m1, m2;
-                            p2g_contrib(I, st, r, i, j, k, mm, m0, m1, m2);
-                            const int o = 4 * (j * 3 + k);
-                            acc[o] += mm; acc[o + 1] += m0; acc[o + 2] += m1; acc[o + 3] += m2;
-                        }
-                }
-                seg_sum_n(acc, gi);
-#pragma unroll
-                for (int j = 0; j < 3; ++j)
-#pragma unroll
-                    for (int k = 0; k < 3; ++k) {
-                        const int o = 4 * (j * 3 + k);
-                        if (gi.tail) {
-                            const int c = key + (i * B.e1 + j) * B.e2 + k;
-                            float4 t = T[c];
-                            t.x += acc[o]; t.y += acc[o + 1]; t.z += acc[o + 2]; t.w += acc[o + 3];
-                            T[c] = t;
-                        }
-                        tile_order();
-                    }
-            }
-        }
-        __syncthreads();
-        for (int c = tid; c < B.vol; c += DCP_WG) {
-            float4 a = tiles[0][c];
-#pragma unroll
-            for (int w = 1; w < DCP_WG / 64; ++w) {
-                float4 b = tiles[w][c];
-                a.x += b.x; a.y += b.y; a.z += b.z; a.w += b.w;
-            }
-            int ck = c % B.e2, t = c / B.e2;
-            p2g_flush_node(sim, I, P, B, TB, tb_touched, t / B.e1, t % B.e1, ck, a);
-        }
-        __syncthreads();
-        p2g_mark_blocks(sim, I, P, TB, tb_touched);
         __syncthreads();
     }
 }

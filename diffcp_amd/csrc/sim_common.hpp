@@ -38,7 +38,6 @@ struct Patch {                     // a chunk of one particle class of one insta
 struct SimDev {
     int B, NP, NE, n_patches;
     int precision, svd_grad, per_elem, max_active;
-    int chunk;                     // particles of one base cell accumulated per lane in the scatters
     const InstDev* inst;
     const Patch* patches;          // point patches then element patches
     const int* tri;                // [3][NE] point indices
