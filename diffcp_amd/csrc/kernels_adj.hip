@@ -99,10 +99,12 @@ __device__ __forceinline__ void g2p_adj_spill(const SimDev& sim, const StateView
 // particle's dL/d grid_v_out contributions to the layer's nine nodes into LDS
 // slots, the runs of each occupied base cell are summed, and the lane owning a
 // tile node adds its nine run sums.  dL/dx of each particle (all 27 nodes) is
-// completed in the first pass and stored.
+// completed in the first pass and stored.  grid_v_out of the tile is staged in
+// LDS once per patch, so every node read after that is an LDS read.
 __global__ __launch_bounds__(DCP_WG, 2) void k_g2p_adj(SimDev sim, StateView S, StateView G1, StateView Gout) {
     __shared__ BinLDS L;
     __shared__ float4 contrib[9 * DCP_SORT_MAX];
+    __shared__ float4 vt[DCP_STILE_CAP];        // grid_v_out over the patch tile
     const int tid = threadIdx.x;
     bin_clear(L);
     if (tid < 3) { L.box[tid] = 1 << 30; L.box[3 + tid] = -(1 << 30); }
@@ -139,7 +141,13 @@ __global__ __launch_bounds__(DCP_WG, 2) void k_g2p_adj(SimDev sim, StateView S, 
                 }
                 float4 a[DCP_NODES];
 #pragma unroll
-                for (int h = 0; h < DCP_NODES; ++h) a[h] = make_float4(0.f, 0.f, 0.f, 0.f);
+                for (int h = 0; h < DCP_NODES; ++h) {
+                    a[h] = make_float4(0.f, 0.f, 0.f, 0.f);
+                    if (has[h]) vt[tid + h * DCP_WG] = sim.vout[cell_index(I, nc[h][0] + B.l0, nc[h][1] + B.l1, nc[h][2] + B.l2)];
+                }
+                __syncthreads();
+                // tile index of the particle's base cell; node (i, j, k) is key + (i * e1 + j) * e2 + k
+                const int key = act ? tile_cell(B, b0, b1, b2) : 0;
 #pragma unroll
                 for (int i = 0; i < 3; ++i) {
                     if (act) {
@@ -150,7 +158,7 @@ __global__ __launch_bounds__(DCP_WG, 2) void k_g2p_adj(SimDev sim, StateView S, 
                         for (int j = 0; j < 3; ++j)
 #pragma unroll
                             for (int k = 0; k < 3; ++k) {
-                                float4 gvo = sim.vout[cell_index(I, st.base[0] + i, st.base[1] + j, st.base[2] + k)];
+                                float4 gvo = vt[key + (i * B.e1 + j) * B.e2 + k];
                                 float gg[3];
                                 g2p_adj_node(I, st, i, j, k, gvo, gv, gC, gg, gfx);
                                 contrib[(j * 3 + k) * DCP_SORT_MAX + pos] = make_float4(0.f, gg[0], gg[1], gg[2]);
@@ -163,7 +171,7 @@ __global__ __launch_bounds__(DCP_WG, 2) void k_g2p_adj(SimDev sim, StateView S, 
                                 for (int j = 0; j < 3; ++j)
 #pragma unroll
                                     for (int k = 0; k < 3; ++k) {
-                                        float4 gvo = sim.vout[cell_index(I, st.base[0] + i2, st.base[1] + j, st.base[2] + k)];
+                                        float4 gvo = vt[key + (i2 * B.e1 + j) * B.e2 + k];
                                         float gg[3];
                                         g2p_adj_node(I, st, i2, j, k, gvo, gv, gC, gg, gfx);
                                     }
@@ -234,96 +242,140 @@ __global__ __launch_bounds__(DCP_WG) void k_gridop_adj(SimDev sim) {
 // ---------------------------------------------------------------------------
 // P2G adjoint (gather)
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(DCP_WG) void k_p2g_adj(SimDev sim, StateView S, StateView Gout, int s, int g) {
-    const int tid = threadIdx.x;
+// Reverse of one particle's P2G scatter: dL/d(m, m v) of its 27 stencil nodes
+// (fetched by `node(i, j, k)`) to the gradients of the particle's record.
+template <typename NodeFetch>
+__device__ __forceinline__ void p2g_adj_particle(const SimDev& sim, const InstDev& I, const StateView& Gout,
+                                                 const Patch& P, int q, int s, int g, const P2GRec& r,
+                                                 const Stencil& st, NodeFetch node) {
     const float ddw[3] = {1.0f, -2.0f, 1.0f};
-    for (int pi = blockIdx.x; pi < sim.n_patches; pi += gridDim.x) {
-        const Patch P = sim.patches[pi];
-        const InstDev I = sim.inst[P.inst];
-        for (int q = tid; q < P.count; q += DCP_WG) {
-            P2GRec r;
-            p2g_record(sim, I, S, P, q, s, g, r);
-            Stencil st;
-            make_stencil(r.x, I.inv_dx, I.G, st, nullptr);
-            float gb[3] = {0.f, 0.f, 0.f}, gfx[3] = {0.f, 0.f, 0.f}, gn[3] = {0.f, 0.f, 0.f}, gk[3] = {0.f, 0.f, 0.f};
-            M3<float> gA = mzero<float>();
-            const float dx = I.dx, idx = I.inv_dx;
+        float gb[3] = {0.f, 0.f, 0.f}, gfx[3] = {0.f, 0.f, 0.f}, gn[3] = {0.f, 0.f, 0.f}, gk[3] = {0.f, 0.f, 0.f};
+        M3<float> gA = mzero<float>();
+        const float dx = I.dx, idx = I.inv_dx;
 #pragma unroll
-            for (int i = 0; i < 3; ++i)
+        for (int i = 0; i < 3; ++i)
 #pragma unroll
-                for (int j = 0; j < 3; ++j)
+            for (int j = 0; j < 3; ++j)
 #pragma unroll
-                    for (int k = 0; k < 3; ++k) {
-                        float4 gg = sim.gmv[cell_index(I, st.base[0] + i, st.base[1] + j, st.base[2] + k)];
-                        float W0 = st.w[i][0], W1 = st.w[j][1], W2 = st.w[k][2];
-                        float D0 = st.dw[i][0], D1 = st.dw[j][1], D2 = st.dw[k][2];
-                        float w = W0 * W1 * W2;
-                        float dp[3] = {((float)i - st.fx[0]) * dx, ((float)j - st.fx[1]) * dx, ((float)k - st.fx[2]) * dx};
-                        float gm3[3] = {gg.y, gg.z, gg.w};
-                        float u[3];
+                for (int k = 0; k < 3; ++k) {
+                    float4 gg = node(i, j, k);
+                    float W0 = st.w[i][0], W1 = st.w[j][1], W2 = st.w[k][2];
+                    float D0 = st.dw[i][0], D1 = st.dw[j][1], D2 = st.dw[k][2];
+                    float w = W0 * W1 * W2;
+                    float dp[3] = {((float)i - st.fx[0]) * dx, ((float)j - st.fx[1]) * dx, ((float)k - st.fx[2]) * dx};
+                    float gm3[3] = {gg.y, gg.z, gg.w};
+                    float u[3];
 #pragma unroll
-                        for (int a = 0; a < 3; ++a) u[a] = (&r.b.x)[a] + r.A(a, 0) * dp[0] + r.A(a, 1) * dp[1] + r.A(a, 2) * dp[2];
+                    for (int a = 0; a < 3; ++a) u[a] = (&r.b.x)[a] + r.A(a, 0) * dp[0] + r.A(a, 1) * dp[1] + r.A(a, 2) * dp[2];
 #pragma unroll
-                        for (int a = 0; a < 3; ++a) {
-                            gb[a] += w * gm3[a];
+                    for (int a = 0; a < 3; ++a) {
+                        gb[a] += w * gm3[a];
 #pragma unroll
-                            for (int b = 0; b < 3; ++b) gA(a, b) += w * gm3[a] * dp[b];
-                        }
-                        float scal = gg.x * r.m + gm3[0] * u[0] + gm3[1] * u[1] + gm3[2] * u[2];
-                        float dW[3] = {D0 * W1 * W2, W0 * D1 * W2, W0 * W1 * D2};
-#pragma unroll
-                        for (int d = 0; d < 3; ++d) {
-                            float gAd = gm3[0] * r.A(0, d) + gm3[1] * r.A(1, d) + gm3[2] * r.A(2, d);
-                            gfx[d] += dW[d] * scal - w * dx * gAd;
-                        }
-                        if (r.elem) {
-                            float gw[3] = {idx * dW[0], idx * dW[1], idx * dW[2]};
-                            float sdot = gw[0] * r.n.x + gw[1] * r.n.y + gw[2] * r.n.z;
-                            float qd = gm3[0] * r.k.x + gm3[1] * r.k.y + gm3[2] * r.k.z;
-#pragma unroll
-                            for (int a = 0; a < 3; ++a) { gk[a] += sdot * gm3[a]; gn[a] += qd * gw[a]; }
-                            float H0 = ddw[i], H1 = ddw[j], H2 = ddw[k];
-                            float dg0 = idx * (H0 * W1 * W2 * r.n.x + D0 * D1 * W2 * r.n.y + D0 * W1 * D2 * r.n.z);
-                            float dg1 = idx * (D0 * D1 * W2 * r.n.x + W0 * H1 * W2 * r.n.y + W0 * D1 * D2 * r.n.z);
-                            float dg2 = idx * (D0 * W1 * D2 * r.n.x + W0 * D1 * D2 * r.n.y + W0 * W1 * H2 * r.n.z);
-                            gfx[0] += qd * dg0; gfx[1] += qd * dg1; gfx[2] += qd * dg2;
-                        }
+                        for (int b = 0; b < 3; ++b) gA(a, b) += w * gm3[a] * dp[b];
                     }
-            V3<float> gx = {idx * gfx[0], idx * gfx[1], idx * gfx[2]};
-            if (P.cls == CLS_ELEMENT) {
-                const int e = P.start + q;
-                V3<float> prev = load_v3(sim.g_xe, sim.NE, e);
-                store_v3(sim.g_xe, sim.NE, e, prev + gx);
-                store_v3(sim.g_ve, sim.NE, e, {r.m * gb[0], r.m * gb[1], r.m * gb[2]});
-                store_m9(Gout.eC(0), sim.NE, e, mscale(r.m, gA));
-                store_v3(sim.g_ne, sim.NE, e, {gn[0], gn[1], gn[2]});
-                store_v3(sim.g_ke, sim.NE, e, {gk[0], gk[1], gk[2]});
+                    float scal = gg.x * r.m + gm3[0] * u[0] + gm3[1] * u[1] + gm3[2] * u[2];
+                    float dW[3] = {D0 * W1 * W2, W0 * D1 * W2, W0 * W1 * D2};
+#pragma unroll
+                    for (int d = 0; d < 3; ++d) {
+                        float gAd = gm3[0] * r.A(0, d) + gm3[1] * r.A(1, d) + gm3[2] * r.A(2, d);
+                        gfx[d] += dW[d] * scal - w * dx * gAd;
+                    }
+                    if (r.elem) {
+                        float gw[3] = {idx * dW[0], idx * dW[1], idx * dW[2]};
+                        float sdot = gw[0] * r.n.x + gw[1] * r.n.y + gw[2] * r.n.z;
+                        float qd = gm3[0] * r.k.x + gm3[1] * r.k.y + gm3[2] * r.k.z;
+#pragma unroll
+                        for (int a = 0; a < 3; ++a) { gk[a] += sdot * gm3[a]; gn[a] += qd * gw[a]; }
+                        float H0 = ddw[i], H1 = ddw[j], H2 = ddw[k];
+                        float dg0 = idx * (H0 * W1 * W2 * r.n.x + D0 * D1 * W2 * r.n.y + D0 * W1 * D2 * r.n.z);
+                        float dg1 = idx * (D0 * D1 * W2 * r.n.x + W0 * H1 * W2 * r.n.y + W0 * D1 * D2 * r.n.z);
+                        float dg2 = idx * (D0 * W1 * D2 * r.n.x + W0 * D1 * D2 * r.n.y + W0 * W1 * H2 * r.n.z);
+                        gfx[0] += qd * dg0; gfx[1] += qd * dg1; gfx[2] += qd * dg2;
+                    }
+                }
+        V3<float> gx = {idx * gfx[0], idx * gfx[1], idx * gfx[2]};
+        if (P.cls == CLS_ELEMENT) {
+            const int e = P.start + q;
+            V3<float> prev = load_v3(sim.g_xe, sim.NE, e);
+            store_v3(sim.g_xe, sim.NE, e, prev + gx);
+            store_v3(sim.g_ve, sim.NE, e, {r.m * gb[0], r.m * gb[1], r.m * gb[2]});
+            store_m9(Gout.eC(0), sim.NE, e, mscale(r.m, gA));
+            store_v3(sim.g_ne, sim.NE, e, {gn[0], gn[1], gn[2]});
+            store_v3(sim.g_ke, sim.NE, e, {gk[0], gk[1], gk[2]});
+        } else {
+            const int p = P.start + q;
+            V3<float> prev = Gout.x(p);
+            store_v3(Gout.px(0), sim.NP, p, prev + gx);
+            store_m9(Gout.pC(0), sim.NP, p, mscale(r.m, gA));
+            if (P.cls == CLS_VERTEX) {
+                store_v3(Gout.pv(0), sim.NP, p, {r.m * gb[0], r.m * gb[1], r.m * gb[2]});
+                store_v3(sim.g_fv, sim.NP, p, {gb[0], gb[1], gb[2]});
             } else {
-                const int p = P.start + q;
-                V3<float> prev = Gout.x(p);
-                store_v3(Gout.px(0), sim.NP, p, prev + gx);
-                store_m9(Gout.pC(0), sim.NP, p, mscale(r.m, gA));
-                if (P.cls == CLS_VERTEX) {
-                    store_v3(Gout.pv(0), sim.NP, p, {r.m * gb[0], r.m * gb[1], r.m * gb[2]});
-                    store_v3(sim.g_fv, sim.NP, p, {gb[0], gb[1], gb[2]});
-                } else {
-                    store_v3(Gout.pv(0), sim.NP, p, {0.f, 0.f, 0.f});
-                    if (P.cls == CLS_GRASPER && sim.traj_grad) {
-                        int frame = grasper_frame(I, s, g);
-                        if (frame + 1 < I.n_actions) {
-                            int gid = (p - I.o_start) / I.group_size;
-                            double* t1 = sim.traj_grad + 3 * ((size_t)I.traj_off + (size_t)(frame + 1) * I.n_grasp + gid);
-                            double* t0 = sim.traj_grad + 3 * ((size_t)I.traj_off + (size_t)frame * I.n_grasp + gid);
-                            for (int a = 0; a < 3; ++a) {
-                                double gvp = (double)r.m * (double)gb[a] / (double)I.interval;
-                                atomicAdd(t1 + a, gvp);
-                                atomicAdd(t0 + a, -gvp);
-                            }
+                store_v3(Gout.pv(0), sim.NP, p, {0.f, 0.f, 0.f});
+                if (P.cls == CLS_GRASPER && sim.traj_grad) {
+                    int frame = grasper_frame(I, s, g);
+                    if (frame + 1 < I.n_actions) {
+                        int gid = (p - I.o_start) / I.group_size;
+                        double* t1 = sim.traj_grad + 3 * ((size_t)I.traj_off + (size_t)(frame + 1) * I.n_grasp + gid);
+                        double* t0 = sim.traj_grad + 3 * ((size_t)I.traj_off + (size_t)frame * I.n_grasp + gid);
+                        for (int a = 0; a < 3; ++a) {
+                            double gvp = (double)r.m * (double)gb[a] / (double)I.interval;
+                            atomicAdd(t1 + a, gvp);
+                            atomicAdd(t0 + a, -gvp);
                         }
                     }
                 }
             }
         }
+}
+
+// P2G adjoint, one particle per lane: dL/d(m, m v) of the patch tile is staged
+// in LDS once per patch; a patch whose tile exceeds the cap reads the grid.
+__global__ __launch_bounds__(DCP_WG, 2) void k_p2g_adj(SimDev sim, StateView S, StateView Gout, int s, int g) {
+    __shared__ int box[6];
+    __shared__ float4 gt[DCP_STILE_CAP];
+    const int tid = threadIdx.x;
+    if (tid < 3) { box[tid] = 1 << 30; box[3 + tid] = -(1 << 30); }
+    __syncthreads();
+    for (int pi = blockIdx.x; pi < sim.n_patches; pi += gridDim.x) {
+        const Patch P = sim.patches[pi];
+        const InstDev I = sim.inst[P.inst];
+        const bool act = tid < P.count;   // patches hold at most DCP_WG particles
+        P2GRec r;
+        Stencil st;
+        if (act) {
+            p2g_record(sim, I, S, P, tid, s, g, r);
+            make_stencil(r.x, I.inv_dx, I.G, st, nullptr);
+        } else {
+            st.base[0] = st.base[1] = st.base[2] = 0;
+        }
+        PatchBox B;
+        box_reduce(st.base[0], st.base[1], st.base[2], act, box, B);
+        const bool tiled = B.vol <= DCP_STILE_CAP;
+        if (tiled) {
+#pragma unroll
+            for (int h = 0; h < DCP_NODES; ++h) {
+                const int c = tid + h * DCP_WG;
+                if (c < B.vol) {
+                    const int ck = c % B.e2, t = c / B.e2;
+                    gt[c] = sim.gmv[cell_index(I, t / B.e1 + B.l0, t % B.e1 + B.l1, ck + B.l2)];
+                }
+            }
+        }
+        __syncthreads();
+        if (act) {
+            if (tiled) {
+                const int key = tile_cell(B, st.base[0], st.base[1], st.base[2]);
+                p2g_adj_particle(sim, I, Gout, P, tid, s, g, r, st,
+                                 [&](int i, int j, int k) { return gt[key + (i * B.e1 + j) * B.e2 + k]; });
+            } else {
+                p2g_adj_particle(sim, I, Gout, P, tid, s, g, r, st, [&](int i, int j, int k) {
+                    return sim.gmv[cell_index(I, st.base[0] + i, st.base[1] + j, st.base[2] + k)];
+                });
+            }
+        }
+        if (tid < 3) { box[tid] = 1 << 30; box[3 + tid] = -(1 << 30); }
+        __syncthreads();
     }
 }
 
