@@ -42,6 +42,7 @@ class Options(ctypes.Structure):
         ("max_states", c_int), ("precision", c_int), ("svd_grad", c_int),
         ("per_element_moduli", c_int), ("patch_size", c_int), ("scatter_wgs", c_int),
         ("grid_wgs", c_int), ("bin_cells", c_int),
+        ("grid_snapshots", c_int), ("reserved_opt", c_int),
     ]
 
 

@@ -139,6 +139,24 @@ struct dcp_ctx {
     double* loss_d = nullptr; double* seed_d = nullptr;
     std::vector<double> ones_h;        // host source of the unit loss seeds
     std::vector<float> traj_h;         // staging of dcp_set_trajectory
+    // per-substep grid snapshots of one checkpoint segment (slot = substep - segment base)
+    float4* snap_acc = nullptr;
+    int* snap_blk = nullptr;
+    int* snap_cnt = nullptr;
+    int* snap_ok = nullptr;
+    int snap_cap = 0, snap_slots = 0;
+    long long init_blocks = 0;     // 4^3 blocks the initial particle stencils touch
+    void free_snapshots() {
+        for (void* q : {(void*)snap_acc, (void*)snap_blk, (void*)snap_cnt, (void*)snap_ok})
+            if (q) (void)hipFree(q);
+        snap_acc = nullptr; snap_blk = snap_cnt = snap_ok = nullptr;
+        snap_cap = snap_slots = 0;
+    }
+    GridSnap snap(int slot) const {
+        if (slot < 0 || slot >= snap_slots) return GridSnap{};
+        return GridSnap{snap_acc + (size_t)slot * snap_cap * 64, snap_blk + (size_t)slot * snap_cap,
+                        snap_cnt + (size_t)slot * B, snap_ok + slot, snap_cap};
+    }
     int* keys_d = nullptr;
     int n_targets_total = 0;
     InstDev* inst_d = nullptr;
@@ -213,8 +231,8 @@ static hipError_t time_mark(dcp_ctx* c) {
         if (on_) HIP_OK(time_mark(c));                                                  \
     } while (0)
 
-static int launch_p2g(dcp_ctx* c, StateView S, int s, int g) {
-    TIMED_LAUNCH(DCP_KERNEL_P2G, k_p2g, dim3(c->grid_patches()), dim3(DCP_WG), 0, c->stream, c->sim, S, s, g);
+static int launch_p2g(dcp_ctx* c, StateView S, int s, int g, const int* skip) {
+    TIMED_LAUNCH(DCP_KERNEL_P2G, k_p2g, dim3(c->grid_patches()), dim3(DCP_WG), 0, c->stream, c->sim, S, s, g, skip);
     return 0;
 }
 
@@ -226,19 +244,30 @@ static int launch_prep(dcp_ctx* c, StateView S, StateView Sn, int fresh) {
     return 0;
 }
 
-static int launch_forward(dcp_ctx* c, int s, int g, StateView S, StateView Sn) {
+// snap >= 0: save this substep's grid into snapshot slot `snap`
+static int launch_forward(dcp_ctx* c, int s, int g, StateView S, StateView Sn, int snap = -1) {
     if (launch_prep(c, S, Sn, s == 0)) return -1;
-    if (launch_p2g(c, S, s, g)) return -1;
-    TIMED_LAUNCH(DCP_KERNEL_GRIDOP, k_gridop, dim3(c->grid_gridop()), dim3(DCP_WG), 0, c->stream, c->sim, 0);
+    if (launch_p2g(c, S, s, g, nullptr)) return -1;
+    TIMED_LAUNCH(DCP_KERNEL_GRIDOP, k_gridop, dim3(c->grid_gridop()), dim3(DCP_WG), 0, c->stream, c->sim, 0, c->snap(snap),
+                 nullptr);
     TIMED_LAUNCH(DCP_KERNEL_G2P, k_g2p, dim3(c->grid_patches()), dim3(DCP_WG), 0, c->stream, c->sim, S, Sn);
     return 0;
 }
 
-static int launch_adjoint(dcp_ctx* c, int s, int g, StateView S, StateView G1, StateView Gout) {
+// snap >= 0: the grid of substep s was saved in snapshot slot `snap` by the
+// forward; it is restored instead of recomputed (P2G and the grid update exit
+// at once unless the snapshot overflowed)
+static int launch_adjoint(dcp_ctx* c, int s, int g, StateView S, StateView G1, StateView Gout, int snap = -1) {
     const int fresh = (s == 0);
+    const GridSnap sn = c->snap(snap);
     if (launch_prep(c, S, c->scratch_view(), fresh)) return -1;
-    if (launch_p2g(c, S, s, g)) return -1;
-    TIMED_LAUNCH(DCP_KERNEL_GRIDOP, k_gridop, dim3(c->grid_gridop()), dim3(DCP_WG), 0, c->stream, c->sim, 1);
+    if (launch_p2g(c, S, s, g, sn.ok)) return -1;
+    TIMED_LAUNCH(DCP_KERNEL_GRIDOP, k_gridop, dim3(c->grid_gridop()), dim3(DCP_WG), 0, c->stream, c->sim, 1, GridSnap{},
+                 sn.ok);
+    if (sn.acc) {
+        hipLaunchKernelGGL(k_grid_restore, dim3(c->grid_gridop()), dim3(DCP_WG), 0, c->stream, c->sim, sn);
+        LAUNCH_OK();
+    }
     TIMED_LAUNCH(DCP_KERNEL_G2P_ADJ, k_g2p_adj, dim3(c->grid_patches()), dim3(DCP_WG), 0, c->stream, c->sim, S, G1, Gout);
     TIMED_LAUNCH(DCP_KERNEL_GRIDOP_ADJ, k_gridop_adj, dim3(c->grid_gridop()), dim3(DCP_WG), 0, c->stream, c->sim);
     TIMED_LAUNCH(DCP_KERNEL_P2G_ADJ, k_p2g_adj, dim3(c->grid_patches()), dim3(DCP_WG), 0, c->stream, c->sim, S, Gout, s, g);
@@ -423,6 +452,27 @@ static int build(dcp_ctx* c, const dcp_instance_desc* d, int B) {
         chunk_bins(b, CLS_ELEMENT, I.e_start, c->hi[b].ebin);
     }
     c->n_patches = (int)c->patches_h.size();
+    // 4^3 blocks the initial stencils touch (sizes the grid snapshots)
+    c->init_blocks = 0;
+    for (int b = 0; b < B; ++b) {
+        const HostInst& H = c->hi[b];
+        const int G = H.G, nb = (G + 3) / 4;
+        std::vector<int> blk;
+        auto add = [&](const std::vector<float>& x) {
+            for (size_t i = 0; i + 2 < x.size(); i += 3) {
+                int base[3];
+                for (int d = 0; d < 3; ++d) base[d] = std::min(G - 3, std::max(0, (int)(x[i + d] * G - 0.5f)));
+                for (int a = base[0] >> 2; a <= (base[0] + 2) >> 2; ++a)
+                    for (int bb = base[1] >> 2; bb <= (base[1] + 2) >> 2; ++bb)
+                        for (int cc = base[2] >> 2; cc <= (base[2] + 2) >> 2; ++cc) blk.push_back((a * nb + bb) * nb + cc);
+            }
+        };
+        add(H.x0v);
+        add(H.x0o);
+        add(H.x0c);
+        std::sort(blk.begin(), blk.end());
+        c->init_blocks += (long long)(std::unique(blk.begin(), blk.end()) - blk.begin());
+    }
     // trajectory pool
     std::vector<float> trajh(3 * (size_t)std::max(1, traj), 0.f);
     for (int b = 0; b < B; ++b) {
@@ -504,6 +554,7 @@ extern "C" int dcp_destroy(dcp_ctx* c) {
     if (c->ev0) hipEventDestroy(c->ev0);
     if (c->ev1) hipEventDestroy(c->ev1);
     c->drop_timing_events();
+    c->free_snapshots();
     if (c->own_stream) hipStreamDestroy(c->stream);
     delete c;
     return 0;
@@ -899,8 +950,11 @@ static int seg_slot(const Plan& p, int cseg, int t) {
 
 static int record_forward_segment(dcp_ctx* c, const Plan& p, int cseg, int g) {
     int base = cseg * p.K, end = std::min(base + p.K, p.S);
+    // the last segment's states stay resident for the backward: keep its grids too
+    const bool keep_grids = cseg == p.nseg - 1 && c->snap_slots >= p.K;
     for (int f = base; f < end; ++f) {
-        if (launch_forward(c, f, g, c->state(seg_slot(p, cseg, f)), c->state(seg_slot(p, cseg, f + 1)))) return -1;
+        const int snap = keep_grids && f < p.L ? f - base : -1;
+        if (launch_forward(c, f, g, c->state(seg_slot(p, cseg, f)), c->state(seg_slot(p, cseg, f + 1)), snap)) return -1;
         if (f + 1 == p.L) {
             if (launch_loss(c, c->state(seg_slot(p, cseg, f + 1)))) return -1;
             if (launch_fill(c, c->adj, c->slot_floats)) return -1;
@@ -918,11 +972,16 @@ static int record_backward_segment(dcp_ctx* c, const Plan& p, int cseg, int g) {
     // recompute states base+1 .. end-1, except in the last forward segment: no
     // later segment reused its slots, so they still hold those states
     const bool resident = cseg == p.nseg - 1;
+    const bool grids = c->snap_slots >= p.K;
     for (int f = base; !resident && f + 1 < end; ++f)
-        if (launch_forward(c, f, g, c->state(seg_slot(p, cseg, f)), c->state(seg_slot(p, cseg, f + 1)))) return -1;
+        if (launch_forward(c, f, g, c->state(seg_slot(p, cseg, f)), c->state(seg_slot(p, cseg, f + 1)),
+                           grids ? f - base : -1))
+            return -1;
     for (int f = end - 1; f >= base; --f) {
         int gin = (p.L - (f + 1)) & 1, gout = (p.L - f) & 1;
-        if (launch_adjoint(c, f, g, c->state(seg_slot(p, cseg, f)), c->adjv(gin), c->adjv(gout))) return -1;
+        // grids saved: every substep of a resident segment, all but the last one otherwise
+        const int snap = grids && (resident || f + 1 < end) ? f - base : -1;
+        if (launch_adjoint(c, f, g, c->state(seg_slot(p, cseg, f)), c->adjv(gin), c->adjv(gout), snap)) return -1;
     }
     return 0;
 }
@@ -950,11 +1009,44 @@ static int run_recorded(dcp_ctx* c, int kind, int cseg, int g, const Plan& p, in
     return 0;
 }
 
+// Grid snapshot slots for a plan with checkpoint interval K (one per substep of
+// a segment).  Capacity per slot: 4x the blocks the initial stencils touch
+// (deformation headroom), within the instances' blocks, and all slots within a
+// quarter of the free device memory; below 256 blocks per slot, none.
+static int ensure_snapshots(dcp_ctx* c, int K) {
+    if (c->opt.grid_snapshots < 0 || c->snap_slots >= K) return 0;
+    long long all = 0;
+    for (auto& I : c->inst_h) all += (long long)I.nb * I.nb * I.nb;
+    size_t free_b = 0, total_b = 0;
+    HIP_OK(hipMemGetInfo(&free_b, &total_b));
+    // cached graphs hold the old buffers
+    HIP_OK(hipStreamSynchronize(c->stream));
+    for (auto& kv : c->graphs) (void)hipGraphExecDestroy(kv.second);
+    c->graphs.clear();
+    c->free_snapshots();
+    const size_t per_block = 64 * sizeof(float4) + sizeof(int);
+    long long cap = std::min(all, std::max<long long>(1024, 4 * c->init_blocks));
+    cap = std::min<long long>(cap, (long long)(free_b / 4 / per_block / (size_t)K));
+    if (cap < 256) return 0;
+    if (hipMalloc(&c->snap_acc, (size_t)K * cap * 64 * sizeof(float4)) != hipSuccess ||
+        hipMalloc(&c->snap_blk, (size_t)K * cap * sizeof(int)) != hipSuccess ||
+        hipMalloc(&c->snap_cnt, (size_t)K * c->B * sizeof(int)) != hipSuccess ||
+        hipMalloc(&c->snap_ok, (size_t)K * sizeof(int)) != hipSuccess) {
+        (void)hipGetLastError();
+        c->free_snapshots();
+        return 0;   // no snapshots: the adjoint recomputes its grids
+    }
+    c->snap_cap = (int)cap;
+    c->snap_slots = K;
+    return 0;
+}
+
 extern "C" int dcp_rollout_grad(dcp_ctx* c, int32_t S, int32_t g, int32_t K, int32_t use_graph, double* loss_out,
                                 double* grads_out) {
     if (!c) return set_err("bad args");
     Plan p;
     if (plan_rollout(c, S, K, p)) return -1;
+    if (ensure_snapshots(c, p.K)) return -1;
     if (c->n_targets_total == 0) return set_err("no targets set (dcp_set_targets)");
     // the reset state must already be in slot 0 (dcp_reset)
     // timing events restart with an eager rollout or with a plan's first capture

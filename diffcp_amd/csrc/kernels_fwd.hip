@@ -145,7 +145,8 @@ __device__ __forceinline__ void p2g_mark_blocks(const SimDev& sim, const InstDev
 // owning tile node n adds the nine run sums of the base cells n - (i, j, k).
 // Node sums stay in registers across the passes and are flushed once into the
 // instance grid with float atomics (tiles of neighbouring patches overlap).
-__global__ __launch_bounds__(DCP_WG, 2) void k_p2g(SimDev sim, StateView S, int s, int g) {
+__global__ __launch_bounds__(DCP_WG, 2) void k_p2g(SimDev sim, StateView S, int s, int g, const int* skip) {
+    if (skip && *skip) return;   // adjoint substep whose grid is restored from a snapshot
     __shared__ BinLDS L;
     __shared__ float4 contrib[9 * DCP_SORT_MAX];
     __shared__ int tb_touched[DCP_TB_CAP];
@@ -232,17 +233,44 @@ __global__ __launch_bounds__(DCP_WG, 2) void k_p2g(SimDev sim, StateView S, int 
 // ---------------------------------------------------------------------------
 // grid update over the active blocks
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(DCP_WG) void k_gridop(SimDev sim, int keep) {
+// Offset of instance b's blocks in a snapshot and the total, from per-instance
+// block counts (B is at most a few hundred: a loop per wave is cheap).
+__device__ __forceinline__ void snap_offsets(const SimDev& sim, const int* counts, int b, int& base, int& total) {
+    base = 0; total = 0;
+    for (int q = 0; q < sim.B; ++q) {
+        const int nb = sim.inst[q].nb;
+        const int n = min(counts[q], nb * nb * nb);
+        if (q < b) base += n;
+        total += n;
+    }
+}
+
+// Grid update; with `save`, the (m, m v) of the active blocks are copied into
+// the snapshot before being cleared.  `skip` != 0: the adjoint restores this
+// substep's grid from its snapshot instead.
+__global__ __launch_bounds__(DCP_WG) void k_gridop(SimDev sim, int keep, GridSnap save, const int* skip) {
+    if (skip && *skip) return;
     const int lane = threadIdx.x & 63;
     ActiveWalk w;
     if (!active_walk(sim, sim.inst, w)) return;
     const InstDev I = sim.inst[w.b];
+    int base = 0, total = 0;
+    if (save.acc) {
+        snap_offsets(sim, sim.n_active, w.b, base, total);
+        if (w.first == 0 && lane == 0) save.cnt[w.b] = w.n;
+        if (blockIdx.x == 0 && threadIdx.x == 0) *save.ok = total <= save.cap;
+    }
+    const bool saving = save.acc && total <= save.cap;
     for (int it = w.first; it < w.n; it += w.stride) {
         const int blk = sim.active[I.flag_base + it];
         const long long cell = I.grid_base + (long long)blk * 64 + lane;
         int ci, cj, ck;
         block_cell_coords(I, blk, lane, ci, cj, ck);
         float4 c = sim.acc[cell];
+        if (saving) {
+            save.acc[(size_t)(base + it) * 64 + lane] = c;
+            if (lane == 0) save.blk[base + it] = blk;
+        }
         float v[3];
         grid_velocity(I, c, ci, cj, ck, v, nullptr);
         sim.vout[cell] = make_float4(v[0], v[1], v[2], 0.f);
@@ -250,6 +278,39 @@ __global__ __launch_bounds__(DCP_WG) void k_gridop(SimDev sim, int keep) {
         if (!keep) {
             sim.acc[cell] = make_float4(0.f, 0.f, 0.f, 0.f);
             if (lane == 0) sim.flags[I.flag_base + blk] = 0;
+        }
+    }
+}
+
+// Rebuild a substep's grid from its snapshot: (m, m v), v_out, block flags and
+// the per-instance active lists (what P2G + the grid update with keep = 1
+// leave behind).  Does nothing when the snapshot overflowed.
+__global__ __launch_bounds__(DCP_WG) void k_grid_restore(SimDev sim, GridSnap snap) {
+    if (!*snap.ok) return;
+    const int lane = threadIdx.x & 63;
+    const int per = max(1, (int)gridDim.x / sim.B);
+    const int b = blockIdx.x / per;
+    if (b >= sim.B) return;
+    const int nw = DCP_WG / 64;
+    const int first = (blockIdx.x % per) * nw + (threadIdx.x >> 6), stride = per * nw;
+    const InstDev I = sim.inst[b];
+    int base, total;
+    snap_offsets(sim, snap.cnt, b, base, total);
+    const int n = min(snap.cnt[b], I.nb * I.nb * I.nb);
+    if (first == 0 && lane == 0) sim.n_active[b] = n;
+    for (int it = first; it < n; it += stride) {
+        const int blk = snap.blk[base + it];
+        const long long cell = I.grid_base + (long long)blk * 64 + lane;
+        int ci, cj, ck;
+        block_cell_coords(I, blk, lane, ci, cj, ck);
+        const float4 c = snap.acc[(size_t)(base + it) * 64 + lane];
+        sim.acc[cell] = c;
+        float v[3];
+        grid_velocity(I, c, ci, cj, ck, v, nullptr);
+        sim.vout[cell] = make_float4(v[0], v[1], v[2], 0.f);
+        if (lane == 0) {
+            sim.flags[I.flag_base + blk] = 1;
+            sim.active[I.flag_base + it] = blk;
         }
     }
 }

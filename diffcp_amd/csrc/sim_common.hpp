@@ -35,6 +35,17 @@ struct Patch {                     // a chunk of one particle class of one insta
     int inst, cls, start, count;
 };
 
+// Compact copy of one substep's grid for the adjoint: the (m, m v) of every
+// active 4^3 block, instances in order, block ids, blocks per instance, and
+// ok = 1 when every instance fitted in `cap` blocks.  acc == nullptr: none.
+struct GridSnap {
+    float4* acc;
+    int* blk;
+    int* cnt;
+    int* ok;
+    int cap;
+};
+
 struct SimDev {
     int B, NP, NE, n_patches;
     int precision, svd_grad, per_elem, max_active;

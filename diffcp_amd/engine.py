@@ -35,7 +35,8 @@ class Engine:
     def __init__(self, instances: Sequence[Instance], *, max_states: int = 2,
                  precision: str = "mixed", svd_grad: str = "reference",
                  per_element_moduli: bool = False, patch_size: int = 0, scatter_wgs: int = 0,
-                 grid_wgs: int = 0, bin_cells: int = 0, stream: Optional[int] = None):
+                 grid_wgs: int = 0, bin_cells: int = 0, grid_snapshots: bool = True,
+                 stream: Optional[int] = None):
         self.lib = _abi.load()
         self.instances = list(instances)
         self.B = len(self.instances)
@@ -73,7 +74,7 @@ class Engine:
         opt = _abi.Options(max_states=max_states, precision=PRECISION[precision],
                            svd_grad=SVD_GRAD[svd_grad], per_element_moduli=int(per_element_moduli),
                            patch_size=patch_size, scatter_wgs=scatter_wgs, grid_wgs=grid_wgs,
-                           bin_cells=bin_cells)
+                           bin_cells=bin_cells, grid_snapshots=0 if grid_snapshots else -1)
         ctx = ctypes.c_void_p()
         _abi.check(self.lib.dcp_create(descs, self.B, ctypes.byref(opt), stream, ctypes.byref(ctx)))
         self.ctx = ctx
