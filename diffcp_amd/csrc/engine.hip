@@ -164,6 +164,7 @@ struct dcp_ctx {
     std::map<std::tuple<int, int, int, int, int>, hipGraphExec_t> graphs;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_ms = 0.0;
+    int last_final_slot = -1;
     // per-launch kernel timing
     int time_kid = 0;
     std::vector<hipEvent_t> tev;
@@ -637,26 +638,28 @@ extern "C" int dcp_reset(dcp_ctx* c, int32_t slot, const double* vx, const doubl
     std::vector<float> st(c->slot_floats, 0.f);
     std::vector<float> Dinv(9 * (size_t)NE), Dii(9 * (size_t)NE);
     size_t nv = 0, no = 0, nc = 0;
+    std::vector<float> xv, xo, xc;   // this reset's positions (the creation ones unless given)
     for (int b = 0; b < c->B; ++b) {
-        HostInst& H = c->hi[b];
+        const HostInst& H = c->hi[b];
         const InstDev& I = c->inst_h[b];
-        if (vx) for (int i = 0; i < 3 * H.nv; ++i) H.x0v[i] = (float)vx[3 * nv + i];
-        if (ox) for (int i = 0; i < 3 * H.no; ++i) H.x0o[i] = (float)ox[3 * no + i];
-        if (cx) for (int i = 0; i < 3 * H.nc; ++i) H.x0c[i] = (float)cx[3 * nc + i];
+        xv = H.x0v; xo = H.x0o; xc = H.x0c;
+        if (vx) for (int i = 0; i < 3 * H.nv; ++i) xv[i] = (float)vx[3 * nv + i];
+        if (ox) for (int i = 0; i < 3 * H.no; ++i) xo[i] = (float)ox[3 * no + i];
+        if (cx) for (int i = 0; i < 3 * H.nc; ++i) xc[i] = (float)cx[3 * nc + i];
         nv += H.nv; no += H.no; nc += H.nc;
         for (int i = 0; i < H.nv; ++i) {
             int p = I.v_start + H.vinv[i];
-            for (int k = 0; k < 3; ++k) st[(size_t)k * NP + p] = H.x0v[3 * i + k];
+            for (int k = 0; k < 3; ++k) st[(size_t)k * NP + p] = xv[3 * i + k];
         }
         for (int i = 0; i < H.no; ++i)
-            for (int k = 0; k < 3; ++k) st[(size_t)k * NP + I.o_start + i] = H.x0o[3 * i + k];
+            for (int k = 0; k < 3; ++k) st[(size_t)k * NP + I.o_start + i] = xo[3 * i + k];
         for (int i = 0; i < H.nc; ++i)
-            for (int k = 0; k < 3; ++k) st[(size_t)k * NP + I.c_start + i] = H.x0c[3 * i + k];
+            for (int k = 0; k < 3; ++k) st[(size_t)k * NP + I.c_start + i] = xc[3 * i + k];
         for (int ei = 0; ei < H.ne; ++ei) {
             int en = H.eperm[ei], e = I.e_start + ei;
-            const float* a = &H.x0v[3 * H.tri[3 * en]];
-            const float* bb = &H.x0v[3 * H.tri[3 * en + 1]];
-            const float* cc = &H.x0v[3 * H.tri[3 * en + 2]];
+            const float* a = &xv[3 * H.tri[3 * en]];
+            const float* bb = &xv[3 * H.tri[3 * en + 1]];
+            const float* cc = &xv[3 * H.tri[3 * en + 2]];
             float d1[3], d2[3], n1[3], n2[3], c3[3], n3[3];
             for (int k = 0; k < 3; ++k) { d1[k] = bb[k] - a[k]; d2[k] = cc[k] - a[k]; }
             float l1 = std::sqrt(d1[0] * d1[0] + d1[1] * d1[1] + d1[2] * d1[2] + 1e-6f);
@@ -1067,6 +1070,7 @@ extern "C" int dcp_rollout_grad(dcp_ctx* c, int32_t S, int32_t g, int32_t K, int
     float ms = 0.f;
     HIP_OK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
     c->last_ms = ms;
+    c->last_final_slot = seg_slot(p, p.nseg - 1, p.S);
     if (grads_out && reduce_param_grads(c, grads_out)) return -1;
     return 0;
 }
@@ -1093,6 +1097,7 @@ extern "C" int dcp_rollout(dcp_ctx* c, int32_t S, int32_t g, int32_t use_graph, 
     float ms = 0.f;
     HIP_OK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
     c->last_ms = ms;
+    c->last_final_slot = cur;
     if (final_slot) *final_slot = cur;
     return 0;
 }
@@ -1132,6 +1137,13 @@ extern "C" int dcp_kernel_timing(dcp_ctx* c, double* total_ms, int32_t* launches
     }
     *total_ms = tot;
     *launches = n;
+    return 0;
+}
+
+extern "C" int dcp_last_rollout_final_slot(dcp_ctx* c, int32_t* slot) {
+    if (!c || !slot) return set_err("bad args");
+    if (c->last_final_slot < 0) return set_err("no rollout has run");
+    *slot = c->last_final_slot;
     return 0;
 }
 

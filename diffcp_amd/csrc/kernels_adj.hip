@@ -98,8 +98,8 @@ __device__ __forceinline__ void g2p_adj_spill(const SimDev& sim, const StateView
 // its G2P-reverse inputs in registers; per x-layer i each lane computes its
 // particle's dL/d grid_v_out contributions to the layer's nine nodes into LDS
 // slots, the runs of each occupied base cell are summed, and the lane owning a
-// tile node adds its nine run sums.  dL/dx of each particle (all 27 nodes) is
-// completed in the first pass and stored.  grid_v_out of the tile is staged in
+// tile node adds its nine run sums.  dL/dx of each particle accumulates over
+// the three layers and is stored after the last.  grid_v_out of the tile is staged in
 // LDS once per patch, so every node read after that is an LDS read.
 __global__ __launch_bounds__(DCP_WG, 2) void k_g2p_adj(SimDev sim, StateView S, StateView G1, StateView Gout) {
     __shared__ BinLDS L;
@@ -148,12 +148,12 @@ __global__ __launch_bounds__(DCP_WG, 2) void k_g2p_adj(SimDev sim, StateView S, 
                 __syncthreads();
                 // tile index of the particle's base cell; node (i, j, k) is key + (i * e1 + j) * e2 + k
                 const int key = act ? tile_cell(B, b0, b1, b2) : 0;
+                float gfx[3] = {0.f, 0.f, 0.f};   // dL/dfx, summed over the three layers
 #pragma unroll
                 for (int i = 0; i < 3; ++i) {
                     if (act) {
                         Stencil st;
                         make_stencil(x, I.inv_dx, I.G, st, nullptr);
-                        float gfx[3] = {0.f, 0.f, 0.f};
 #pragma unroll
                         for (int j = 0; j < 3; ++j)
 #pragma unroll
@@ -163,20 +163,7 @@ __global__ __launch_bounds__(DCP_WG, 2) void k_g2p_adj(SimDev sim, StateView S, 
                                 g2p_adj_node(I, st, i, j, k, gvo, gv, gC, gg, gfx);
                                 contrib[(j * 3 + k) * DCP_SORT_MAX + pos] = make_float4(0.f, gg[0], gg[1], gg[2]);
                             }
-                        if (i == 0) {
-                            // the remaining x-layers' share of dL/dfx, then the store
-#pragma unroll
-                            for (int i2 = 1; i2 < 3; ++i2)
-#pragma unroll
-                                for (int j = 0; j < 3; ++j)
-#pragma unroll
-                                    for (int k = 0; k < 3; ++k) {
-                                        float4 gvo = vt[key + (i2 * B.e1 + j) * B.e2 + k];
-                                        float gg[3];
-                                        g2p_adj_node(I, st, i2, j, k, gvo, gv, gC, gg, gfx);
-                                    }
-                            g2p_adj_store(sim, I, Gout, P, tid, gx1, gfx);
-                        }
+                        if (i == 2) g2p_adj_store(sim, I, Gout, P, tid, gx1, gfx);
                     }
                     __syncthreads();
                     run_sums(L, contrib);

@@ -248,6 +248,12 @@ class Engine:
         _abi.check(self.lib.dcp_last_rollout_ms(self.ctx, ctypes.byref(ms)))
         return loss, grads, ms.value
 
+    def last_final_slot(self) -> int:
+        """State slot of the final state of the last rollout."""
+        s = ctypes.c_int32()
+        _abi.check(self.lib.dcp_last_rollout_final_slot(self.ctx, ctypes.byref(s)))
+        return s.value
+
     def synchronize(self) -> None:
         """Wait for all work queued on the engine's stream."""
         _abi.check(self.lib.dcp_synchronize(self.ctx))

@@ -207,6 +207,11 @@ int dcp_rollout(dcp_ctx* ctx, int32_t n_substeps, int32_t g, int32_t use_graph,
 /* Device time of the last dcp_rollout_grad / dcp_rollout call in ms. */
 int dcp_last_rollout_ms(dcp_ctx* ctx, double* ms);
 
+/* State slot holding the final state (n_substeps) of the last
+ * dcp_rollout_grad / dcp_rollout call: what vertex_x.to_numpy()[max_steps-1]
+ * reads for the window carry-over (parameter_fitting_functions.py:40-49). */
+int dcp_last_rollout_final_slot(dcp_ctx* ctx, int32_t* slot);
+
 /* Block until all work queued on the context's stream has finished. */
 int dcp_synchronize(dcp_ctx* ctx);
 

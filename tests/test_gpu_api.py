@@ -63,6 +63,12 @@ def test_tape_window_matches_fused_rollout():
     st0 = s.engine.download_state(0)
     assert np.abs(st0["vertex_x"] - final).max() < 1e-6
     assert np.abs(st0["vertex_v"]).max() == 0.0
+    # a reset without positions goes back to the creation state, not the carried-over one
+    s.engine.reset(0)
+    st0 = s.engine.download_state(0)
+    assert np.abs(st0["vertex_x"] - inst.vertex_x).max() < 1e-6
+    slot, _ = s.engine.rollout(S - 2)
+    assert _rel(s.engine.loss(slot)[0], loss[0]) < 1e-5
 
 
 def test_autograd_function_scales_unit_seed_gradients():
