@@ -103,11 +103,12 @@ __device__ __forceinline__ void g2p_adj_spill(const SimDev& sim, const StateView
 // LDS once per patch, so every node read after that is an LDS read.
 __global__ __launch_bounds__(DCP_WG, 2) void k_g2p_adj(SimDev sim, StateView S, StateView G1, StateView Gout) {
     __shared__ BinLDS L;
-    __shared__ float4 contrib[9 * DCP_SORT_MAX];
+    __shared__ float4 contrib[9 * DCP_SORT_MAX + 1];
     __shared__ float4 vt[DCP_STILE_CAP];        // grid_v_out over the patch tile
     const int tid = threadIdx.x;
     bin_clear(L);
     if (tid < 3) { L.box[tid] = 1 << 30; L.box[3 + tid] = -(1 << 30); }
+    if (tid == 0) contrib[DCP_CONTRIB_ZERO] = make_float4(0.f, 0.f, 0.f, 0.f);   // never overwritten
     __syncthreads();
     for (int pi = blockIdx.x; pi < sim.n_patches; pi += gridDim.x) {
         const Patch P = sim.patches[pi];

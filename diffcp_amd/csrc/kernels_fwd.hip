@@ -148,11 +148,12 @@ __device__ __forceinline__ void p2g_mark_blocks(const SimDev& sim, const InstDev
 __global__ __launch_bounds__(DCP_WG, 2) void k_p2g(SimDev sim, StateView S, int s, int g, const int* skip) {
     if (skip && *skip) return;   // adjoint substep whose grid is restored from a snapshot
     __shared__ BinLDS L;
-    __shared__ float4 contrib[9 * DCP_SORT_MAX];
+    __shared__ float4 contrib[9 * DCP_SORT_MAX + 1];
     __shared__ int tb_touched[DCP_TB_CAP];
     const int tid = threadIdx.x;
     bin_clear(L);
     if (tid < 3) { L.box[tid] = 1 << 30; L.box[3 + tid] = -(1 << 30); }
+    if (tid == 0) contrib[DCP_CONTRIB_ZERO] = make_float4(0.f, 0.f, 0.f, 0.f);   // never overwritten
     __syncthreads();
     for (int pi = blockIdx.x; pi < sim.n_patches; pi += gridDim.x) {
         const Patch P = sim.patches[pi];

@@ -103,7 +103,9 @@ __device__ __forceinline__ int patch_bin(int key, const PatchBox& B, BinLDS& L) 
 // position], slot = (j, k) of the node relative to the particle's base cell.
 //
 // Stage 1 (run_sums): one task per (run, slot) sums the run's entries of that
-// slot and leaves the sum in the run's first entry.  Loads go four at a time.
+// slot and leaves the sum in the run's first entry.  Loads go eight at a time
+// (the dense cells of a fine cloth hold tens of particles).
+__device__ __forceinline__ void add4(float4& a, const float4& b) { a.x += b.x; a.y += b.y; a.z += b.z; a.w += b.w; }
 __device__ __forceinline__ void run_sums(const BinLDS& L, float4* contrib) {
     const int ntask = 9 * L.n_runs;
     for (int t = threadIdx.x; t < ntask; t += DCP_WG) {
@@ -112,6 +114,14 @@ __device__ __forceinline__ void run_sums(const BinLDS& L, float4* contrib) {
         const int len = L.run_len[r];
         float4 a = c[0];
         int u = 1;
+        for (; u + 7 < len; u += 8) {
+            float4 b[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) b[q] = c[u + q];
+            add4(b[0], b[1]); add4(b[2], b[3]); add4(b[4], b[5]); add4(b[6], b[7]);
+            add4(b[0], b[2]); add4(b[4], b[6]); add4(b[0], b[4]);
+            add4(a, b[0]);
+        }
         for (; u + 3 < len; u += 4) {
             const float4 b0 = c[u], b1 = c[u + 1], b2 = c[u + 2], b3 = c[u + 3];
             a.x += (b0.x + b1.x) + (b2.x + b3.x); a.y += (b0.y + b1.y) + (b2.y + b3.y);
@@ -126,27 +136,28 @@ __device__ __forceinline__ void run_sums(const BinLDS& L, float4* contrib) {
 }
 
 // Stage 2 (node_sum): the tile node (ci, cj, ck) adds, for its nine base cells
-// node - (i, dj, dk), the run sum of slot (dj, dk) — nine independent loads,
-// the same count for every node.  `has` = false contributes nothing.
+// node - (i, dj, dk), the run sum of slot (dj, dk): the 18 run bounds are
+// loaded together, then nine unconditional loads (an empty or out-of-box cell
+// reads the zero entry at DCP_CONTRIB_ZERO).  `has` = false contributes nothing.
+#define DCP_CONTRIB_ZERO (9 * DCP_SORT_MAX)    // contrib arrays hold 9 * DCP_SORT_MAX + 1 entries
 __device__ __forceinline__ float4 node_sum(const BinLDS& L, const PatchBox& B, int count, const float4* contrib,
                                            int ci, int cj, int ck, int i, bool has) {
     float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
     const int bi = ci - i;
     if (!has || bi < 0 || bi > B.e0 - 3) return acc;
+    int idx[9];
 #pragma unroll
     for (int o = 0; o < 9; ++o) {
         const int dj = o / 3, dk = o % 3;
         const int bj = cj - dj, bk = ck - dk;
-        if (bj >= 0 && bj <= B.e1 - 3 && bk >= 0 && bk <= B.e2 - 3) {
-            const int kb = (bi * B.e1 + bj) * B.e2 + bk;
-            const int a = L.off[kb];
-            const int e = kb + 1 < B.vol ? L.off[kb + 1] : count;
-            if (e > a) {
-                const float4 v = contrib[o * DCP_SORT_MAX + a];
-                acc.x += v.x; acc.y += v.y; acc.z += v.z; acc.w += v.w;
-            }
-        }
+        const bool in = bj >= 0 && bj <= B.e1 - 3 && bk >= 0 && bk <= B.e2 - 3;
+        const int kb = in ? (bi * B.e1 + bj) * B.e2 + bk : 0;
+        const int a = L.off[kb];
+        const int e = kb + 1 < B.vol ? L.off[kb + 1] : count;
+        idx[o] = in && e > a ? o * DCP_SORT_MAX + a : DCP_CONTRIB_ZERO;
     }
+#pragma unroll
+    for (int o = 0; o < 9; ++o) add4(acc, contrib[idx[o]]);
     return acc;
 }
 
