@@ -232,8 +232,15 @@ static hipError_t time_mark(dcp_ctx* c) {
         if (on_) HIP_OK(time_mark(c));                                                  \
     } while (0)
 
+// A launch with a skip flag is left out of the kernel timing: it exits at once
+// whenever the grid snapshot holds the substep.
 static int launch_p2g(dcp_ctx* c, StateView S, int s, int g, const int* skip) {
-    TIMED_LAUNCH(DCP_KERNEL_P2G, k_p2g, dim3(c->grid_patches()), dim3(DCP_WG), 0, c->stream, c->sim, S, s, g, skip);
+    if (skip) {
+        hipLaunchKernelGGL(k_p2g, dim3(c->grid_patches()), dim3(DCP_WG), 0, c->stream, c->sim, S, s, g, skip);
+        LAUNCH_OK();
+    } else {
+        TIMED_LAUNCH(DCP_KERNEL_P2G, k_p2g, dim3(c->grid_patches()), dim3(DCP_WG), 0, c->stream, c->sim, S, s, g, skip);
+    }
     return 0;
 }
 
@@ -263,8 +270,13 @@ static int launch_adjoint(dcp_ctx* c, int s, int g, StateView S, StateView G1, S
     const GridSnap sn = c->snap(snap);
     if (launch_prep(c, S, c->scratch_view(), fresh)) return -1;
     if (launch_p2g(c, S, s, g, sn.ok)) return -1;
-    TIMED_LAUNCH(DCP_KERNEL_GRIDOP, k_gridop, dim3(c->grid_gridop()), dim3(DCP_WG), 0, c->stream, c->sim, 1, GridSnap{},
-                 sn.ok);
+    if (sn.ok) {
+        hipLaunchKernelGGL(k_gridop, dim3(c->grid_gridop()), dim3(DCP_WG), 0, c->stream, c->sim, 1, GridSnap{}, sn.ok);
+        LAUNCH_OK();
+    } else {
+        TIMED_LAUNCH(DCP_KERNEL_GRIDOP, k_gridop, dim3(c->grid_gridop()), dim3(DCP_WG), 0, c->stream, c->sim, 1, GridSnap{},
+                     nullptr);
+    }
     if (sn.acc) {
         hipLaunchKernelGGL(k_grid_restore, dim3(c->grid_gridop()), dim3(DCP_WG), 0, c->stream, c->sim, sn);
         LAUNCH_OK();
@@ -508,7 +520,7 @@ static int build(dcp_ctx* c, const dcp_instance_desc* d, int B) {
         c->alloc(&S.flags, (size_t)flags) || c->alloc(&S.active, (size_t)flags) || c->alloc(&S.n_active, (size_t)B) ||
         c->alloc(&S.status, 1) || c->alloc(&S.g_xe, 3 * (size_t)ne) || c->alloc(&S.g_ve, 3 * (size_t)ne) ||
         c->alloc(&S.g_ne, 3 * (size_t)ne) || c->alloc(&S.g_ke, 3 * (size_t)ne) || c->alloc(&S.g_fv, 3 * (size_t)np) ||
-        c->alloc(&S.ec_gx, 9 * (size_t)ne) || c->alloc(&S.ec_gv, 9 * (size_t)ne) ||
+        c->alloc(&S.ec_gx, 9 * (size_t)ne) || c->alloc(&S.ec_gv, 3 * (size_t)ne) ||
         c->alloc(&S.pgrad_patch, 4 * (size_t)c->n_patches) || c->alloc(&S.pgrad_inst, (size_t)DCP_N_PARAMS * B) ||
         c->alloc(&S.emod_grad, 3 * (size_t)ne) || c->alloc(&S.traj_grad, 3 * (size_t)std::max(1, traj)))
         return -1;

@@ -104,6 +104,38 @@ struct P2GRec {
     bool elem;
 };
 
+// Sums, in CSR order, of the per-corner vectors a[corner][.][element] (and of
+// the per-element vectors b[.][element], when TWO) over the (element, corner)
+// entries incident to vertex p.  Entries go eight at a time with their list and
+// vector loads issued together: one entry after the other would chain two load
+// latencies per entry.
+template <bool TWO>
+__device__ __forceinline__ void incident_sum(const SimDev& sim, int p, const float* a, const float* b,
+                                             V3<float>& sa, V3<float>& sb) {
+    const int t0 = sim.inc_off[p], t1 = sim.inc_off[p + 1];
+    for (int t = t0; t < t1; t += 8) {
+        int ec[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) ec[u] = t + u < t1 ? sim.inc_list[t + u] : -1;
+        V3<float> va[8], vb[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            if (ec[u] >= 0) {
+                const int e = ec[u] / 3, c = ec[u] - 3 * e;
+                va[u] = load_v3(a + (size_t)3 * c * sim.NE, sim.NE, e);
+                if (TWO) vb[u] = load_v3(b, sim.NE, e);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            if (ec[u] >= 0) {
+                sa = sa + va[u];
+                if (TWO) sb = sb + vb[u];
+            }
+        }
+    }
+}
+
 // everything a particle scatters: per cell w*(b + A dpos) + (grad w . n) k, mass w*m
 __device__ __forceinline__ void p2g_record(const SimDev& sim, const InstDev& I, const StateView& S, const Patch& P,
                                            int q, int s, int g, P2GRec& r) {
@@ -129,12 +161,8 @@ __device__ __forceinline__ void p2g_record(const SimDev& sim, const InstDev& I, 
             m = I.m_vertex;
             V3<float> v = S.v(p);
             v.y += I.gdt;                                   // simulation.py:181-182
-            V3<float> f = {0.f, 0.f, 0.f};
-            for (int t = sim.inc_off[p]; t < sim.inc_off[p + 1]; ++t) {
-                int ec = sim.inc_list[t];
-                int e = ec / 3, c = ec - 3 * e;
-                f = f + load_v3(sim.fc + (size_t)3 * c * sim.NE, sim.NE, e);
-            }
+            V3<float> f = {0.f, 0.f, 0.f}, unused = {0.f, 0.f, 0.f};
+            incident_sum<false>(sim, p, sim.fc, nullptr, f, unused);
             b = m * v + f;
         } else if (P.cls == CLS_GRASPER) {
             m = I.m_op;

@@ -444,9 +444,7 @@ __global__ __launch_bounds__(DCP_WG) void k_prep_adj(SimDev sim, StateView S, St
             store_v3(sim.ec_gx, NE, e, gx1);
             store_v3(sim.ec_gx + (size_t)3 * NE, NE, e, gx2);
             store_v3(sim.ec_gx + (size_t)6 * NE, NE, e, gx3);
-            store_v3(sim.ec_gv, NE, e, gvc);
-            store_v3(sim.ec_gv + (size_t)3 * NE, NE, e, gvc);
-            store_v3(sim.ec_gv + (size_t)6 * NE, NE, e, gvc);
+            store_v3(sim.ec_gv, NE, e, gvc);          // the same for the three corners
             if (sim.per_elem) {
                 sim.emod_grad[e] += (double)gp[0];
                 sim.emod_grad[(size_t)NE + e] += (double)gp[2];
@@ -481,12 +479,7 @@ __global__ __launch_bounds__(DCP_WG) void k_vertex_final(SimDev sim, StateView G
         for (int q = threadIdx.x; q < P.count; q += DCP_WG) {
             const int p = P.start + q;
             V3<float> gx = Gout.x(p), gv = Gout.v(p);
-            for (int t = sim.inc_off[p]; t < sim.inc_off[p + 1]; ++t) {
-                int ec = sim.inc_list[t];
-                int e = ec / 3, c = ec - 3 * e;
-                gx = gx + load_v3(sim.ec_gx + (size_t)3 * c * NE, NE, e);
-                gv = gv + load_v3(sim.ec_gv + (size_t)3 * c * NE, NE, e);
-            }
+            incident_sum<true>(sim, p, sim.ec_gx, sim.ec_gv, gx, gv);
             store_v3(Gout.px(0), sim.NP, p, gx);
             store_v3(Gout.pv(0), sim.NP, p, gv);
         }

@@ -74,7 +74,7 @@ struct SimDev {
     // adjoint temporaries
     float* g_xe; float* g_ve; float* g_ne; float* g_ke;  // [3][NE]
     float* g_fv;                   // [3][NP] d/d (vertex force impulse)
-    float* ec_gx; float* ec_gv;    // [9][NE] per-corner d/dx, d/dv
+    float* ec_gx; float* ec_gv;    // [9][NE] per-corner d/dx; [3][NE] d/dv (one for the three corners)
     double* pgrad_patch;           // [n_patches][4]  E, nu, shear, contact (element patches)
     double* pgrad_inst;            // [B][DCP_N_PARAMS] (damping, friction via atomics)
     double* emod_grad;             // [3][NE]

@@ -96,7 +96,9 @@ def test_kernel_timing_events_eager_and_graph():
     eng.set_kernel_timing("p2g", 4 * S)
     eng.rollout_grad(S, 0, S, use_graph=False)
     ms_e, n_e = eng.kernel_timing()
-    assert n_e > 2 * (S - 2) and ms_e > 0
+    # the forward's P2G launches; the adjoint's, which exit at once when the
+    # grid snapshot holds the substep, are not timed
+    assert S - 1 <= n_e <= S and ms_e > 0
     eng.set_kernel_timing(None)
     ms_0, n_0 = eng.kernel_timing()
     assert n_0 == 0
