@@ -109,6 +109,8 @@ template <typename T> struct DevBuf {
 
 }  // namespace
 
+#define DCP_PREP_FLOATS 21   // x_e, v_e, k, n (3 each) and three corner forces (9) per element
+
 struct dcp_ctx {
     int B = 0, NP = 0, NE = 0, n_patches = 0, n_cu = 256;
     dcp_options opt{};
@@ -146,10 +148,21 @@ struct dcp_ctx {
     int* snap_ok = nullptr;
     int snap_cap = 0, snap_slots = 0;
     long long init_blocks = 0;     // 4^3 blocks the initial particle stencils touch
+    // element prep outputs (x_e, v_e, k, n, corner forces: DCP_PREP_FLOATS per
+    // element): the working block, and one block per snapshot slot when memory
+    // allows, so that the adjoint reuses the forward's instead of recomputing
+    float* prep_work = nullptr;
+    float* snap_prep = nullptr;
+    void use_prep(int slot) {
+        float* b = slot >= 0 && snap_prep ? snap_prep + (size_t)slot * DCP_PREP_FLOATS * NE : prep_work;
+        sim.xe = b; sim.ve = b + (size_t)3 * NE; sim.ke = b + (size_t)6 * NE; sim.nd = b + (size_t)9 * NE;
+        sim.fc = b + (size_t)12 * NE;
+    }
     void free_snapshots() {
-        for (void* q : {(void*)snap_acc, (void*)snap_blk, (void*)snap_cnt, (void*)snap_ok})
+        for (void* q : {(void*)snap_acc, (void*)snap_blk, (void*)snap_cnt, (void*)snap_ok, (void*)snap_prep})
             if (q) (void)hipFree(q);
         snap_acc = nullptr; snap_blk = snap_cnt = snap_ok = nullptr;
+        snap_prep = nullptr;
         snap_cap = snap_slots = 0;
     }
     GridSnap snap(int slot) const {
@@ -252,8 +265,18 @@ static int launch_prep(dcp_ctx* c, StateView S, StateView Sn, int fresh) {
     return 0;
 }
 
-// snap >= 0: save this substep's grid into snapshot slot `snap`
+// Points the kernels' element prep arrays at snapshot slot `slot` for the
+// lifetime of the guard (no slot, or no prep snapshots: the working arrays).
+struct PrepSlot {
+    dcp_ctx* c;
+    PrepSlot(dcp_ctx* c_, int slot) : c(c_) { c->use_prep(slot); }
+    ~PrepSlot() { c->use_prep(-1); }
+};
+
+// snap >= 0: save this substep's grid (and its element prep outputs) into
+// snapshot slot `snap`
 static int launch_forward(dcp_ctx* c, int s, int g, StateView S, StateView Sn, int snap = -1) {
+    PrepSlot ps(c, snap >= 0 && snap < c->snap_slots ? snap : -1);
     if (launch_prep(c, S, Sn, s == 0)) return -1;
     if (launch_p2g(c, S, s, g, nullptr)) return -1;
     TIMED_LAUNCH(DCP_KERNEL_GRIDOP, k_gridop, dim3(c->grid_gridop()), dim3(DCP_WG), 0, c->stream, c->sim, 0, c->snap(snap),
@@ -264,11 +287,21 @@ static int launch_forward(dcp_ctx* c, int s, int g, StateView S, StateView Sn, i
 
 // snap >= 0: the grid of substep s was saved in snapshot slot `snap` by the
 // forward; it is restored instead of recomputed (P2G and the grid update exit
-// at once unless the snapshot overflowed)
+// at once unless the snapshot overflowed), and the element prep outputs the
+// forward left in that slot are used as they are
 static int launch_adjoint(dcp_ctx* c, int s, int g, StateView S, StateView G1, StateView Gout, int snap = -1) {
     const int fresh = (s == 0);
     const GridSnap sn = c->snap(snap);
-    if (launch_prep(c, S, c->scratch_view(), fresh)) return -1;
+    const bool prep_saved = sn.acc && c->snap_prep;
+    PrepSlot ps(c, prep_saved ? snap : -1);
+    if (prep_saved) {
+        // what the prep would still do: clear the active-block counts for a
+        // P2G that runs because the grid snapshot overflowed
+        hipLaunchKernelGGL(k_clear_active, dim3(1), dim3(DCP_WG), 0, c->stream, c->sim, sn.ok);
+        LAUNCH_OK();
+    } else if (launch_prep(c, S, c->scratch_view(), fresh)) {
+        return -1;
+    }
     if (launch_p2g(c, S, s, g, sn.ok)) return -1;
     if (sn.ok) {
         hipLaunchKernelGGL(k_gridop, dim3(c->grid_gridop()), dim3(DCP_WG), 0, c->stream, c->sim, 1, GridSnap{}, sn.ok);
@@ -514,8 +547,9 @@ static int build(dcp_ctx* c, const dcp_instance_desc* d, int B) {
         if (c->alloc(&emod, 3 * (size_t)ne)) return -1;
         S.emod = emod;
     }
-    if (c->alloc(&S.xe, 3 * (size_t)ne) || c->alloc(&S.ve, 3 * (size_t)ne) || c->alloc(&S.ke, 3 * (size_t)ne) ||
-        c->alloc(&S.fc, 9 * (size_t)ne) || c->alloc(&S.nd, 3 * (size_t)ne) || c->alloc(&S.acc, (size_t)cells) ||
+    if (c->alloc(&c->prep_work, DCP_PREP_FLOATS * (size_t)ne)) return -1;
+    c->use_prep(-1);
+    if (c->alloc(&S.acc, (size_t)cells) ||
         c->alloc(&S.vout, (size_t)cells) || c->alloc(&S.gacc, (size_t)cells) || c->alloc(&S.gmv, (size_t)cells) ||
         c->alloc(&S.flags, (size_t)flags) || c->alloc(&S.active, (size_t)flags) || c->alloc(&S.n_active, (size_t)B) ||
         c->alloc(&S.status, 1) || c->alloc(&S.g_xe, 3 * (size_t)ne) || c->alloc(&S.g_ve, 3 * (size_t)ne) ||
@@ -1040,9 +1074,17 @@ static int ensure_snapshots(dcp_ctx* c, int K) {
     c->graphs.clear();
     c->free_snapshots();
     const size_t per_block = 64 * sizeof(float4) + sizeof(int);
+    // element prep slots when they fit in an eighth of the free memory
+    const size_t prep_b = (size_t)DCP_PREP_FLOATS * c->NE * sizeof(float) * (size_t)K;
+    const bool prep = prep_b <= free_b / 8;
+    const size_t grid_budget = free_b / 4 - (prep ? prep_b : 0);
     long long cap = std::min(all, std::max<long long>(1024, 4 * c->init_blocks));
-    cap = std::min<long long>(cap, (long long)(free_b / 4 / per_block / (size_t)K));
-    if (cap < 256) return 0;
+    cap = std::min<long long>(cap, (long long)(grid_budget / per_block / (size_t)K));
+    if (c->opt.grid_snapshots > 0)
+        cap = std::min<long long>(cap, c->opt.grid_snapshots);   // the caller's bound on blocks per slot
+    else if (cap < 256)
+        return 0;
+    if (cap < 1) return 0;
     if (hipMalloc(&c->snap_acc, (size_t)K * cap * 64 * sizeof(float4)) != hipSuccess ||
         hipMalloc(&c->snap_blk, (size_t)K * cap * sizeof(int)) != hipSuccess ||
         hipMalloc(&c->snap_cnt, (size_t)K * c->B * sizeof(int)) != hipSuccess ||
@@ -1050,6 +1092,10 @@ static int ensure_snapshots(dcp_ctx* c, int K) {
         (void)hipGetLastError();
         c->free_snapshots();
         return 0;   // no snapshots: the adjoint recomputes its grids
+    }
+    if (prep && hipMalloc(&c->snap_prep, prep_b) != hipSuccess) {
+        (void)hipGetLastError();
+        c->snap_prep = nullptr;   // grids only: the adjoint recomputes the element prep
     }
     c->snap_cap = (int)cap;
     c->snap_slots = K;

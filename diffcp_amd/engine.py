@@ -25,6 +25,15 @@ def _fp(a: Optional[np.ndarray]):
     return a.ctypes.data_as(_abi.c_flt_p)
 
 
+def _snap_option(v) -> int:
+    """dcp_options.grid_snapshots from True (on), False (off) or a block bound per substep."""
+    if isinstance(v, bool):
+        return 0 if v else -1
+    if int(v) <= 0:
+        raise ValueError("grid_snapshots: True, False or a positive block bound")
+    return int(v)
+
+
 class Engine:
     """Owns a ``dcp_ctx``: all device buffers of B instances.
 
@@ -74,7 +83,7 @@ class Engine:
         opt = _abi.Options(max_states=max_states, precision=PRECISION[precision],
                            svd_grad=SVD_GRAD[svd_grad], per_element_moduli=int(per_element_moduli),
                            patch_size=patch_size, scatter_wgs=scatter_wgs, grid_wgs=grid_wgs,
-                           bin_cells=bin_cells, grid_snapshots=0 if grid_snapshots else -1)
+                           bin_cells=bin_cells, grid_snapshots=_snap_option(grid_snapshots))
         ctx = ctypes.c_void_p()
         _abi.check(self.lib.dcp_create(descs, self.B, ctypes.byref(opt), stream, ctypes.byref(ctx)))
         self.ctx = ctx

@@ -74,8 +74,10 @@ typedef struct dcp_options {
     int32_t scatter_wgs;         /* persistent workgroups of the per-patch kernels; 0 = default (8 per CU) */
     int32_t grid_wgs;            /* persistent grid-update workgroups; 0 = default */
     int32_t bin_cells;           /* side of the spatial bins patches are cut from (grid cells, 2..16); 0 = 6, -1 = per-instance from particle density */
-    int32_t grid_snapshots;      /* 0: dcp_rollout_grad keeps each substep's grid of a checkpoint segment for the
-                                    adjoint instead of recomputing it (memory permitting); -1: always recompute */
+    int32_t grid_snapshots;      /* 0: dcp_rollout_grad keeps each substep's grid (and element prep outputs) of a
+                                    checkpoint segment for the adjoint instead of recomputing them (memory
+                                    permitting); > 0: the same with at most this many 4^3 blocks per substep
+                                    (a substep whose grid overflows it is recomputed); -1: always recompute */
     int32_t reserved;
 } dcp_options;
 

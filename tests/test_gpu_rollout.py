@@ -90,6 +90,25 @@ def test_checkpoint_plans_agree():
         assert _rel(g[:, :5], res[0][1][:, :5]) < 1e-5
 
 
+def test_grid_snapshots_agree_with_recompute():
+    """Snapshots on (grids and element prep reused), off (recomputed) and
+    bounded to 4 blocks per substep (every grid overflows: P2G reruns on the
+    saved element prep) give the same window, with checkpoint recompute."""
+    S, K = 120, 40
+    res = []
+    for snaps in (False, True, 4):
+        eng = _c1_engine(S, K, grid_snapshots=snaps)
+        l, g, _ = eng.rollout_grad(S, 0, K, use_graph=True)
+        l2, g2, _ = eng.rollout_grad(S, 0, K, use_graph=True)
+        assert _rel(l2, l) < 1e-5 and _rel(g2[:, :5], g[:, :5]) < 1e-5, snaps
+        assert eng.status() & 3 == 0
+        res.append((l, g))
+        eng.close()
+    for l, g in res[1:]:
+        assert _rel(l, res[0][0]) < 1e-5
+        assert _rel(g[:, :5], res[0][1][:, :5]) < 1e-5
+
+
 def test_kernel_timing_events_eager_and_graph():
     S = 60
     eng = _c1_engine(S, S)
