@@ -232,55 +232,112 @@ __global__ __launch_bounds__(DCP_WG) void k_gridop_adj(SimDev sim) {
 // ---------------------------------------------------------------------------
 // Reverse of one particle's P2G scatter: dL/d(m, m v) of its 27 stencil nodes
 // (fetched by `node(i, j, k)`) to the gradients of the particle's record.
+//
+// Per node N = (i, j, k) the scatter adds w (b + A dp) + (grad w . n) k and
+// w m, with w = W0(i) W1(j) W2(k), dp = ((i, j, k) - fx) dx.  Every sum over
+// the 27 nodes that its reverse needs is of the form
+//   sum_N f0(i) f1(j) f2(k) g_c(N),   f in {W, D = dW/dfx, H = d2W/dfx2, W dp, D dp},
+// with g_c a component of dL/d(m, m v) at N, so it is contracted one axis at
+// a time: over k into T (per (i, j)), over j into U (per i), over i into the
+// record gradients.  U(f1, f2) below holds sum_{j,k} f1(j) f2(k) g(i, j, k).
+// The terms and their grouping follow the per-node form
+//   gb += w g,  gA += w g dp^T,  gfx += dW (g_m m + g . (b + A dp)) - w dx A^T g
+//   elements: gk += (dW . n) idx g,  gn += (g . k) idx dW,
+//             gfx += (g . k) idx (d2W n)
 template <typename NodeFetch>
 __device__ __forceinline__ void p2g_adj_particle(const SimDev& sim, const InstDev& I, const StateView& Gout,
                                                  const Patch& P, int q, int s, int g, const P2GRec& r,
                                                  const Stencil& st, NodeFetch node) {
-    const float ddw[3] = {1.0f, -2.0f, 1.0f};
+        const float hf[3] = {1.0f, -2.0f, 1.0f};   // d2W/dfx2 of the quadratic B-spline nodes
+        const float dx = I.dx, idx = I.inv_dx;
+        const bool elem = r.elem;
         float gb[3] = {0.f, 0.f, 0.f}, gfx[3] = {0.f, 0.f, 0.f}, gn[3] = {0.f, 0.f, 0.f}, gk[3] = {0.f, 0.f, 0.f};
         M3<float> gA = mzero<float>();
-        const float dx = I.dx, idx = I.inv_dx;
 #pragma unroll
-        for (int i = 0; i < 3; ++i)
+        for (int i = 0; i < 3; ++i) {
+            // U(f1, f2): .x = mass channel, .yzw = momentum channels
+            float4 uWW = {0.f, 0.f, 0.f, 0.f}, uDW = {0.f, 0.f, 0.f, 0.f}, uWD = {0.f, 0.f, 0.f, 0.f};
+            float uPW[3] = {0.f, 0.f, 0.f}, uQW[3] = {0.f, 0.f, 0.f}, uHW[3] = {0.f, 0.f, 0.f};   // P = W dp, Q = D dp
+            float uWP[3] = {0.f, 0.f, 0.f}, uDP[3] = {0.f, 0.f, 0.f}, uPD[3] = {0.f, 0.f, 0.f};
+            float uDD[3] = {0.f, 0.f, 0.f}, uWQ[3] = {0.f, 0.f, 0.f}, uWH[3] = {0.f, 0.f, 0.f};
 #pragma unroll
-            for (int j = 0; j < 3; ++j)
+            for (int j = 0; j < 3; ++j) {
+                float4 tW = {0.f, 0.f, 0.f, 0.f}, tD = {0.f, 0.f, 0.f, 0.f};
+                float tP[3] = {0.f, 0.f, 0.f}, tQ[3] = {0.f, 0.f, 0.f}, tH[3] = {0.f, 0.f, 0.f};
 #pragma unroll
                 for (int k = 0; k < 3; ++k) {
-                    float4 gg = node(i, j, k);
-                    float W0 = st.w[i][0], W1 = st.w[j][1], W2 = st.w[k][2];
-                    float D0 = st.dw[i][0], D1 = st.dw[j][1], D2 = st.dw[k][2];
-                    float w = W0 * W1 * W2;
-                    float dp[3] = {((float)i - st.fx[0]) * dx, ((float)j - st.fx[1]) * dx, ((float)k - st.fx[2]) * dx};
-                    float gm3[3] = {gg.y, gg.z, gg.w};
-                    float u[3];
+                    const float4 gg = node(i, j, k);
+                    const float w2 = st.w[k][2], d2 = st.dw[k][2], p2 = ((float)k - st.fx[2]) * dx;
+                    const float wp2 = w2 * p2, dp2 = d2 * p2;
+                    tW.x += w2 * gg.x; tW.y += w2 * gg.y; tW.z += w2 * gg.z; tW.w += w2 * gg.w;
+                    tD.x += d2 * gg.x; tD.y += d2 * gg.y; tD.z += d2 * gg.z; tD.w += d2 * gg.w;
+                    tP[0] += wp2 * gg.y; tP[1] += wp2 * gg.z; tP[2] += wp2 * gg.w;
+                    tQ[0] += dp2 * gg.y; tQ[1] += dp2 * gg.z; tQ[2] += dp2 * gg.w;
+                    if (elem) { tH[0] += hf[k] * gg.y; tH[1] += hf[k] * gg.z; tH[2] += hf[k] * gg.w; }
+                }
+                const float w1 = st.w[j][1], d1 = st.dw[j][1], p1 = ((float)j - st.fx[1]) * dx;
+                const float wp1 = w1 * p1, dp1 = d1 * p1;
+                uWW.x += w1 * tW.x; uWW.y += w1 * tW.y; uWW.z += w1 * tW.z; uWW.w += w1 * tW.w;
+                uDW.x += d1 * tW.x; uDW.y += d1 * tW.y; uDW.z += d1 * tW.z; uDW.w += d1 * tW.w;
+                uWD.x += w1 * tD.x; uWD.y += w1 * tD.y; uWD.z += w1 * tD.z; uWD.w += w1 * tD.w;
+                const float tWa[3] = {tW.y, tW.z, tW.w}, tDa[3] = {tD.y, tD.z, tD.w};
 #pragma unroll
-                    for (int a = 0; a < 3; ++a) u[a] = (&r.b.x)[a] + r.A(a, 0) * dp[0] + r.A(a, 1) * dp[1] + r.A(a, 2) * dp[2];
-#pragma unroll
-                    for (int a = 0; a < 3; ++a) {
-                        gb[a] += w * gm3[a];
-#pragma unroll
-                        for (int b = 0; b < 3; ++b) gA(a, b) += w * gm3[a] * dp[b];
-                    }
-                    float scal = gg.x * r.m + gm3[0] * u[0] + gm3[1] * u[1] + gm3[2] * u[2];
-                    float dW[3] = {D0 * W1 * W2, W0 * D1 * W2, W0 * W1 * D2};
-#pragma unroll
-                    for (int d = 0; d < 3; ++d) {
-                        float gAd = gm3[0] * r.A(0, d) + gm3[1] * r.A(1, d) + gm3[2] * r.A(2, d);
-                        gfx[d] += dW[d] * scal - w * dx * gAd;
-                    }
-                    if (r.elem) {
-                        float gw[3] = {idx * dW[0], idx * dW[1], idx * dW[2]};
-                        float sdot = gw[0] * r.n.x + gw[1] * r.n.y + gw[2] * r.n.z;
-                        float qd = gm3[0] * r.k.x + gm3[1] * r.k.y + gm3[2] * r.k.z;
-#pragma unroll
-                        for (int a = 0; a < 3; ++a) { gk[a] += sdot * gm3[a]; gn[a] += qd * gw[a]; }
-                        float H0 = ddw[i], H1 = ddw[j], H2 = ddw[k];
-                        float dg0 = idx * (H0 * W1 * W2 * r.n.x + D0 * D1 * W2 * r.n.y + D0 * W1 * D2 * r.n.z);
-                        float dg1 = idx * (D0 * D1 * W2 * r.n.x + W0 * H1 * W2 * r.n.y + W0 * D1 * D2 * r.n.z);
-                        float dg2 = idx * (D0 * W1 * D2 * r.n.x + W0 * D1 * D2 * r.n.y + W0 * W1 * H2 * r.n.z);
-                        gfx[0] += qd * dg0; gfx[1] += qd * dg1; gfx[2] += qd * dg2;
+                for (int a = 0; a < 3; ++a) {
+                    uPW[a] += wp1 * tWa[a];
+                    uQW[a] += dp1 * tWa[a];
+                    uWP[a] += w1 * tP[a];
+                    uDP[a] += d1 * tP[a];
+                    uPD[a] += wp1 * tDa[a];
+                    uWQ[a] += w1 * tQ[a];
+                    if (elem) {
+                        uHW[a] += hf[j] * tWa[a];
+                        uDD[a] += d1 * tDa[a];
+                        uWH[a] += w1 * tH[a];
                     }
                 }
+                __builtin_amdgcn_sched_barrier(0);
+            }
+            const float w0 = st.w[i][0], d0 = st.dw[i][0], p0 = ((float)i - st.fx[0]) * dx;
+            const float wp0 = w0 * p0, dp0 = d0 * p0;
+            const float aWW[3] = {uWW.y, uWW.z, uWW.w}, aDW[3] = {uDW.y, uDW.z, uDW.w}, aWD[3] = {uWD.y, uWD.z, uWD.w};
+            float s0 = r.m * uWW.x, s1 = r.m * uDW.x, s2 = r.m * uWD.x;       // g_m m + g . b
+            float t00 = 0.f, t01 = 0.f, t02 = 0.f, t10 = 0.f, t11 = 0.f, t12 = 0.f, t20 = 0.f, t21 = 0.f, t22 = 0.f;
+            float c1 = 0.f, c2 = 0.f;                                          // (A^T sum w g)_{1,2}
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                const float ba = (&r.b.x)[a];
+                gb[a] += w0 * aWW[a];
+                gA(a, 0) += wp0 * aWW[a];
+                gA(a, 1) += w0 * uPW[a];
+                gA(a, 2) += w0 * uWP[a];
+                s0 += ba * aWW[a]; s1 += ba * aDW[a]; s2 += ba * aWD[a];
+                t00 += r.A(a, 0) * aWW[a]; t01 += r.A(a, 1) * uPW[a]; t02 += r.A(a, 2) * uWP[a];
+                t10 += r.A(a, 0) * aDW[a]; t11 += r.A(a, 1) * uQW[a]; t12 += r.A(a, 2) * uDP[a];
+                t20 += r.A(a, 0) * aWD[a]; t21 += r.A(a, 1) * uPD[a]; t22 += r.A(a, 2) * uWQ[a];
+                c1 += r.A(a, 1) * aWW[a]; c2 += r.A(a, 2) * aWW[a];
+            }
+            gfx[0] += d0 * (s0 + t01 + t02) + (dp0 - dx * w0) * t00;
+            gfx[1] += w0 * (s1 + t11 + t12 - dx * c1) + wp0 * t10;
+            gfx[2] += w0 * (s2 + t21 + t22 - dx * c2) + wp0 * t20;
+            if (elem) {
+                const float n0 = r.n.x, n1 = r.n.y, n2 = r.n.z;
+                float kWW = 0.f, kDW = 0.f, kWD = 0.f, kHW = 0.f, kDD = 0.f, kWH = 0.f;
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    const float ka = (&r.k.x)[a];
+                    gk[a] += idx * (n0 * d0 * aWW[a] + n1 * w0 * aDW[a] + n2 * w0 * aWD[a]);
+                    kWW += ka * aWW[a]; kDW += ka * aDW[a]; kWD += ka * aWD[a];
+                    kHW += ka * uHW[a]; kDD += ka * uDD[a]; kWH += ka * uWH[a];
+                }
+                gn[0] += idx * d0 * kWW; gn[1] += idx * w0 * kDW; gn[2] += idx * w0 * kWD;
+                gfx[0] += idx * (n0 * hf[i] * kWW + n1 * d0 * kDW + n2 * d0 * kWD);
+                gfx[1] += idx * (n0 * d0 * kDW + n1 * w0 * kHW + n2 * w0 * kDD);
+                gfx[2] += idx * (n0 * d0 * kWD + n1 * w0 * kDD + n2 * w0 * kWH);
+            }
+            // keep the next plane's node reads from being hoisted above this
+            // one's contraction (they would all be live at once)
+            __builtin_amdgcn_sched_barrier(0);
+        }
         V3<float> gx = {idx * gfx[0], idx * gfx[1], idx * gfx[2]};
         if (P.cls == CLS_ELEMENT) {
             const int e = P.start + q;
