@@ -335,25 +335,43 @@ __global__ __launch_bounds__(DCP_WG) void k_g2p(SimDev sim, StateView S, StateVi
             V3<float> x = particle_x(sim, S, P, q);
             Stencil st;
             make_stencil(x, I.inv_dx, I.G, st, nullptr);
+            // new_v = sum w g, new_C = 4 inv_dx sum w g dp^T (simulation.py:317-330),
+            // dp = (i, j, k) - fx in cells; w = W0(i) W1(j) W2(k) is separable, so
+            // the sums are contracted over k, then j, then i:
+            // u*(f1, f2) = sum_{j,k} f1(j) f2(k) g with f in {W, W dp}
             V3<float> nv = {0.f, 0.f, 0.f};
             M3<float> nC = mzero<float>();
 #pragma unroll
-            for (int i = 0; i < 3; ++i)
+            for (int i = 0; i < 3; ++i) {
+                float uWW[3] = {0.f, 0.f, 0.f}, uPW[3] = {0.f, 0.f, 0.f}, uWP[3] = {0.f, 0.f, 0.f};
 #pragma unroll
-                for (int j = 0; j < 3; ++j)
+                for (int j = 0; j < 3; ++j) {
+                    float tW[3] = {0.f, 0.f, 0.f}, tP[3] = {0.f, 0.f, 0.f};
 #pragma unroll
                     for (int k = 0; k < 3; ++k) {
-                        float4 gv = sim.vout[cell_index(I, st.base[0] + i, st.base[1] + j, st.base[2] + k)];
-                        float w = st.w[i][0] * st.w[j][1] * st.w[k][2];
-                        float dp[3] = {(float)i - st.fx[0], (float)j - st.fx[1], (float)k - st.fx[2]};
-                        nv.x += w * gv.x; nv.y += w * gv.y; nv.z += w * gv.z;
-                        float w4 = 4.0f * w;
-                        float g3[3] = {gv.x, gv.y, gv.z};
-#pragma unroll
-                        for (int a = 0; a < 3; ++a)
-#pragma unroll
-                            for (int b = 0; b < 3; ++b) nC(a, b) += (w4 * (g3[a] * dp[b])) * I.inv_dx;
+                        const float4 gv = sim.vout[cell_index(I, st.base[0] + i, st.base[1] + j, st.base[2] + k)];
+                        const float w2 = st.w[k][2], wp2 = w2 * ((float)k - st.fx[2]);
+                        tW[0] += w2 * gv.x; tW[1] += w2 * gv.y; tW[2] += w2 * gv.z;
+                        tP[0] += wp2 * gv.x; tP[1] += wp2 * gv.y; tP[2] += wp2 * gv.z;
                     }
+                    const float w1 = st.w[j][1], wp1 = w1 * ((float)j - st.fx[1]);
+#pragma unroll
+                    for (int a = 0; a < 3; ++a) {
+                        uWW[a] += w1 * tW[a];
+                        uPW[a] += wp1 * tW[a];
+                        uWP[a] += w1 * tP[a];
+                    }
+                }
+                const float w0 = st.w[i][0], wp0 = w0 * ((float)i - st.fx[0]);
+                nv.x += w0 * uWW[0]; nv.y += w0 * uWW[1]; nv.z += w0 * uWW[2];
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    nC(a, 0) += wp0 * uWW[a];
+                    nC(a, 1) += w0 * uPW[a];
+                    nC(a, 2) += w0 * uWP[a];
+                }
+            }
+            nC = mscale(4.0f * I.inv_dx, nC);
             if (P.cls == CLS_ELEMENT) {
                 const int e = P.start + q;
                 M3<float> out;
