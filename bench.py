@@ -15,11 +15,12 @@ Also reported on the same JSON line:
     every 50, one launch per kernel, no graph) and the batched/plain ratio;
   * the GPU gradient w.r.t. E and nu vs central finite differences of the
     FP64 oracle on the 32x32 cloth (committed fixture, h = 1e-4 |theta|);
-  * ``roofline``: the P2G kernel's algorithmic bytes per launch over its
-    average launch duration, measured with HIP events recorded on the
-    library's stream around every P2G launch of the timed region, against the
-    8 TB/s HBM peak; ``traffic`` comes from the committed rocprofv3 counter
-    summary when present;
+  * ``roofline``: the G2P adjoint (the kernel with the largest share of the
+    window): its algorithmic bytes per launch over its average launch
+    duration, measured with HIP events recorded on the library's stream
+    around every G2P-adjoint launch of the timed region, against the 8 TB/s
+    HBM peak; ``traffic`` comes from the committed rocprofv3 counter summary
+    when present;
   * ``cpu_baseline``: the FP64 CPU restatement in ``oracle/`` timed on the
     host on a bounded sample of the same workload.
 
@@ -77,18 +78,23 @@ def distinct_active_cells(eng, slot: int) -> int:
     return int(np.unique(np.concatenate(cells)).size)
 
 
-def p2g_algorithmic_bytes(w, active_cells: int) -> float:
-    """Compulsory HBM bytes of one P2G launch: vertex x, v, C (60 B), element
-    C, normal column, Dinv and triangle (96 B), and (m, m v) written once per
-    distinct active cell (16 B)."""
-    nv = sum(i.n_vertices for i in w.instances)
+ROOFLINE_KERNEL = "g2p_adj"     # the kernel with the largest share of the window
+
+
+def g2p_adj_algorithmic_bytes(w, active_cells: int) -> float:
+    """Compulsory HBM bytes of one G2P-adjoint launch: per vertex, grasper and
+    cube particle its position and the incoming dL/d(x, v, C) read and dL/dx
+    written (84 B); per element its position and dL/dC read and dL/dx written
+    (60 B); per distinct active cell grid_v_out read and dL/d(m v) written
+    (12 + 12 B)."""
+    npt = sum(i.n_vertices + i.n_grasper + i.n_cube for i in w.instances)
     ne = sum(i.n_elements for i in w.instances)
-    return 60.0 * nv + 96.0 * ne + 16.0 * active_cells
+    return 84.0 * npt + 60.0 * ne + 24.0 * active_cells
 
 
 def load_traffic():
-    """HBM bytes per P2G launch from the committed rocprofv3 counter summary."""
-    path = os.path.join(ROOT, "profiles", "k_p2g_c3_traffic.json")
+    """HBM bytes per G2P-adjoint launch from the committed rocprofv3 counter summary."""
+    path = os.path.join(ROOT, "profiles", "k_g2p_adj_c3_traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
@@ -108,11 +114,11 @@ def bench_batched(steps: int, warmup: int, use_graph: bool = True):
     slot, _ = eng.rollout(S - 2)
     cells = 0.5 * (distinct_active_cells(eng, 0) + distinct_active_cells(eng, slot))
     eng.reset(0)
-    # per-launch HIP events around P2G, captured into the rollout graphs.  torch
+    # per-launch HIP events around the G2P adjoint, captured into the rollout graphs.  torch
     # is kept out of this process until the GPU legs are done: with torch's
     # HIP runtime initialised first, capturing an event-record node fails
     # (hipErrorInvalidValue), while the same capture succeeds without it.
-    eng.set_kernel_timing("p2g", max_launches=4 * S + 16)
+    eng.set_kernel_timing(ROOFLINE_KERNEL, max_launches=4 * S + 16)
     for _ in range(warmup):
         eng.rollout_grad(S, 0, K, use_graph=use_graph)
     eng.status(clear=True)
@@ -124,11 +130,11 @@ def bench_batched(steps: int, warmup: int, use_graph: bool = True):
         dev_ms += ms
     eng.synchronize()
     wall = time.perf_counter() - t0
-    p2g_ms, p2g_n = eng.kernel_timing()
+    k_ms, k_n = eng.kernel_timing()
     status = eng.status()
     finite = bool(np.isfinite(loss).all() and np.isfinite(grads).all())
     eng.close()
-    return dict(w=w, S=S, K=K, wall=wall, dev_ms=dev_ms, cells=cells, p2g_ms=p2g_ms, p2g_n=p2g_n,
+    return dict(w=w, S=S, K=K, wall=wall, dev_ms=dev_ms, cells=cells, k_ms=k_ms, k_n=k_n,
                 status=status, finite=finite)
 
 
@@ -242,13 +248,13 @@ def main() -> int:
     value = w.particles * S * args.steps / b["wall"]
     log(f"batched: {value / 1e9:.4f} G particle-substeps/s, {b['wall'] / args.steps * 1e3:.1f} ms/step")
 
-    avg_ms = b["p2g_ms"] / max(b["p2g_n"], 1)
-    alg = p2g_algorithmic_bytes(w, int(b["cells"]))
-    achieved = alg / (avg_ms * 1e-3) / 1e9 if b["p2g_n"] else None
+    avg_ms = b["k_ms"] / max(b["k_n"], 1)
+    alg = g2p_adj_algorithmic_bytes(w, int(b["cells"]))
+    achieved = alg / (avg_ms * 1e-3) / 1e9 if b["k_n"] else None
     traffic = load_traffic()
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": traffic,
-                "kernel": "k_p2g", "launches_timed": b["p2g_n"], "avg_launch_us": avg_ms * 1e3,
+                "kernel": "k_" + ROOFLINE_KERNEL, "launches_timed": b["k_n"], "avg_launch_us": avg_ms * 1e3,
                 "algorithmic_bytes_per_launch": alg}
 
     out = {

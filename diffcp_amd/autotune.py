@@ -100,6 +100,8 @@ def tune(instances: Sequence[Instance], substeps: int = 24, use_cache: bool = Tr
     best = dict(DEFAULTS)
     best_ms = measure(instances, best, substeps, **engine_kw)
     trials = [(dict(best), best_ms)]
+    if verbose:
+        print(f"[autotune] {best} -> {best_ms:.2f} ms (defaults)", flush=True)
     for knob in ORDER:
         for value in cands[knob]:
             if value == best[knob]:

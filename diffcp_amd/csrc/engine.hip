@@ -294,13 +294,12 @@ static int launch_adjoint(dcp_ctx* c, int s, int g, StateView S, StateView G1, S
     const GridSnap sn = c->snap(snap);
     const bool prep_saved = sn.acc && c->snap_prep;
     PrepSlot ps(c, prep_saved ? snap : -1);
-    if (prep_saved) {
-        // what the prep would still do: clear the active-block counts for a
-        // P2G that runs because the grid snapshot overflowed
-        hipLaunchKernelGGL(k_clear_active, dim3(1), dim3(DCP_WG), 0, c->stream, c->sim, sn.ok);
+    if (!prep_saved && launch_prep(c, S, c->scratch_view(), fresh)) return -1;
+    if (sn.acc) {
+        // before the P2G: an overflowed snapshot leaves the active-block counts
+        // cleared for it
+        hipLaunchKernelGGL(k_grid_restore, dim3(c->grid_gridop()), dim3(DCP_WG), 0, c->stream, c->sim, sn);
         LAUNCH_OK();
-    } else if (launch_prep(c, S, c->scratch_view(), fresh)) {
-        return -1;
     }
     if (launch_p2g(c, S, s, g, sn.ok)) return -1;
     if (sn.ok) {
@@ -309,10 +308,6 @@ static int launch_adjoint(dcp_ctx* c, int s, int g, StateView S, StateView G1, S
     } else {
         TIMED_LAUNCH(DCP_KERNEL_GRIDOP, k_gridop, dim3(c->grid_gridop()), dim3(DCP_WG), 0, c->stream, c->sim, 1, GridSnap{},
                      nullptr);
-    }
-    if (sn.acc) {
-        hipLaunchKernelGGL(k_grid_restore, dim3(c->grid_gridop()), dim3(DCP_WG), 0, c->stream, c->sim, sn);
-        LAUNCH_OK();
     }
     TIMED_LAUNCH(DCP_KERNEL_G2P_ADJ, k_g2p_adj, dim3(c->grid_patches()), dim3(DCP_WG), 0, c->stream, c->sim, S, G1, Gout);
     TIMED_LAUNCH(DCP_KERNEL_GRIDOP_ADJ, k_gridop_adj, dim3(c->grid_gridop()), dim3(DCP_WG), 0, c->stream, c->sim);

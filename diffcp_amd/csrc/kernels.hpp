@@ -299,7 +299,6 @@ template <typename PT> __global__ void k_prep(SimDev sim, StateView S, StateView
 __global__ void k_p2g(SimDev sim, StateView S, int s, int g, const int* skip);
 __global__ void k_gridop(SimDev sim, int keep, GridSnap save, const int* skip);
 __global__ void k_grid_restore(SimDev sim, GridSnap snap);
-__global__ void k_clear_active(SimDev sim, const int* skip);
 __global__ void k_g2p(SimDev sim, StateView S, StateView Sn);
 __global__ void k_loss_min(SimDev sim, StateView S, const float* targets, const int4* tchunks, int n_chunks,
                            const int* vnat, float* tmin, int* targ);

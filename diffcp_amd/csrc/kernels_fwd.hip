@@ -283,18 +283,16 @@ __global__ __launch_bounds__(DCP_WG) void k_gridop(SimDev sim, int keep, GridSna
     }
 }
 
-// The active-block counts a P2G starts from, cleared unless `skip` says the
-// P2G will not run (one workgroup).
-__global__ __launch_bounds__(DCP_WG) void k_clear_active(SimDev sim, const int* skip) {
-    if (*skip) return;
-    for (int b = threadIdx.x; b < sim.B; b += blockDim.x) sim.n_active[b] = 0;
-}
-
 // Rebuild a substep's grid from its snapshot: (m, m v), v_out, block flags and
 // the per-instance active lists (what P2G + the grid update with keep = 1
-// leave behind).  Does nothing when the snapshot overflowed.
+// leave behind).  When the snapshot overflowed it only clears the active-block
+// counts, for the P2G that then runs.
 __global__ __launch_bounds__(DCP_WG) void k_grid_restore(SimDev sim, GridSnap snap) {
-    if (!*snap.ok) return;
+    if (!*snap.ok) {
+        if (blockIdx.x == 0)
+            for (int b = threadIdx.x; b < sim.B; b += blockDim.x) sim.n_active[b] = 0;
+        return;
+    }
     const int lane = threadIdx.x & 63;
     const int per = max(1, (int)gridDim.x / sim.B);
     const int b = blockIdx.x / per;
