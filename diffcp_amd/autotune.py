@@ -12,8 +12,10 @@ The knobs (all in ``dcp_options``):
 ``tune`` times short forward + adjoint windows of the caller's own instances
 (device time from ``dcp_rollout_grad``, graphs on) and walks the knobs one at
 a time (coordinate descent from the defaults), keeping the fastest value of
-each.  Results are cached in a JSON file keyed by a signature of the workload
-(instance count, grid sizes, particle counts) and the device's CU count.
+each.  Results are cached in a JSON file (``$DIFFCP_TUNE_CACHE``, default
+``~/.cache/diffcp_amd/autotune.json``, one per machine) keyed by a signature
+of the workload (instance count, grid sizes, particle counts) and the window
+length.
 """
 from __future__ import annotations
 
@@ -31,7 +33,7 @@ DEFAULTS = {"bin_cells": 6, "patch_size": 256, "scatter_wgs": 0, "grid_wgs": 0}
 CANDIDATES = {
     "bin_cells": (4, 5, 6, 8, 10),
     "patch_size": (128, 192, 256),
-    "scatter_wgs": (0, 1024, 4096),
+    "scatter_wgs": (0, 2048, 8192),
     "grid_wgs": (0, 512, 2048),
 }
 ORDER = ("bin_cells", "patch_size", "scatter_wgs", "grid_wgs")

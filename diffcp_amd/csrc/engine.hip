@@ -211,7 +211,7 @@ struct dcp_ctx {
         return 0;
     }
     int grid_patches() const {
-        return std::max(1, std::min(n_patches, opt.scatter_wgs > 0 ? opt.scatter_wgs : n_cu * 8));
+        return std::max(1, std::min(n_patches, opt.scatter_wgs > 0 ? opt.scatter_wgs : n_cu * 16));
     }
     int grid_elems() const { return std::max(1, std::min((NE + DCP_WG - 1) / DCP_WG, n_cu * 16)); }
     // B x per workgroups walking the per-instance active block lists

@@ -71,7 +71,7 @@ typedef struct dcp_options {
     int32_t svd_grad;            /* 0: reference clamped SVD backward; 1: exact polar derivative */
     int32_t per_element_moduli;  /* 1: per-triangle (E, shear, contact) arrays     */
     int32_t patch_size;          /* particles per P2G/G2P patch (<= 1024); 0 = default */
-    int32_t scatter_wgs;         /* persistent workgroups of the per-patch kernels; 0 = default (8 per CU) */
+    int32_t scatter_wgs;         /* persistent workgroups of the per-patch kernels; 0 = default (16 per CU) */
     int32_t grid_wgs;            /* persistent grid-update workgroups; 0 = default */
     int32_t bin_cells;           /* side of the spatial bins patches are cut from (grid cells, 2..16); 0 = 6, -1 = per-instance from particle density */
     int32_t grid_snapshots;      /* 0: dcp_rollout_grad keeps each substep's grid (and element prep outputs) of a

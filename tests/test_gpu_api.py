@@ -88,3 +88,21 @@ def test_autograd_function_scales_unit_seed_gradients():
     l2, g2, _ = eng.rollout_grad(S, 0, 8, use_graph=False)
     assert _rel(loss.detach().numpy(), l2) < 1e-5
     assert _rel(params.grad.numpy(), w.numpy()[:, None] * g2) < 1e-5
+
+
+def test_autotune_returns_a_measured_candidate(tmp_path, monkeypatch):
+    from diffcp_amd import autotune
+    from diffcp_amd.engine import Engine
+
+    monkeypatch.setenv("DIFFCP_TUNE_CACHE", str(tmp_path / "tune.json"))
+    insts = [_instance(), _instance()]
+    cands = {"bin_cells": (4, 6), "patch_size": (128, 256), "scatter_wgs": (0,), "grid_wgs": (0,)}
+    cfg = autotune.tune(insts, substeps=8, candidates=cands)
+    assert cfg["bin_cells"] in (4, 6) and cfg["patch_size"] in (128, 256)
+    assert (tmp_path / "tune.json").exists()
+    # the cached result comes back without measuring
+    assert autotune.tune(insts, substeps=8, candidates=cands) == cfg
+    eng = Engine(insts, max_states=Engine.slots_for(8, 8), **cfg)
+    eng.reset(0)
+    loss, grads, ms = eng.rollout_grad(8, 0, 8, use_graph=True)
+    assert np.isfinite(loss).all() and np.isfinite(grads).all() and ms > 0
