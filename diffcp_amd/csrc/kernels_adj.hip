@@ -101,14 +101,15 @@ __device__ __forceinline__ void g2p_adj_spill(const SimDev& sim, const StateView
 // tile node adds its nine run sums.  dL/dx of each particle accumulates over
 // the three layers and is stored after the last.  grid_v_out of the tile is staged in
 // LDS once per patch, so every node read after that is an LDS read.
-__global__ __launch_bounds__(DCP_WG, 2) void k_g2p_adj(SimDev sim, StateView S, StateView G1, StateView Gout) {
+__global__ __launch_bounds__(DCP_WG, 3) void k_g2p_adj(SimDev sim, StateView S, StateView G1, StateView Gout) {
     __shared__ BinLDS L;
-    __shared__ float4 contrib[9 * DCP_SORT_MAX + 1];
+    __shared__ float2 cA[9 * DCP_SORT_MAX + 1];   // contributions, channels 0 and 1
+    __shared__ float cB[9 * DCP_SORT_MAX + 1];    // channel 2
     __shared__ float4 vt[DCP_STILE_CAP];        // grid_v_out over the patch tile
     const int tid = threadIdx.x;
     bin_clear(L);
     if (tid < 3) { L.box[tid] = 1 << 30; L.box[3 + tid] = -(1 << 30); }
-    if (tid == 0) contrib[DCP_CONTRIB_ZERO] = make_float4(0.f, 0.f, 0.f, 0.f);   // never overwritten
+    if (tid == 0) { cA[DCP_CONTRIB_ZERO] = make_float2(0.f, 0.f); cB[DCP_CONTRIB_ZERO] = 0.f; }   // never overwritten
     __syncthreads();
     for (int pi = blockIdx.x; pi < sim.n_patches; pi += gridDim.x) {
         const Patch P = sim.patches[pi];
@@ -162,17 +163,18 @@ __global__ __launch_bounds__(DCP_WG, 2) void k_g2p_adj(SimDev sim, StateView S, 
                                 float4 gvo = vt[key + (i * B.e1 + j) * B.e2 + k];
                                 float gg[3];
                                 g2p_adj_node(I, st, i, j, k, gvo, gv, gC, gg, gfx);
-                                contrib[(j * 3 + k) * DCP_SORT_MAX + pos] = make_float4(0.f, gg[0], gg[1], gg[2]);
+                                cA[(j * 3 + k) * DCP_SORT_MAX + pos] = make_float2(gg[0], gg[1]);
+                                cB[(j * 3 + k) * DCP_SORT_MAX + pos] = gg[2];
                             }
                         if (i == 2) g2p_adj_store(sim, I, Gout, P, tid, gx1, gfx);
                     }
                     __syncthreads();
-                    run_sums(L, contrib);
+                    run_sums3(L, cA, cB);
                     __syncthreads();
 #pragma unroll
                     for (int h = 0; h < DCP_NODES; ++h) {
-                        const float4 v = node_sum(L, B, P.count, contrib, nc[h][0], nc[h][1], nc[h][2], i, has[h]);
-                        a[h].y += v.y; a[h].z += v.z; a[h].w += v.w;
+                        const float3 v = node_sum3(L, B, P.count, cA, cB, nc[h][0], nc[h][1], nc[h][2], i, has[h]);
+                        a[h].y += v.x; a[h].z += v.y; a[h].w += v.z;
                     }
                     __syncthreads();
                 }

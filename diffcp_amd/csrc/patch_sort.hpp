@@ -161,6 +161,70 @@ __device__ __forceinline__ float4 node_sum(const BinLDS& L, const PatchBox& B, i
     return acc;
 }
 
+// The same two stages for three-channel contributions (the G2P adjoint's
+// dL/d grid_v_out), kept as a float2 array (channels 0, 1) and a float array
+// (channel 2): 12 B per entry instead of 16, so the workgroup's LDS leaves
+// room for a third workgroup per CU.
+__device__ __forceinline__ void run_sums3(const BinLDS& L, float2* ca, float* cb) {
+    const int ntask = 9 * L.n_runs;
+    for (int t = threadIdx.x; t < ntask; t += DCP_WG) {
+        const int r = t / 9, slot = t - 9 * r;
+        const int base = slot * DCP_SORT_MAX + L.run_start[r];
+        const int len = L.run_len[r];
+        float2* a2 = ca + base;
+        float* b1 = cb + base;
+        float2 a = a2[0];
+        float c = b1[0];
+        int u = 1;
+        for (; u + 7 < len; u += 8) {
+            float2 x[8];
+            float y[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) { x[q] = a2[u + q]; y[q] = b1[u + q]; }
+            a.x += ((x[0].x + x[1].x) + (x[2].x + x[3].x)) + ((x[4].x + x[5].x) + (x[6].x + x[7].x));
+            a.y += ((x[0].y + x[1].y) + (x[2].y + x[3].y)) + ((x[4].y + x[5].y) + (x[6].y + x[7].y));
+            c += ((y[0] + y[1]) + (y[2] + y[3])) + ((y[4] + y[5]) + (y[6] + y[7]));
+        }
+        for (; u + 3 < len; u += 4) {
+            const float2 x0 = a2[u], x1 = a2[u + 1], x2 = a2[u + 2], x3 = a2[u + 3];
+            const float y0 = b1[u], y1 = b1[u + 1], y2 = b1[u + 2], y3 = b1[u + 3];
+            a.x += (x0.x + x1.x) + (x2.x + x3.x);
+            a.y += (x0.y + x1.y) + (x2.y + x3.y);
+            c += (y0 + y1) + (y2 + y3);
+        }
+        for (; u < len; ++u) {
+            const float2 x = a2[u];
+            a.x += x.x; a.y += x.y; c += b1[u];
+        }
+        a2[0] = a;
+        b1[0] = c;
+    }
+}
+
+__device__ __forceinline__ float3 node_sum3(const BinLDS& L, const PatchBox& B, int count, const float2* ca,
+                                            const float* cb, int ci, int cj, int ck, int i, bool has) {
+    float3 acc = make_float3(0.f, 0.f, 0.f);
+    const int bi = ci - i;
+    if (!has || bi < 0 || bi > B.e0 - 3) return acc;
+    int idx[9];
+#pragma unroll
+    for (int o = 0; o < 9; ++o) {
+        const int dj = o / 3, dk = o % 3;
+        const int bj = cj - dj, bk = ck - dk;
+        const bool in = bj >= 0 && bj <= B.e1 - 3 && bk >= 0 && bk <= B.e2 - 3;
+        const int kb = in ? (bi * B.e1 + bj) * B.e2 + bk : 0;
+        const int a = L.off[kb];
+        const int e = kb + 1 < B.vol ? L.off[kb + 1] : count;
+        idx[o] = in && e > a ? o * DCP_SORT_MAX + a : DCP_CONTRIB_ZERO;
+    }
+#pragma unroll
+    for (int o = 0; o < 9; ++o) {
+        const float2 x = ca[idx[o]];
+        acc.x += x.x; acc.y += x.y; acc.z += cb[idx[o]];
+    }
+    return acc;
+}
+
 // Box of the patch's base cells from one candidate per lane.
 __device__ __forceinline__ void box_reduce(int b0, int b1, int b2, bool act, int* box, PatchBox& B) {
     const int big = 1 << 30;
