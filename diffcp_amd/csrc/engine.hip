@@ -313,11 +313,11 @@ static int launch_adjoint(dcp_ctx* c, int s, int g, StateView S, StateView G1, S
     TIMED_LAUNCH(DCP_KERNEL_GRIDOP_ADJ, k_gridop_adj, dim3(c->grid_gridop()), dim3(DCP_WG), 0, c->stream, c->sim);
     TIMED_LAUNCH(DCP_KERNEL_P2G_ADJ, k_p2g_adj, dim3(c->grid_patches()), dim3(DCP_WG), 0, c->stream, c->sim, S, Gout, s, g);
     if (c->opt.precision == 1)
-        TIMED_LAUNCH(DCP_KERNEL_PREP_ADJ, k_prep_adj<float>, dim3(c->grid_patches()), dim3(DCP_WG), 0, c->stream, c->sim, S, G1,
-                     Gout, fresh);
+        TIMED_LAUNCH(DCP_KERNEL_PREP_ADJ, k_elem_adj<float>, dim3(c->grid_patches()), dim3(DCP_WG), 0, c->stream, c->sim, S, G1,
+                     Gout, fresh, s, g);
     else
-        TIMED_LAUNCH(DCP_KERNEL_PREP_ADJ, k_prep_adj<double>, dim3(c->grid_patches()), dim3(DCP_WG), 0, c->stream, c->sim, S, G1,
-                     Gout, fresh);
+        TIMED_LAUNCH(DCP_KERNEL_PREP_ADJ, k_elem_adj<double>, dim3(c->grid_patches()), dim3(DCP_WG), 0, c->stream, c->sim, S, G1,
+                     Gout, fresh, s, g);
     TIMED_LAUNCH(DCP_KERNEL_VERTEX_FINAL, k_vertex_final, dim3(c->grid_patches()), dim3(DCP_WG), 0, c->stream, c->sim, Gout);
     return 0;
 }
@@ -547,8 +547,8 @@ static int build(dcp_ctx* c, const dcp_instance_desc* d, int B) {
     if (c->alloc(&S.acc, (size_t)cells) ||
         c->alloc(&S.vout, (size_t)cells) || c->alloc(&S.gacc, (size_t)cells) || c->alloc(&S.gmv, (size_t)cells) ||
         c->alloc(&S.flags, (size_t)flags) || c->alloc(&S.active, (size_t)flags) || c->alloc(&S.n_active, (size_t)B) ||
-        c->alloc(&S.status, 1) || c->alloc(&S.g_xe, 3 * (size_t)ne) || c->alloc(&S.g_ve, 3 * (size_t)ne) ||
-        c->alloc(&S.g_ne, 3 * (size_t)ne) || c->alloc(&S.g_ke, 3 * (size_t)ne) || c->alloc(&S.g_fv, 3 * (size_t)np) ||
+        c->alloc(&S.status, 1) || c->alloc(&S.g_xe, 3 * (size_t)ne) ||
+        c->alloc(&S.g_fv, 3 * (size_t)np) ||
         c->alloc(&S.ec_gx, 9 * (size_t)ne) || c->alloc(&S.ec_gv, 3 * (size_t)ne) ||
         c->alloc(&S.pgrad_patch, 4 * (size_t)c->n_patches) || c->alloc(&S.pgrad_inst, (size_t)DCP_N_PARAMS * B) ||
         c->alloc(&S.emod_grad, 3 * (size_t)ne) || c->alloc(&S.traj_grad, 3 * (size_t)std::max(1, traj)))

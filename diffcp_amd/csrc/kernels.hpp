@@ -308,7 +308,8 @@ __global__ void k_binning_keys(SimDev sim, StateView S, int* keys);
 __global__ void k_g2p_adj(SimDev sim, StateView S, StateView G1, StateView Gout);
 __global__ void k_gridop_adj(SimDev sim);
 __global__ void k_p2g_adj(SimDev sim, StateView S, StateView Gout, int s, int g);
-template <typename PT> __global__ void k_prep_adj(SimDev sim, StateView S, StateView G1, StateView Gout, int fresh);
+template <typename PT>
+__global__ void k_elem_adj(SimDev sim, StateView S, StateView G1, StateView Gout, int fresh, int s, int g);
 __global__ void k_vertex_final(SimDev sim, StateView Gout);
 __global__ void k_loss_adj(SimDev sim, StateView S, StateView Gout, const float* targets, const int* toff,
                            const int* tcount, const int* targ, const double* seed);

@@ -5,9 +5,12 @@
 //   k_g2p_adj       per patch: dL/d grid_v_out scattered through an LDS tile
 //                   (the same tiling as P2G), dL/dx through the weights
 //   k_gridop_adj    per active block: wall masks, damping, normalisation
-//   k_p2g_adj       per patch: pure 27-cell gather of dL/d(m, mv)
-//   k_prep_adj      per element patch: stress, QR, polar (FP64), kinematics;
-//                   parameter gradients reduced per patch in FP64, fixed order
+//   k_p2g_adj       per vertex / grasper / cube patch: 27-cell gather of
+//                   dL/d(m, mv) over an LDS tile
+//   k_elem_adj      per element patch: the same gather for the element
+//                   particles feeding, in registers, the element prep adjoint
+//                   (stress, QR, polar (FP64), kinematics); parameter
+//                   gradients reduced per patch in FP64, fixed order
 //   k_vertex_final  per vertex: incident-element corner gradients (CSR order)
 #include "kernels.hpp"
 #include "patch_sort.hpp"
@@ -246,15 +249,29 @@ __global__ __launch_bounds__(DCP_WG) void k_gridop_adj(SimDev sim) {
 //   gb += w g,  gA += w g dp^T,  gfx += dW (g_m m + g . (b + A dp)) - w dx A^T g
 //   elements: gk += (dW . n) idx g,  gn += (g . k) idx dW,
 //             gfx += (g . k) idx (d2W n)
-template <typename NodeFetch>
-__device__ __forceinline__ void p2g_adj_particle(const SimDev& sim, const InstDev& I, const StateView& Gout,
-                                                 const Patch& P, int q, int s, int g, const P2GRec& r,
-                                                 const Stencil& st, NodeFetch node) {
+// Results: gx = dL/dx, gb = dL/db / m (b = m v), gA = dL/dA / m (A = m C),
+// gn = dL/dn, gk = dL/dk (the last two for elements only).
+struct P2GAdj {
+    V3<float> gx;
+    float gb[3], gn[3], gk[3];
+    M3<float> gA;
+};
+
+// ELEM: the particle is an element (normal-force terms); known per kernel.
+template <bool ELEM, typename NodeFetch>
+__device__ __forceinline__ void p2g_adj_gather(const InstDev& I, const P2GRec& r, const Stencil& st, NodeFetch node,
+                                               P2GAdj& out) {
         const float hf[3] = {1.0f, -2.0f, 1.0f};   // d2W/dfx2 of the quadratic B-spline nodes
         const float dx = I.dx, idx = I.inv_dx;
-        const bool elem = r.elem;
-        float gb[3] = {0.f, 0.f, 0.f}, gfx[3] = {0.f, 0.f, 0.f}, gn[3] = {0.f, 0.f, 0.f}, gk[3] = {0.f, 0.f, 0.f};
-        M3<float> gA = mzero<float>();
+        constexpr bool elem = ELEM;
+        float* gb = out.gb;
+        float* gn = out.gn;
+        float* gk = out.gk;
+        M3<float>& gA = out.gA;
+        float gfx[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+        for (int a = 0; a < 3; ++a) gb[a] = gn[a] = gk[a] = 0.f;
+        gA = mzero<float>();
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
             // U(f1, f2): .x = mass channel, .yzw = momentum channels
@@ -340,44 +357,75 @@ __device__ __forceinline__ void p2g_adj_particle(const SimDev& sim, const InstDe
             // one's contraction (they would all be live at once)
             __builtin_amdgcn_sched_barrier(0);
         }
-        V3<float> gx = {idx * gfx[0], idx * gfx[1], idx * gfx[2]};
-        if (P.cls == CLS_ELEMENT) {
-            const int e = P.start + q;
-            V3<float> prev = load_v3(sim.g_xe, sim.NE, e);
-            store_v3(sim.g_xe, sim.NE, e, prev + gx);
-            store_v3(sim.g_ve, sim.NE, e, {r.m * gb[0], r.m * gb[1], r.m * gb[2]});
-            store_m9(Gout.eC(0), sim.NE, e, mscale(r.m, gA));
-            store_v3(sim.g_ne, sim.NE, e, {gn[0], gn[1], gn[2]});
-            store_v3(sim.g_ke, sim.NE, e, {gk[0], gk[1], gk[2]});
-        } else {
-            const int p = P.start + q;
-            V3<float> prev = Gout.x(p);
-            store_v3(Gout.px(0), sim.NP, p, prev + gx);
-            store_m9(Gout.pC(0), sim.NP, p, mscale(r.m, gA));
-            if (P.cls == CLS_VERTEX) {
-                store_v3(Gout.pv(0), sim.NP, p, {r.m * gb[0], r.m * gb[1], r.m * gb[2]});
-                store_v3(sim.g_fv, sim.NP, p, {gb[0], gb[1], gb[2]});
-            } else {
-                store_v3(Gout.pv(0), sim.NP, p, {0.f, 0.f, 0.f});
-                if (P.cls == CLS_GRASPER && sim.traj_grad) {
-                    int frame = grasper_frame(I, s, g);
-                    if (frame + 1 < I.n_actions) {
-                        int gid = (p - I.o_start) / I.group_size;
-                        double* t1 = sim.traj_grad + 3 * ((size_t)I.traj_off + (size_t)(frame + 1) * I.n_grasp + gid);
-                        double* t0 = sim.traj_grad + 3 * ((size_t)I.traj_off + (size_t)frame * I.n_grasp + gid);
-                        for (int a = 0; a < 3; ++a) {
-                            double gvp = (double)r.m * (double)gb[a] / (double)I.interval;
-                            atomicAdd(t1 + a, gvp);
-                            atomicAdd(t0 + a, -gvp);
-                        }
-                    }
+        out.gx = {idx * gfx[0], idx * gfx[1], idx * gfx[2]};
+}
+
+// Store the P2G-reverse gradients of a vertex, grasper or cube particle: dL/dx
+// adds to the G2P reverse's, dL/dC and dL/dv (vertices; 0 for the prescribed
+// classes) are written, the vertex force-impulse gradient dL/df = dL/db is
+// kept for the elements' adjoint, and a grasper's dL/dv goes to the two
+// trajectory frames its velocity is the difference of (simulation.py:160-165).
+__device__ __forceinline__ void p2g_adj_store_point(const SimDev& sim, const InstDev& I, const StateView& Gout,
+                                                    const Patch& P, int q, int s, int g, const P2GRec& r,
+                                                    const P2GAdj& o) {
+    const int p = P.start + q;
+    V3<float> prev = Gout.x(p);
+    store_v3(Gout.px(0), sim.NP, p, prev + o.gx);
+    store_m9(Gout.pC(0), sim.NP, p, mscale(r.m, o.gA));
+    if (P.cls == CLS_VERTEX) {
+        store_v3(Gout.pv(0), sim.NP, p, {r.m * o.gb[0], r.m * o.gb[1], r.m * o.gb[2]});
+        store_v3(sim.g_fv, sim.NP, p, {o.gb[0], o.gb[1], o.gb[2]});
+    } else {
+        store_v3(Gout.pv(0), sim.NP, p, {0.f, 0.f, 0.f});
+        if (P.cls == CLS_GRASPER && sim.traj_grad) {
+            int frame = grasper_frame(I, s, g);
+            if (frame + 1 < I.n_actions) {
+                int gid = (p - I.o_start) / I.group_size;
+                double* t1 = sim.traj_grad + 3 * ((size_t)I.traj_off + (size_t)(frame + 1) * I.n_grasp + gid);
+                double* t0 = sim.traj_grad + 3 * ((size_t)I.traj_off + (size_t)frame * I.n_grasp + gid);
+                for (int a = 0; a < 3; ++a) {
+                    double gvp = (double)r.m * (double)o.gb[a] / (double)I.interval;
+                    atomicAdd(t1 + a, gvp);
+                    atomicAdd(t0 + a, -gvp);
                 }
             }
         }
+    }
 }
 
-// P2G adjoint, one particle per lane: dL/d(m, m v) of the patch tile is staged
-// in LDS once per patch; a patch whose tile exceeds the cap reads the grid.
+// One patch's particle records, their base-cell box and the patch tile of
+// dL/d(m, m v) in LDS (when the box fits the tile cap).  Returns whether the
+// tile was staged; the caller's lane holds particle tid when `act`.
+__device__ __forceinline__ bool p2g_adj_stage(const SimDev& sim, const InstDev& I, const StateView& S,
+                                              const Patch& P, int s, int g, bool act, P2GRec& r, Stencil& st,
+                                              int* box, float4* gt, PatchBox& B) {
+    const int tid = threadIdx.x;
+    if (act) {
+        p2g_record(sim, I, S, P, tid, s, g, r);
+        make_stencil(r.x, I.inv_dx, I.G, st, nullptr);
+    } else {
+        st.base[0] = st.base[1] = st.base[2] = 0;
+    }
+    box_reduce(st.base[0], st.base[1], st.base[2], act, box, B);
+    const bool tiled = B.vol <= DCP_STILE_CAP;
+    if (tiled) {
+#pragma unroll
+        for (int h = 0; h < DCP_NODES; ++h) {
+            const int c = tid + h * DCP_WG;
+            if (c < B.vol) {
+                const int ck = c % B.e2, t = c / B.e2;
+                gt[c] = sim.gmv[cell_index(I, t / B.e1 + B.l0, t % B.e1 + B.l1, ck + B.l2)];
+            }
+        }
+    }
+    __syncthreads();
+    return tiled;
+}
+
+// P2G adjoint of the vertex, grasper and cube patches, one particle per lane:
+// dL/d(m, m v) of the patch tile is staged in LDS once per patch; a patch
+// whose tile exceeds the cap reads the grid.  Element patches are left to
+// k_elem_adj, which needs the vertex force-impulse gradients stored here.
 __global__ __launch_bounds__(DCP_WG, 2) void k_p2g_adj(SimDev sim, StateView S, StateView Gout, int s, int g) {
     __shared__ int box[6];
     __shared__ float4 gt[DCP_STILE_CAP];
@@ -386,40 +434,24 @@ __global__ __launch_bounds__(DCP_WG, 2) void k_p2g_adj(SimDev sim, StateView S, 
     __syncthreads();
     for (int pi = blockIdx.x; pi < sim.n_patches; pi += gridDim.x) {
         const Patch P = sim.patches[pi];
+        if (P.cls == CLS_ELEMENT) continue;
         const InstDev I = sim.inst[P.inst];
         const bool act = tid < P.count;   // patches hold at most DCP_WG particles
         P2GRec r;
         Stencil st;
-        if (act) {
-            p2g_record(sim, I, S, P, tid, s, g, r);
-            make_stencil(r.x, I.inv_dx, I.G, st, nullptr);
-        } else {
-            st.base[0] = st.base[1] = st.base[2] = 0;
-        }
         PatchBox B;
-        box_reduce(st.base[0], st.base[1], st.base[2], act, box, B);
-        const bool tiled = B.vol <= DCP_STILE_CAP;
-        if (tiled) {
-#pragma unroll
-            for (int h = 0; h < DCP_NODES; ++h) {
-                const int c = tid + h * DCP_WG;
-                if (c < B.vol) {
-                    const int ck = c % B.e2, t = c / B.e2;
-                    gt[c] = sim.gmv[cell_index(I, t / B.e1 + B.l0, t % B.e1 + B.l1, ck + B.l2)];
-                }
-            }
-        }
-        __syncthreads();
+        const bool tiled = p2g_adj_stage(sim, I, S, P, s, g, act, r, st, box, gt, B);
         if (act) {
+            P2GAdj o;
             if (tiled) {
                 const int key = tile_cell(B, st.base[0], st.base[1], st.base[2]);
-                p2g_adj_particle(sim, I, Gout, P, tid, s, g, r, st,
-                                 [&](int i, int j, int k) { return gt[key + (i * B.e1 + j) * B.e2 + k]; });
+                p2g_adj_gather<false>(I, r, st, [&](int i, int j, int k) { return gt[key + (i * B.e1 + j) * B.e2 + k]; }, o);
             } else {
-                p2g_adj_particle(sim, I, Gout, P, tid, s, g, r, st, [&](int i, int j, int k) {
+                p2g_adj_gather<false>(I, r, st, [&](int i, int j, int k) {
                     return sim.gmv[cell_index(I, st.base[0] + i, st.base[1] + j, st.base[2] + k)];
-                });
+                }, o);
             }
+            p2g_adj_store_point(sim, I, Gout, P, tid, s, g, r, o);
         }
         if (tid < 3) { box[tid] = 1 << 30; box[3 + tid] = -(1 << 30); }
         __syncthreads();
@@ -427,91 +459,134 @@ __global__ __launch_bounds__(DCP_WG, 2) void k_p2g_adj(SimDev sim, StateView S, 
 }
 
 // ---------------------------------------------------------------------------
-// element prep adjoint
+// element adjoint: P2G reverse of the element particles + element prep adjoint
 // ---------------------------------------------------------------------------
+// Adjoint of one element's prep (kinematics, stress, QR, polar; the reverse of
+// k_prep) given the gradients of its prep outputs: dL/dx_e and dL/dv_e (the
+// G2P and P2G reverses), dL/dC_e from the P2G reverse, dL/dn from the P2G
+// reverse plus the next state's normal, dL/dk from the P2G reverse, and the
+// three vertices' force-impulse gradients (stored by k_p2g_adj).  Writes the
+// element's dL/dC and dL/dn of the state, its corner gradients for the vertex
+// gather, and adds its parameter gradients into acc (or the per-element
+// moduli gradients).
 template <typename PT>
-__global__ __launch_bounds__(DCP_WG) void k_prep_adj(SimDev sim, StateView S, StateView G1, StateView Gout, int fresh) {
+__device__ __forceinline__ void elem_adjoint(const SimDev& sim, const InstDev& I, const StateView& S,
+                                             const StateView& G1, const StateView& Gout, int e, int fresh,
+                                             const V3<float>& gxe, const V3<float>& gve, const M3<float>& gC0,
+                                             const V3<float>& gn0, const V3<float>& gkv, double acc[4]) {
+    const int NE = sim.NE;
+    ElemKin K;
+    element_kinematics(sim, I, S, e, fresh, K);
+    ElemParams prm{I.E, I.nu, I.ks, I.kc};
+    if (sim.per_elem) {
+        prm.E = sim.emod[e]; prm.ks = sim.emod[(size_t)NE + e]; prm.kc = sim.emod[(size_t)2 * NE + e];
+    }
+    StressState SS;
+    M3<float> P = stress_forward<PT>(K.F, prm, SS);
+    const float vd = I.vol_quad * I.dt;
+    V3<float> r1 = {K.Dinv(0, 0), K.d(0, 1), K.d(0, 2)};
+    V3<float> r2 = {K.Dinv(1, 0), K.d(1, 1), K.d(1, 2)};
+    V3<float> gf1 = load_v3(sim.g_fv, sim.NP, K.i1);
+    V3<float> gf2 = load_v3(sim.g_fv, sim.NP, K.i2);
+    V3<float> gf3 = load_v3(sim.g_fv, sim.NP, K.i3);
+    M3<float> gP = mzero<float>();
+    gP(0, 2) += -vd * gkv.x; gP(1, 2) += -vd * gkv.y; gP(2, 2) += -vd * gkv.z;
+    gP = madd(gP, mscale(vd, outer(gf1, r1 + r2)));
+    gP = madd(gP, mscale(-vd, outer(gf2, r1)));
+    gP = madd(gP, mscale(-vd, outer(gf3, r2)));
+    V3<float> gr12 = vd * mtv(P, gf1);
+    V3<float> gr1 = gr12 - vd * mtv(P, gf2);
+    V3<float> gr2 = gr12 - vd * mtv(P, gf3);
+    M3<float> gd = mzero<float>();
+    gd(0, 1) += gr1.y; gd(0, 2) += gr1.z;
+    gd(1, 1) += gr2.y; gd(1, 2) += gr2.z;
+    V3<float> gnt = gn0 + G1.n(e);
+    gd(0, 2) += gnt.x; gd(1, 2) += gnt.y; gd(2, 2) += gnt.z;
+    M3<float> gF = mzero<float>();
+    float gp[4] = {0.f, 0.f, 0.f, 0.f};
+    stress_adjoint<PT>(K.F, prm, SS, gP, sim.svd_grad, gF, gp);
+    M3<float> gdn;
+    if (!fresh) {
+        M3<float> Dii = load_m9(sim.Dinvinv, NE, e);
+        gF = madd(gF, mmul(gd, mT(Dii)));     // d = F Dinv^-1
+        gdn = mmul(gF, mT(K.Dinv));           // F = d_new Dinv
+    } else {
+        gdn = gd;                             // F = I is a constant of the fresh state
+    }
+    const float third = 1.0f / 3.0f;
+    V3<float> c0 = {gdn(0, 0), gdn(1, 0), gdn(2, 0)};
+    V3<float> c1 = {gdn(0, 1), gdn(1, 1), gdn(2, 1)};
+    V3<float> gcol3 = {gdn(0, 2), gdn(1, 2), gdn(2, 2)};
+    V3<float> gx1 = third * gxe - c0 - c1;
+    V3<float> gx2 = third * gxe + c0;
+    V3<float> gx3 = third * gxe + c1;
+    // the element velocity of a fresh state is the constant 0 (simulation.py:466)
+    V3<float> gvc = fresh ? V3<float>{0.f, 0.f, 0.f} : third * gve;
+    M3<float> gC = gC0;
+    V3<float> gnp = gcol3;
+    if (!fresh) {
+        gnp = gcol3 + mtv(K.M, gcol3);
+        M3<float> gM = outer(gcol3, K.np);
+        gC = madd(gC, madd(mscale(I.dt, gM), mscale(I.half_dt2, madd(mmul(gM, mT(K.C)), mmul(mT(K.C), gM)))));
+    }
+    store_m9(Gout.eC(0), NE, e, gC);
+    store_v3(Gout.en(0), NE, e, gnp);
+    store_v3(sim.ec_gx, NE, e, gx1);
+    store_v3(sim.ec_gx + (size_t)3 * NE, NE, e, gx2);
+    store_v3(sim.ec_gx + (size_t)6 * NE, NE, e, gx3);
+    store_v3(sim.ec_gv, NE, e, gvc);          // the same for the three corners
+    if (sim.per_elem) {
+        sim.emod_grad[e] += (double)gp[0];
+        sim.emod_grad[(size_t)NE + e] += (double)gp[2];
+        sim.emod_grad[(size_t)2 * NE + e] += (double)gp[3];
+        acc[1] += (double)gp[1];
+    } else {
+        for (int t = 0; t < 4; ++t) acc[t] += (double)gp[t];
+    }
+}
+
+// Element patches, one element per lane: the P2G reverse of the element
+// particle (gather of dL/d(m, m v) over the patch tile) feeds the element's
+// prep adjoint in registers.  Parameter gradients are reduced per patch in
+// FP64 in a fixed order.  Runs after k_p2g_adj (the vertex force-impulse
+// gradients).
+template <typename PT>
+__global__ __launch_bounds__(DCP_WG, 2) void k_elem_adj(SimDev sim, StateView S, StateView G1, StateView Gout,
+                                                        int fresh, int s, int g) {
+    __shared__ int box[6];
+    __shared__ float4 gt[DCP_STILE_CAP];
     __shared__ double red[4][DCP_WG / 64];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    const int NE = sim.NE;
+    if (tid < 3) { box[tid] = 1 << 30; box[3 + tid] = -(1 << 30); }
+    __syncthreads();
     for (int pi = blockIdx.x; pi < sim.n_patches; pi += gridDim.x) {
-        const Patch Pt = sim.patches[pi];
-        if (Pt.cls != CLS_ELEMENT) continue;
-        const InstDev I = sim.inst[Pt.inst];
+        const Patch P = sim.patches[pi];
+        if (P.cls != CLS_ELEMENT) continue;
+        const InstDev I = sim.inst[P.inst];
+        const bool act = tid < P.count;   // patches hold at most DCP_WG particles
+        P2GRec r;
+        Stencil st;
+        PatchBox B;
+        const bool tiled = p2g_adj_stage(sim, I, S, P, s, g, act, r, st, box, gt, B);
         double acc[4] = {0.0, 0.0, 0.0, 0.0};
-        for (int q = tid; q < Pt.count; q += DCP_WG) {
-            const int e = Pt.start + q;
-            ElemKin K;
-            element_kinematics(sim, I, S, e, fresh, K);
-            ElemParams prm{I.E, I.nu, I.ks, I.kc};
-            if (sim.per_elem) {
-                prm.E = sim.emod[e]; prm.ks = sim.emod[(size_t)NE + e]; prm.kc = sim.emod[(size_t)2 * NE + e];
-            }
-            StressState SS;
-            M3<float> P = stress_forward<PT>(K.F, prm, SS);
-            const float vd = I.vol_quad * I.dt;
-            V3<float> r1 = {K.Dinv(0, 0), K.d(0, 1), K.d(0, 2)};
-            V3<float> r2 = {K.Dinv(1, 0), K.d(1, 1), K.d(1, 2)};
-            V3<float> gkv = load_v3(sim.g_ke, NE, e);
-            V3<float> gf1 = load_v3(sim.g_fv, sim.NP, K.i1);
-            V3<float> gf2 = load_v3(sim.g_fv, sim.NP, K.i2);
-            V3<float> gf3 = load_v3(sim.g_fv, sim.NP, K.i3);
-            M3<float> gP = mzero<float>();
-            gP(0, 2) += -vd * gkv.x; gP(1, 2) += -vd * gkv.y; gP(2, 2) += -vd * gkv.z;
-            gP = madd(gP, mscale(vd, outer(gf1, r1 + r2)));
-            gP = madd(gP, mscale(-vd, outer(gf2, r1)));
-            gP = madd(gP, mscale(-vd, outer(gf3, r2)));
-            V3<float> gr12 = vd * mtv(P, gf1);
-            V3<float> gr1 = gr12 - vd * mtv(P, gf2);
-            V3<float> gr2 = gr12 - vd * mtv(P, gf3);
-            M3<float> gd = mzero<float>();
-            gd(0, 1) += gr1.y; gd(0, 2) += gr1.z;
-            gd(1, 1) += gr2.y; gd(1, 2) += gr2.z;
-            V3<float> gnt = load_v3(sim.g_ne, NE, e) + G1.n(e);
-            gd(0, 2) += gnt.x; gd(1, 2) += gnt.y; gd(2, 2) += gnt.z;
-            M3<float> gF = mzero<float>();
-            float gp[4] = {0.f, 0.f, 0.f, 0.f};
-            stress_adjoint<PT>(K.F, prm, SS, gP, sim.svd_grad, gF, gp);
-            M3<float> gdn;
-            if (!fresh) {
-                M3<float> Dii = load_m9(sim.Dinvinv, NE, e);
-                gF = madd(gF, mmul(gd, mT(Dii)));     // d = F Dinv^-1
-                gdn = mmul(gF, mT(K.Dinv));           // F = d_new Dinv
+        if (act) {
+            P2GAdj o;
+            if (tiled) {
+                const int key = tile_cell(B, st.base[0], st.base[1], st.base[2]);
+                p2g_adj_gather<true>(I, r, st, [&](int i, int j, int k) { return gt[key + (i * B.e1 + j) * B.e2 + k]; }, o);
             } else {
-                gdn = gd;                             // F = I is a constant of the fresh state
+                p2g_adj_gather<true>(I, r, st, [&](int i, int j, int k) {
+                    return sim.gmv[cell_index(I, st.base[0] + i, st.base[1] + j, st.base[2] + k)];
+                }, o);
             }
-            V3<float> gxe = load_v3(sim.g_xe, NE, e);
-            V3<float> gve = load_v3(sim.g_ve, NE, e);
-            const float third = 1.0f / 3.0f;
-            V3<float> c0 = {gdn(0, 0), gdn(1, 0), gdn(2, 0)};
-            V3<float> c1 = {gdn(0, 1), gdn(1, 1), gdn(2, 1)};
-            V3<float> gcol3 = {gdn(0, 2), gdn(1, 2), gdn(2, 2)};
-            V3<float> gx1 = third * gxe - c0 - c1;
-            V3<float> gx2 = third * gxe + c0;
-            V3<float> gx3 = third * gxe + c1;
-            // the element velocity of a fresh state is the constant 0 (simulation.py:466)
-            V3<float> gvc = fresh ? V3<float>{0.f, 0.f, 0.f} : third * gve;
-            M3<float> gC = Gout.Ce(e);
-            V3<float> gnp = gcol3;
-            if (!fresh) {
-                gnp = gcol3 + mtv(K.M, gcol3);
-                M3<float> gM = outer(gcol3, K.np);
-                gC = madd(gC, madd(mscale(I.dt, gM), mscale(I.half_dt2, madd(mmul(gM, mT(K.C)), mmul(mT(K.C), gM)))));
-            }
-            store_m9(Gout.eC(0), NE, e, gC);
-            store_v3(Gout.en(0), NE, e, gnp);
-            store_v3(sim.ec_gx, NE, e, gx1);
-            store_v3(sim.ec_gx + (size_t)3 * NE, NE, e, gx2);
-            store_v3(sim.ec_gx + (size_t)6 * NE, NE, e, gx3);
-            store_v3(sim.ec_gv, NE, e, gvc);          // the same for the three corners
-            if (sim.per_elem) {
-                sim.emod_grad[e] += (double)gp[0];
-                sim.emod_grad[(size_t)NE + e] += (double)gp[2];
-                sim.emod_grad[(size_t)2 * NE + e] += (double)gp[3];
-                acc[1] += (double)gp[1];
-            } else {
-                for (int t = 0; t < 4; ++t) acc[t] += (double)gp[t];
-            }
+            // the gather's live values end here: keep the prep adjoint's loads below
+            __builtin_amdgcn_sched_barrier(0);
+            const int e = P.start + tid;
+            const V3<float> gxe = load_v3(sim.g_xe, sim.NE, e) + o.gx;   // G2P reverse + P2G reverse
+            const V3<float> gve = {r.m * o.gb[0], r.m * o.gb[1], r.m * o.gb[2]};
+            const V3<float> gn = {o.gn[0], o.gn[1], o.gn[2]};
+            const V3<float> gk = {o.gk[0], o.gk[1], o.gk[2]};
+            elem_adjoint<PT>(sim, I, S, G1, Gout, e, fresh, gxe, gve, mscale(r.m, o.gA), gn, gk, acc);
         }
         for (int t = 0; t < 4; ++t) {
             double v = wave_sum_d(acc[t]);
@@ -523,6 +598,7 @@ __global__ __launch_bounds__(DCP_WG) void k_prep_adj(SimDev sim, StateView S, St
             for (int w = 0; w < DCP_WG / 64; ++w) v += red[tid][w];
             sim.pgrad_patch[(size_t)pi * 4 + tid] += v;
         }
+        if (tid < 3) { box[tid] = 1 << 30; box[3 + tid] = -(1 << 30); }
         __syncthreads();
     }
 }
@@ -579,5 +655,5 @@ __global__ __launch_bounds__(DCP_WG) void k_loss_adj(SimDev sim, StateView S, St
     }
 }
 
-template __global__ void k_prep_adj<double>(SimDev, StateView, StateView, StateView, int);
-template __global__ void k_prep_adj<float>(SimDev, StateView, StateView, StateView, int);
+template __global__ void k_elem_adj<double>(SimDev, StateView, StateView, StateView, int, int, int);
+template __global__ void k_elem_adj<float>(SimDev, StateView, StateView, StateView, int, int, int);

@@ -72,7 +72,7 @@ struct SimDev {
     int* n_active;                 // [B] active blocks per instance
     unsigned* status;
     // adjoint temporaries
-    float* g_xe; float* g_ve; float* g_ne; float* g_ke;  // [3][NE]
+    float* g_xe;                   // [3][NE] dL/dx_e of the G2P reverse
     float* g_fv;                   // [3][NP] d/d (vertex force impulse)
     float* ec_gx; float* ec_gv;    // [9][NE] per-corner d/dx; [3][NE] d/dv (one for the three corners)
     double* pgrad_patch;           // [n_patches][4]  E, nu, shear, contact (element patches)
