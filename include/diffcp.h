@@ -225,8 +225,8 @@ int dcp_synchronize(dcp_ctx* ctx);
 #define DCP_KERNEL_G2P          4
 #define DCP_KERNEL_G2P_ADJ      5
 #define DCP_KERNEL_GRIDOP_ADJ   6
-#define DCP_KERNEL_P2G_ADJ      7
-#define DCP_KERNEL_PREP_ADJ     8
+#define DCP_KERNEL_P2G_ADJ      7   /* P2G adjoint of the vertex, grasper and cube patches */
+#define DCP_KERNEL_PREP_ADJ     8   /* element patches: P2G adjoint + element prep adjoint */
 #define DCP_KERNEL_VERTEX_FINAL 9
 
 /* Record a pair of HIP events on the context's stream around each launch of
