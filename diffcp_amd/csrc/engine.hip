@@ -288,8 +288,13 @@ static int launch_forward(dcp_ctx* c, int s, int g, StateView S, StateView Sn, i
 // snap >= 0: the grid of substep s was saved in snapshot slot `snap` by the
 // forward; it is restored instead of recomputed (P2G and the grid update exit
 // at once unless the snapshot overflowed), and the element prep outputs the
-// forward left in that slot are used as they are
-static int launch_adjoint(dcp_ctx* c, int s, int g, StateView S, StateView G1, StateView Gout, int snap = -1) {
+// forward left in that slot are used as they are.
+// pending: G1's vertex entries still lack the element corner gradients of the
+// previous adjoint substep (k_g2p_adj adds them); finalize: gather this
+// substep's corner gradients into Gout now (k_vertex_final) instead of leaving
+// them to the next adjoint substep.
+static int launch_adjoint(dcp_ctx* c, int s, int g, StateView S, StateView G1, StateView Gout, int snap = -1,
+                          int pending = 0, bool finalize = true) {
     const int fresh = (s == 0);
     const GridSnap sn = c->snap(snap);
     const bool prep_saved = sn.acc && c->snap_prep;
@@ -309,7 +314,8 @@ static int launch_adjoint(dcp_ctx* c, int s, int g, StateView S, StateView G1, S
         TIMED_LAUNCH(DCP_KERNEL_GRIDOP, k_gridop, dim3(c->grid_gridop()), dim3(DCP_WG), 0, c->stream, c->sim, 1, GridSnap{},
                      nullptr);
     }
-    TIMED_LAUNCH(DCP_KERNEL_G2P_ADJ, k_g2p_adj, dim3(c->grid_patches()), dim3(DCP_WG), 0, c->stream, c->sim, S, G1, Gout);
+    TIMED_LAUNCH(DCP_KERNEL_G2P_ADJ, k_g2p_adj, dim3(c->grid_patches()), dim3(DCP_WG), 0, c->stream, c->sim, S, G1, Gout,
+                 pending);
     TIMED_LAUNCH(DCP_KERNEL_GRIDOP_ADJ, k_gridop_adj, dim3(c->grid_gridop()), dim3(DCP_WG), 0, c->stream, c->sim);
     TIMED_LAUNCH(DCP_KERNEL_P2G_ADJ, k_p2g_adj, dim3(c->grid_patches()), dim3(DCP_WG), 0, c->stream, c->sim, S, Gout, s, g);
     if (c->opt.precision == 1)
@@ -318,7 +324,9 @@ static int launch_adjoint(dcp_ctx* c, int s, int g, StateView S, StateView G1, S
     else
         TIMED_LAUNCH(DCP_KERNEL_PREP_ADJ, k_elem_adj<double>, dim3(c->grid_patches()), dim3(DCP_WG), 0, c->stream, c->sim, S, G1,
                      Gout, fresh, s, g);
-    TIMED_LAUNCH(DCP_KERNEL_VERTEX_FINAL, k_vertex_final, dim3(c->grid_patches()), dim3(DCP_WG), 0, c->stream, c->sim, Gout);
+    if (finalize)
+        TIMED_LAUNCH(DCP_KERNEL_VERTEX_FINAL, k_vertex_final, dim3(c->grid_patches()), dim3(DCP_WG), 0, c->stream, c->sim,
+                     Gout);
     return 0;
 }
 
@@ -1025,7 +1033,11 @@ static int record_backward_segment(dcp_ctx* c, const Plan& p, int cseg, int g) {
         int gin = (p.L - (f + 1)) & 1, gout = (p.L - f) & 1;
         // grids saved: every substep of a resident segment, all but the last one otherwise
         const int snap = grids && (resident || f + 1 < end) ? f - base : -1;
-        if (launch_adjoint(c, f, g, c->state(seg_slot(p, cseg, f)), c->adjv(gin), c->adjv(gout), snap)) return -1;
+        // the window's first adjoint substep reads the loss adjoint (complete);
+        // its last one completes the gradient of the initial state
+        const int pending = f + 1 < p.L;
+        if (launch_adjoint(c, f, g, c->state(seg_slot(p, cseg, f)), c->adjv(gin), c->adjv(gout), snap, pending, f == 0))
+            return -1;
     }
     return 0;
 }

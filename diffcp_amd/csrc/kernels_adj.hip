@@ -24,9 +24,11 @@ __global__ void k_fill_f(float* p, size_t n, float v) {
 // ---------------------------------------------------------------------------
 // adjoint inputs of one particle for the G2P reverse: dL/dv_new (incl. the
 // x_new = x + dt v_new term), dL/dC_new (element: through the damping map) and
-// the direct dL/dx_new
+// the direct dL/dx_new.  `pending`: the vertex entries of G1 still lack their
+// incident elements' corner gradients (the previous adjoint substep left them
+// in ec_gx / ec_gv instead of gathering them), which are added here.
 __device__ __forceinline__ void g2p_adj_inputs(const SimDev& sim, const InstDev& I, const StateView& G1, const Patch& P,
-                                               int q, V3<float>& gv, V3<float>& gx1, M3<float>& gC) {
+                                               int q, int pending, V3<float>& gv, V3<float>& gx1, M3<float>& gC) {
     const float csym = 0.5f * DCP_C_SYM_KEEP;
     if (P.cls == CLS_ELEMENT) {
         M3<float> go = G1.Ce(P.start + q);
@@ -38,6 +40,7 @@ __device__ __forceinline__ void g2p_adj_inputs(const SimDev& sim, const InstDev&
         const int p = P.start + q;
         gx1 = G1.x(p);
         V3<float> g1v = G1.v(p);
+        if (pending && P.cls == CLS_VERTEX) incident_sum<true>(sim, p, sim.ec_gx, sim.ec_gv, gx1, g1v);
         gv = {g1v.x + I.dt * gx1.x, g1v.y + I.dt * gx1.y, g1v.z + I.dt * gx1.z};
         gC = G1.C(p);
     }
@@ -77,13 +80,13 @@ __device__ __forceinline__ void g2p_adj_store(const SimDev& sim, const InstDev& 
 }
 
 __device__ __forceinline__ void g2p_adj_spill(const SimDev& sim, const StateView& S, const StateView& G1,
-                                           const StateView& Gout, const Patch& P, const InstDev& I) {
+                                           const StateView& Gout, const Patch& P, const InstDev& I, int pending) {
     for (int q = threadIdx.x; q < P.count; q += DCP_WG) {
         Stencil st;
         make_stencil(particle_x(sim, S, P, q), I.inv_dx, I.G, st, nullptr);
         V3<float> gv, gx1;
         M3<float> gC;
-        g2p_adj_inputs(sim, I, G1, P, q, gv, gx1, gC);
+        g2p_adj_inputs(sim, I, G1, P, q, pending, gv, gx1, gC);
         float gfx[3] = {0.f, 0.f, 0.f};
         for (int i = 0; i < 3; ++i)
             for (int j = 0; j < 3; ++j)
@@ -104,7 +107,7 @@ __device__ __forceinline__ void g2p_adj_spill(const SimDev& sim, const StateView
 // tile node adds its nine run sums.  dL/dx of each particle accumulates over
 // the three layers and is stored after the last.  grid_v_out of the tile is staged in
 // LDS once per patch, so every node read after that is an LDS read.
-__global__ __launch_bounds__(DCP_WG, 3) void k_g2p_adj(SimDev sim, StateView S, StateView G1, StateView Gout) {
+__global__ __launch_bounds__(DCP_WG, 3) void k_g2p_adj(SimDev sim, StateView S, StateView G1, StateView Gout, int pending) {
     __shared__ BinLDS L;
     __shared__ float2 cA[9 * DCP_SORT_MAX + 1];   // contributions, channels 0 and 1
     __shared__ float cB[9 * DCP_SORT_MAX + 1];    // channel 2
@@ -125,7 +128,7 @@ __global__ __launch_bounds__(DCP_WG, 3) void k_g2p_adj(SimDev sim, StateView S, 
             int b0 = 0, b1 = 0, b2 = 0;
             if (act) {
                 x = particle_x(sim, S, P, tid);
-                g2p_adj_inputs(sim, I, G1, P, tid, gv, gx1, gC);
+                g2p_adj_inputs(sim, I, G1, P, tid, pending, gv, gx1, gC);
                 b0 = stencil_base(x.x, I.inv_dx, I.G);
                 b1 = stencil_base(x.y, I.inv_dx, I.G);
                 b2 = stencil_base(x.z, I.inv_dx, I.G);
@@ -191,7 +194,7 @@ __global__ __launch_bounds__(DCP_WG, 3) void k_g2p_adj(SimDev sim, StateView S, 
                 }
             }
         }
-        if (!tiled) g2p_adj_spill(sim, S, G1, Gout, P, I);
+        if (!tiled) g2p_adj_spill(sim, S, G1, Gout, P, I, pending);
         bin_clear(L);
         if (tid < 3) { L.box[tid] = 1 << 30; L.box[3 + tid] = -(1 << 30); }
         __syncthreads();
