@@ -176,6 +176,10 @@ struct dcp_ctx {
     // graphs
     std::map<std::tuple<int, int, int, int, int>, hipGraphExec_t> graphs;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // second stream (default priority) for an adjoint substep's grid restore
+    // beside the previous substep's tail, and its fork / join events
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     double last_ms = 0.0;
     int last_final_slot = -1;
     // per-launch kernel timing
@@ -293,13 +297,15 @@ static int launch_forward(dcp_ctx* c, int s, int g, StateView S, StateView Sn, i
 // previous adjoint substep (k_g2p_adj adds them); finalize: gather this
 // substep's corner gradients into Gout now (k_vertex_final) instead of leaving
 // them to the next adjoint substep.
-static int launch_adjoint(dcp_ctx* c, int s, int g, StateView S, StateView G1, StateView Gout, int snap = -1,
-                          int pending = 0, bool finalize = true) {
-    const int fresh = (s == 0);
+static bool prep_saved(const dcp_ctx* c, int snap) { return c->snap(snap).acc && c->snap_prep; }
+
+// The adjoint substep's grid: restored from the snapshot (P2G and the grid
+// update then exit at once), or recomputed with the element prep.
+static int launch_adjoint_grid(dcp_ctx* c, int s, int g, StateView S, int snap) {
     const GridSnap sn = c->snap(snap);
-    const bool prep_saved = sn.acc && c->snap_prep;
-    PrepSlot ps(c, prep_saved ? snap : -1);
-    if (!prep_saved && launch_prep(c, S, c->scratch_view(), fresh)) return -1;
+    const bool saved = prep_saved(c, snap);
+    PrepSlot ps(c, saved ? snap : -1);
+    if (!saved && launch_prep(c, S, c->scratch_view(), s == 0)) return -1;
     if (sn.acc) {
         // before the P2G: an overflowed snapshot leaves the active-block counts
         // cleared for it
@@ -314,9 +320,26 @@ static int launch_adjoint(dcp_ctx* c, int s, int g, StateView S, StateView G1, S
         TIMED_LAUNCH(DCP_KERNEL_GRIDOP, k_gridop, dim3(c->grid_gridop()), dim3(DCP_WG), 0, c->stream, c->sim, 1, GridSnap{},
                      nullptr);
     }
+    return 0;
+}
+
+// G2P adjoint and grid-update adjoint (they need the substep's grid).
+static int launch_adjoint_head(dcp_ctx* c, StateView S, StateView G1, StateView Gout, int snap, int pending) {
+    PrepSlot ps(c, prep_saved(c, snap) ? snap : -1);
     TIMED_LAUNCH(DCP_KERNEL_G2P_ADJ, k_g2p_adj, dim3(c->grid_patches()), dim3(DCP_WG), 0, c->stream, c->sim, S, G1, Gout,
                  pending);
     TIMED_LAUNCH(DCP_KERNEL_GRIDOP_ADJ, k_gridop_adj, dim3(c->grid_gridop()), dim3(DCP_WG), 0, c->stream, c->sim);
+    return 0;
+}
+
+// P2G adjoint of the point classes, the fused element adjoint and, with
+// finalize, the vertex gather of the element corner gradients.  These no
+// longer touch the grid accumulators, so the next adjoint substep's grid can
+// be rebuilt beside them.
+static int launch_adjoint_tail(dcp_ctx* c, int s, int g, StateView S, StateView G1, StateView Gout, int snap,
+                               bool finalize) {
+    PrepSlot ps(c, prep_saved(c, snap) ? snap : -1);
+    const int fresh = (s == 0);
     TIMED_LAUNCH(DCP_KERNEL_P2G_ADJ, k_p2g_adj, dim3(c->grid_patches()), dim3(DCP_WG), 0, c->stream, c->sim, S, Gout, s, g);
     if (c->opt.precision == 1)
         TIMED_LAUNCH(DCP_KERNEL_PREP_ADJ, k_elem_adj<float>, dim3(c->grid_patches()), dim3(DCP_WG), 0, c->stream, c->sim, S, G1,
@@ -328,6 +351,13 @@ static int launch_adjoint(dcp_ctx* c, int s, int g, StateView S, StateView G1, S
         TIMED_LAUNCH(DCP_KERNEL_VERTEX_FINAL, k_vertex_final, dim3(c->grid_patches()), dim3(DCP_WG), 0, c->stream, c->sim,
                      Gout);
     return 0;
+}
+
+static int launch_adjoint(dcp_ctx* c, int s, int g, StateView S, StateView G1, StateView Gout, int snap = -1,
+                          int pending = 0, bool finalize = true) {
+    if (launch_adjoint_grid(c, s, g, S, snap)) return -1;
+    if (launch_adjoint_head(c, S, G1, Gout, snap, pending)) return -1;
+    return launch_adjoint_tail(c, s, g, S, G1, Gout, snap, finalize);
 }
 
 static int launch_fill(dcp_ctx* c, float* p, size_t n) {
@@ -569,6 +599,9 @@ static int build(dcp_ctx* c, const dcp_instance_desc* d, int B) {
     HIP_OK(hipMemcpy(c->seed_d, c->ones_h.data(), B * sizeof(double), hipMemcpyHostToDevice));
     HIP_OK(hipEventCreate(&c->ev0));
     HIP_OK(hipEventCreate(&c->ev1));
+    HIP_OK(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
+    HIP_OK(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
+    HIP_OK(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
     return 0;
 }
 
@@ -603,6 +636,9 @@ extern "C" int dcp_destroy(dcp_ctx* c) {
     for (void* p : c->allocs) hipFree(p);
     if (c->ev0) hipEventDestroy(c->ev0);
     if (c->ev1) hipEventDestroy(c->ev1);
+    if (c->ev_fork) hipEventDestroy(c->ev_fork);
+    if (c->ev_join) hipEventDestroy(c->ev_join);
+    if (c->side) hipStreamDestroy(c->side);
     c->drop_timing_events();
     c->free_snapshots();
     if (c->own_stream) hipStreamDestroy(c->stream);
@@ -1029,15 +1065,35 @@ static int record_backward_segment(dcp_ctx* c, const Plan& p, int cseg, int g) {
         if (launch_forward(c, f, g, c->state(seg_slot(p, cseg, f)), c->state(seg_slot(p, cseg, f + 1)),
                            grids ? f - base : -1))
             return -1;
+    // grids saved: every substep of a resident segment, all but the last one otherwise
+    auto snap_of = [&](int f) { return grids && (resident || f + 1 < end) ? f - base : -1; };
+    bool grid_ready = false;   // the grid of substep f was rebuilt on the side stream
     for (int f = end - 1; f >= base; --f) {
         int gin = (p.L - (f + 1)) & 1, gout = (p.L - f) & 1;
-        // grids saved: every substep of a resident segment, all but the last one otherwise
-        const int snap = grids && (resident || f + 1 < end) ? f - base : -1;
+        const int snap = snap_of(f);
+        const StateView S = c->state(seg_slot(p, cseg, f));
         // the window's first adjoint substep reads the loss adjoint (complete);
         // its last one completes the gradient of the initial state
         const int pending = f + 1 < p.L;
-        if (launch_adjoint(c, f, g, c->state(seg_slot(p, cseg, f)), c->adjv(gin), c->adjv(gout), snap, pending, f == 0))
-            return -1;
+        if (!grid_ready && launch_adjoint_grid(c, f, g, S, snap)) return -1;
+        if (launch_adjoint_head(c, S, c->adjv(gin), c->adjv(gout), snap, pending)) return -1;
+        // Substep f-1's grid restore needs only this substep's grid-update
+        // adjoint to have run (it rewrites the accumulators, flags and lists
+        // that one cleared): it goes on the side stream beside this substep's
+        // tail, when both substeps use saved element prep (no shared arrays).
+        grid_ready = f - 1 >= base && prep_saved(c, snap) && prep_saved(c, snap_of(f - 1));
+        if (grid_ready) {
+            HIP_OK(hipEventRecord(c->ev_fork, c->stream));
+            HIP_OK(hipStreamWaitEvent(c->side, c->ev_fork, 0));
+            hipStream_t main_stream = c->stream;
+            c->stream = c->side;
+            const int r = launch_adjoint_grid(c, f - 1, g, c->state(seg_slot(p, cseg, f - 1)), snap_of(f - 1));
+            c->stream = main_stream;
+            if (r) return -1;
+            HIP_OK(hipEventRecord(c->ev_join, c->side));
+        }
+        if (launch_adjoint_tail(c, f, g, S, c->adjv(gin), c->adjv(gout), snap, f == 0)) return -1;
+        if (grid_ready) HIP_OK(hipStreamWaitEvent(c->stream, c->ev_join, 0));
     }
     return 0;
 }

@@ -1,9 +1,12 @@
 """The drop-in module (reference names, Tape) and the torch.autograd.Function
 against the library's fused window call, through the C-ABI.
 
-Tolerance 1e-5 relative: the same kernels run in a different launch order
-(substep by substep vs checkpointed segments); only float-atomic summation
-order differs.
+Tolerance 1e-5 relative on losses, 5e-5 on gradients: the same kernels run
+in a different launch order (substep by substep vs checkpointed segments);
+only float-atomic summation order differs, and it varies from run to run.
+With 16 targets on a small cloth a last-bit change can move one target's
+nearest vertex (the loss takes the first minimum), shifting a gradient by
+about 1e-5 relative in either path.
 """
 import numpy as np
 import pytest
@@ -53,7 +56,7 @@ def test_tape_window_matches_fused_rollout():
     eng.reset(0)
     loss, g, _ = eng.rollout_grad(S, 0, S, use_graph=False)
     assert _rel(sim.loss[None], loss[0]) < 1e-5
-    assert _rel(grads, g[0, [0, 1, 2, 4]]) < 1e-5
+    assert _rel(grads, g[0, [0, 1, 2, 4]]) < 5e-5
 
     # window carry-over (parameter_fitting_functions.py:40-49)
     sim.initial_vertex_x.from_numpy(final)
@@ -87,7 +90,7 @@ def test_autograd_function_scales_unit_seed_gradients():
     eng.reset(0)
     l2, g2, _ = eng.rollout_grad(S, 0, 8, use_graph=False)
     assert _rel(loss.detach().numpy(), l2) < 1e-5
-    assert _rel(params.grad.numpy(), w.numpy()[:, None] * g2) < 1e-5
+    assert _rel(params.grad.numpy(), w.numpy()[:, None] * g2) < 5e-5
 
 
 def test_autotune_returns_a_measured_candidate(tmp_path, monkeypatch):

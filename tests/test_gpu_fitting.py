@@ -1,8 +1,14 @@
 """Windowed fitting on the HIP path: the carry-over of compute_grad against the
 substep-by-substep drop-in module (Tape windows with initial_*.from_numpy +
 initialize_ti_field, parameter_fitting_functions.py:20-49), and a short fit
-that must lower the loss.  Tolerance 1e-5 relative between the two paths
-(the same kernels, only float-atomic summation order differs)."""
+that must lower the loss.
+
+Tolerance between the two paths: 1e-5 relative on the loss, 5e-5 on the
+gradients.  The same kernels run; only float-atomic summation order differs,
+and it varies from run to run.  With 16 targets on a 36-vertex cloth such a
+last-bit change can move one target's nearest vertex (the loss takes the
+first minimum), which shifts the nu gradient by about 1.2e-5 relative in
+either path (measured over repeated runs of both)."""
 import numpy as np
 import pytest
 
@@ -54,7 +60,7 @@ def test_compute_grad_carry_over_matches_tape_windows():
         sim.initialize_ti_field()
     assert abs(loss[0] - np.mean(losses)) <= 1e-5 * abs(np.mean(losses))
     ref = np.mean(gE, axis=0)
-    assert np.abs(grads[0, :2] - ref).max() <= 1e-5 * np.abs(ref).max()
+    assert np.abs(grads[0, :2] - ref).max() <= 5e-5 * np.abs(ref).max()
     s.engine.close()
 
 
