@@ -56,6 +56,7 @@ SIGNATURES = {
     "dcp_set_element_moduli": (c_int, [c_void_p, c_dbl_p]),
     "dcp_set_targets": (c_int, [c_void_p, c_dbl_p, c_int_p]),
     "dcp_reset": (c_int, [c_void_p, c_int, c_dbl_p, c_dbl_p, c_dbl_p]),
+    "dcp_rebase_elements": (c_int, [c_void_p, c_int]),
     "dcp_substep": (c_int, [c_void_p, c_int, c_int, c_int, c_int]),
     "dcp_substep_adjoint": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_int]),
     "dcp_zero_adjoint": (c_int, [c_void_p, c_int]),

@@ -153,6 +153,10 @@ struct dcp_ctx {
     // allows, so that the adjoint reuses the forward's instead of recomputing
     float* prep_work = nullptr;
     float* snap_prep = nullptr;
+    // rest-shape re-base (dcp_rebase_elements): the old-rest-shape F of every
+    // element, used once by the next forward substep's stress
+    float* f_old = nullptr;
+    bool rebase_pending = false;
     void use_prep(int slot) {
         float* b = slot >= 0 && snap_prep ? snap_prep + (size_t)slot * DCP_PREP_FLOATS * NE : prep_work;
         sim.xe = b; sim.ve = b + (size_t)3 * NE; sim.ke = b + (size_t)6 * NE; sim.nd = b + (size_t)9 * NE;
@@ -281,7 +285,16 @@ struct PrepSlot {
 // snapshot slot `snap`
 static int launch_forward(dcp_ctx* c, int s, int g, StateView S, StateView Sn, int snap = -1) {
     PrepSlot ps(c, snap >= 0 && snap < c->snap_slots ? snap : -1);
-    if (launch_prep(c, S, Sn, s == 0)) return -1;
+    if (c->rebase_pending) {
+        // the first forward substep after a re-base takes its stress F from f_old
+        c->sim.f_override = c->f_old;
+        c->rebase_pending = false;
+        const int r = launch_prep(c, S, Sn, s == 0);
+        c->sim.f_override = nullptr;
+        if (r) return -1;
+    } else if (launch_prep(c, S, Sn, s == 0)) {
+        return -1;
+    }
     if (launch_p2g(c, S, s, g, nullptr)) return -1;
     TIMED_LAUNCH(DCP_KERNEL_GRIDOP, k_gridop, dim3(c->grid_gridop()), dim3(DCP_WG), 0, c->stream, c->sim, 0, c->snap(snap),
                  nullptr);
@@ -768,6 +781,17 @@ extern "C" int dcp_reset(dcp_ctx* c, int32_t slot, const double* vx, const doubl
     HIP_OK(hipMemcpyAsync(c->states + (size_t)slot * c->slot_floats, st.data(), st.size() * 4, hipMemcpyHostToDevice,
                           c->stream));
     HIP_OK(hipStreamSynchronize(c->stream));
+    c->rebase_pending = false;
+    return 0;
+}
+
+extern "C" int dcp_rebase_elements(dcp_ctx* c, int32_t slot) {
+    if (!c || !slot_ok(c, slot)) return set_err("dcp_rebase_elements: bad slot");
+    if (!c->f_old && c->alloc(&c->f_old, 9 * (size_t)std::max(1, c->NE))) return -1;
+    hipLaunchKernelGGL(k_rebase, dim3(c->grid_elems()), dim3(DCP_WG), 0, c->stream, c->sim, c->state(slot), c->f_old,
+                       (float*)c->sim.Dinv, (float*)c->sim.Dinvinv);
+    LAUNCH_OK();
+    c->rebase_pending = true;
     return 0;
 }
 
@@ -1168,6 +1192,7 @@ static int ensure_snapshots(dcp_ctx* c, int K) {
 extern "C" int dcp_rollout_grad(dcp_ctx* c, int32_t S, int32_t g, int32_t K, int32_t use_graph, double* loss_out,
                                 double* grads_out) {
     if (!c) return set_err("bad args");
+    if (c->rebase_pending) return set_err("a rest-shape re-base is pending: advance it with dcp_substep");
     Plan p;
     if (plan_rollout(c, S, K, p)) return -1;
     if (ensure_snapshots(c, p.K)) return -1;
@@ -1198,6 +1223,7 @@ extern "C" int dcp_rollout_grad(dcp_ctx* c, int32_t S, int32_t g, int32_t K, int
 
 extern "C" int dcp_rollout(dcp_ctx* c, int32_t S, int32_t g, int32_t use_graph, int32_t* final_slot) {
     if (!c) return set_err("bad args");
+    if (c->rebase_pending) return set_err("a rest-shape re-base is pending: advance it with dcp_substep");
     Plan p;
     if (S < 1) return set_err("n_substeps must be >= 1");
     p.S = S; p.K = S; p.L = -1; p.nseg = 1; p.nck = 1; p.seg0 = 2;

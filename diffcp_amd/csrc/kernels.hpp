@@ -304,6 +304,7 @@ __global__ void k_loss_min(SimDev sim, StateView S, const float* targets, const 
                            const int* vnat, float* tmin, int* targ);
 __global__ void k_loss_sum(const float* tmin, const int* toff, const int* tcount, double* loss, int B);
 __global__ void k_binning_keys(SimDev sim, StateView S, int* keys);
+__global__ void k_rebase(SimDev sim, StateView S, float* f_old, float* Dinv, float* Dinvinv);
 
 __global__ void k_g2p_adj(SimDev sim, StateView S, StateView G1, StateView Gout, int pending);
 __global__ void k_gridop_adj(SimDev sim);

@@ -36,7 +36,11 @@ __global__ __launch_bounds__(DCP_WG) void k_prep(SimDev sim, StateView S, StateV
         V3<float> n = {K.d(0, 2), K.d(1, 2), K.d(2, 2)};
         ElemParams prm = elem_params(sim, I, e);
         StressState SS;
-        M3<float> P = stress_forward<PT>(K.F, prm, SS);
+        // after a rest-shape re-base the first substep's stress keeps the F of
+        // the old rest shape (the reference re-bases d and Dinv but not F,
+        // simulation_without_grad.py:472-486)
+        const M3<float> Fs = sim.f_override ? load_m9(sim.f_override, NE, e) : K.F;
+        M3<float> P = stress_forward<PT>(Fs, prm, SS);
         float vd = I.vol_quad * I.dt;
         V3<float> k = {-vd * P(0, 2), -vd * P(1, 2), -vd * P(2, 2)};
         V3<float> r1 = {K.Dinv(0, 0), K.d(0, 1), K.d(0, 2)};   // simulation.py:247
@@ -449,6 +453,41 @@ __global__ __launch_bounds__(DCP_WG) void k_loss_sum(const float* tmin, const in
         }
         if (threadIdx.x == 0) loss[b] = n > 0 ? red[0] / (double)n : 0.0;
         __syncthreads();
+    }
+}
+
+// Re-derive every element's rest shape from the vertex positions of state S
+// (reset_quadrature_particles, simulation_without_grad.py:472-486): d = [x1 -
+// x0 | x2 - x0 | n] with n the unit normal of the normalised edges (norms with
+// eps 1e-6, as initialize_ti_field) and Dinv = d^-1.  Velocities and affine
+// matrices are kept.  The state carries the normal column before this
+// substep's (I + dt C + dt^2/2 C C) update, which the prep applies; the
+// reference overwrites d after it, so the stored column is (I + M)^-1 n.
+// f_old receives the F each element has under the old rest shape, which the
+// reference still uses for the stress of the next substep.
+__global__ __launch_bounds__(DCP_WG) void k_rebase(SimDev sim, StateView S, float* f_old, float* Dinv, float* Dinvinv) {
+    const int NE = sim.NE;
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < NE; e += gridDim.x * blockDim.x) {
+        const InstDev I = sim.inst[sim.elem_inst[e]];
+        ElemKin K;
+        element_kinematics(sim, I, S, e, 0, K);
+        store_m9(f_old, NE, e, K.F);
+        const V3<float> x0 = S.x(K.i1), x1 = S.x(K.i2), x2 = S.x(K.i3);
+        const V3<float> d1 = x1 - x0, d2 = x2 - x0;
+        const float l1 = sqrtf(d1.x * d1.x + d1.y * d1.y + d1.z * d1.z + 1e-6f);
+        const float l2 = sqrtf(d2.x * d2.x + d2.y * d2.y + d2.z * d2.z + 1e-6f);
+        const V3<float> n1 = {d1.x / l1, d1.y / l1, d1.z / l1}, n2 = {d2.x / l2, d2.y / l2, d2.z / l2};
+        const V3<float> c3 = {n1.y * n2.z - n1.z * n2.y, n1.z * n2.x - n1.x * n2.z, n1.x * n2.y - n1.y * n2.x};
+        const float l3 = sqrtf(c3.x * c3.x + c3.y * c3.y + c3.z * c3.z + 1e-6f);
+        const V3<float> n3 = {c3.x / l3, c3.y / l3, c3.z / l3};
+        M3<float> d;
+        d(0, 0) = d1.x; d(1, 0) = d1.y; d(2, 0) = d1.z;
+        d(0, 1) = d2.x; d(1, 1) = d2.y; d(2, 1) = d2.z;
+        d(0, 2) = n3.x; d(1, 2) = n3.y; d(2, 2) = n3.z;
+        const M3<float> di = minv(d);
+        store_m9(Dinv, NE, e, di);
+        store_m9(Dinvinv, NE, e, minv(di));
+        store_v3(S.en(0), NE, e, mv(minv(madd(meye<float>(), K.M)), n3));
     }
 }
 

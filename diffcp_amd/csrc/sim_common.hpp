@@ -55,6 +55,7 @@ struct SimDev {
     const float* Dinv;             // [9][NE] inverse of the rest d
     const float* Dinvinv;          // [9][NE] inverse(Dinv) (the d <- F @ Dinv^-1 round trip)
     const float* emod;             // [3][NE] per-element (E, shear, contact) or null
+    const float* f_override;       // [9][NE] stress F of the first substep after a rest-shape re-base, or null
     const int* inc_off;            // [NP+1] vertex -> incident (element, corner)
     const int* inc_list;           // [3 NE]   e*3 + corner
     const float* traj;             // trajectory pool [frames*grasp][3]

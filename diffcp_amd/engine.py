@@ -133,6 +133,10 @@ class Engine:
         vx, ox, cx = cat(vertex_x), cat(grasper_x), cat(cube_x)
         _abi.check(self.lib.dcp_reset(self.ctx, slot, _dp(vx), _dp(ox), _dp(cx)))
 
+    def rebase_elements(self, slot: int) -> None:
+        """Element rest shapes from the vertex positions in ``slot`` (reset_quadrature_particles)."""
+        _abi.check(self.lib.dcp_rebase_elements(self.ctx, slot))
+
     # ------------------------------------------------------------------ steps
     def substep(self, s: int, g: int, slot_in: int, slot_out: int) -> None:
         _abi.check(self.lib.dcp_substep(self.ctx, s, g, slot_in, slot_out))

@@ -114,6 +114,16 @@ int dcp_set_targets(dcp_ctx* ctx, const double* targets, const int32_t* counts);
 int dcp_reset(dcp_ctx* ctx, int32_t slot, const double* vertex_x,
               const double* grasper_x, const double* cube_x);
 
+/* Re-derive every element's rest shape from the vertex positions in `slot`
+ * (centroid, d = [x1-x0 | x2-x0 | unit normal], Dinv = d^-1; the slot's normal
+ * column becomes the unit normal), keeping all velocities and affine
+ * matrices.  As in the reference, the next forward substep still computes its
+ * stress from each element's F under the old rest shape.  The rest shape is
+ * context-wide until the next dcp_reset.  Forward-only: dcp_rollout and
+ * dcp_rollout_grad refuse to start while such a substep is pending.
+ * Replaces reset_quadrature_particles (simulation_without_grad.py:472-486). */
+int dcp_rebase_elements(dcp_ctx* ctx, int32_t slot);
+
 /* One forward substep: state s in slot_in -> state s+1 in slot_out, window g.
  * Replaces forward(s, grad_update) (simulation.py:480-488). */
 int dcp_substep(dcp_ctx* ctx, int32_t s, int32_t g, int32_t slot_in, int32_t slot_out);

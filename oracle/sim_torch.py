@@ -265,6 +265,26 @@ def initialize_state(inst, dtype=torch.float64) -> Dict[str, torch.Tensor]:
     return st, static
 
 
+def reset_quadrature_particles(st: Dict[str, torch.Tensor], static):
+    """reset_quadrature_particles, simulation_without_grad.py:472-486: element
+    centroids, d and Dinv re-derived from the current vertex positions (the
+    same normalised-normal construction as initialize_ti_field); F, v and C are
+    kept.  Returns (new state, new static)."""
+    vx, tri = st["vx"], static["tri"]
+    x0, x1, x2 = vx[tri[:, 0]], vx[tri[:, 1]], vx[tri[:, 2]]
+    d1, d2 = x1 - x0, x2 - x0
+    n1 = d1 / tnorm(d1, 1e-6)[:, None]
+    n2 = d2 / tnorm(d2, 1e-6)[:, None]
+    c3 = torch.linalg.cross(n1, n2)
+    d = cols(d1, d2, c3 / tnorm(c3, 1e-6)[:, None])
+    new = dict(st)
+    new["qx"] = (x0 + x1 + x2) / 3
+    new["qd"] = d
+    new_static = dict(static)
+    new_static["Dinv"] = inv3(d)
+    return new, new_static
+
+
 def make_params(inst, dtype=torch.float64, requires_grad=False) -> Dict[str, torch.Tensor]:
     p = {k: torch.tensor(float(getattr(inst, a)), dtype=dtype, requires_grad=requires_grad)
          for k, a in (("E", "E"), ("nu", "nu"), ("ks", "shearing_stiffness"),
