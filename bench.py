@@ -21,6 +21,9 @@ Also reported on the same JSON line:
     around every G2P-adjoint launch of the timed region, against the 8 TB/s
     HBM peak; ``traffic`` comes from the committed rocprofv3 counter summary
     when present;
+  * ``large``: the large single cloth (256x256 vertices, 256^3 grid, 1000
+    substeps, sqrt(S) checkpoints, per-triangle moduli gradients): window
+    time, throughput, G2P-adjoint HBM rate and the device memory held;
   * ``cpu_baseline``: the FP64 CPU restatement in ``oracle/`` timed on the
     host on a bounded sample of the same workload.
 
@@ -156,6 +159,54 @@ def bench_plain(reps: int = 2):
     return dict(rate=rate, ms=wall * 1e3, particles=w.particles, S=S)
 
 
+def bench_large(reps: int = 2):
+    """The large single cloth: 256x256 vertices (130,050 triangles) on a 256^3
+    grid, 1000 substeps fwd+adj with sqrt(S) = 32-substep checkpoints, gradient
+    w.r.t. per-triangle (E, shearing, contact) moduli (seeded lognormal).
+    Targets: the loss-state vertices of a run with every E scaled by 1.6.
+    Reports the window time, the G2P-adjoint launch average and HBM rate, and
+    the device memory the context holds (state slots, grids, snapshots)."""
+    from diffcp_amd import scene
+    from diffcp_amd.engine import Engine
+
+    w = scene.config_c5()
+    S, K = w.substeps, w.checkpoint_every
+    inst = w.instances[0]
+    eng = Engine(w.instances, max_states=Engine.slots_for(S, K), per_element_moduli=True)
+    gt = inst.element_moduli.copy()
+    gt[:, 0] *= 1.6
+    eng.set_element_moduli(gt)
+    eng.reset(0)
+    slot, _ = eng.rollout(S - 2)
+    vx = eng.download_state(slot)["vertex_x"]
+    eng.set_targets([scene.sample_targets(vx, N_TARGETS, w.seed * 100)])
+    cells = 0.5 * (distinct_active_cells(eng, 0) + distinct_active_cells(eng, slot))
+    eng.set_element_moduli(inst.element_moduli)
+    eng.reset(0)
+    eng.set_kernel_timing(ROOFLINE_KERNEL, max_launches=4 * S + 16)
+    eng.rollout_grad(S, 0, K, use_graph=True)
+    eng.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        loss, grads, _ = eng.rollout_grad(S, 0, K, use_graph=True)
+    eng.synchronize()
+    wall = (time.perf_counter() - t0) / reps
+    k_ms, k_n = eng.kernel_timing()
+    ge = eng.element_moduli_grads()
+    dev = eng.device_bytes()
+    status = eng.status()
+    eng.close()
+    avg_ms = k_ms / max(k_n, 1)
+    alg = g2p_adj_algorithmic_bytes(w, int(cells))
+    return {"workload": "256x256-vertex cloth (130,050 triangles), 256^3 grid, 1000 substeps fwd+adj, "
+                        "32-substep checkpoints, per-triangle moduli gradients",
+            "particles": w.particles, "substeps": S, "checkpoint_every": K, "ms_per_window": wall * 1e3,
+            "value": w.particles * S / wall, "unit": "particle-substeps/s",
+            "device_bytes": dev, "g2p_adj_avg_launch_us": avg_ms * 1e3,
+            "g2p_adj_achieved_gbs": alg / (avg_ms * 1e-3) / 1e9 if k_n else None,
+            "finite": bool(np.isfinite(loss).all() and np.isfinite(ge).all()), "status": status}
+
+
 def grad_vs_fd():
     """Gradient of the 32x32 cloth's loss (64^3 grid, 200 substeps) w.r.t. E
     and nu on the GPU vs central finite differences of the FP64 oracle
@@ -218,6 +269,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-large", action="store_true", help="skip the large single-cloth leg")
     ap.add_argument("--eager", action="store_true",
                     help="launch kernels one by one instead of replaying graphs (for counter profiling)")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -282,6 +334,9 @@ def main() -> int:
             out["grad_rel_err_fd_stable_svd_grad"] = g["stable"]
             out["grad_fd_reference"] = ("FP64 oracle central FD, h = 1e-4 relative, 32x32 cloth, 64^3 grid, "
                                         "200 substeps (tests/golden/c1_grad_fd.npz)")
+    if not args.no_large:
+        log("large single cloth (256x256, 256^3 grid, 1000 substeps)")
+        out["large"] = bench_large()
     if not args.no_cpu_baseline:
         log("cpu baseline (oracle restatement on the host)")
         out["cpu_baseline"] = cpu_baseline()

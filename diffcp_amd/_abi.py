@@ -77,6 +77,7 @@ SIGNATURES = {
     "dcp_rollout": (c_int, [c_void_p, c_int, c_int, c_int, c_int_p]),
     "dcp_last_rollout_ms": (c_int, [c_void_p, c_dbl_p]),
     "dcp_sizes": (c_int, [c_void_p, ctypes.POINTER(ctypes.c_int64)]),
+    "dcp_device_bytes": (c_int, [c_void_p, ctypes.POINTER(ctypes.c_int64)]),
     "dcp_synchronize": (c_int, [c_void_p]),
     "dcp_last_rollout_final_slot": (c_int, [c_void_p, c_int_p]),
     "dcp_set_kernel_timing": (c_int, [c_void_p, c_int, c_int]),

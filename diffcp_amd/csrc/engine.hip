@@ -147,6 +147,8 @@ struct dcp_ctx {
     int* snap_cnt = nullptr;
     int* snap_ok = nullptr;
     int snap_cap = 0, snap_slots = 0;
+    size_t snap_bytes = 0;         // device bytes of the snapshots
+    size_t alloc_bytes = 0;        // device bytes of everything else the context allocated
     long long init_blocks = 0;     // 4^3 blocks the initial particle stencils touch
     // element prep outputs (x_e, v_e, k, n, corner forces: DCP_PREP_FLOATS per
     // element): the working block, and one block per snapshot slot when memory
@@ -168,6 +170,7 @@ struct dcp_ctx {
         snap_acc = nullptr; snap_blk = snap_cnt = snap_ok = nullptr;
         snap_prep = nullptr;
         snap_cap = snap_slots = 0;
+        snap_bytes = 0;
     }
     GridSnap snap(int slot) const {
         if (slot < 0 || slot >= snap_slots) return GridSnap{};
@@ -210,6 +213,7 @@ struct dcp_ctx {
         e = hipMemset(q, 0, bytes);
         if (e != hipSuccess) return set_err(std::string("hipMemset: ") + hipGetErrorString(e));
         allocs.push_back(q);
+        alloc_bytes += bytes;
         *p = (T*)q;
         return 0;
     }
@@ -662,6 +666,12 @@ extern "C" int dcp_destroy(dcp_ctx* c) {
 extern "C" int dcp_sizes(dcp_ctx* c, int64_t* o) {
     if (!c || !o) return set_err("bad args");
     o[0] = c->B; o[1] = c->NP; o[2] = c->NE; o[3] = (int64_t)c->slot_floats; o[4] = c->max_states;
+    return 0;
+}
+
+extern "C" int dcp_device_bytes(dcp_ctx* c, int64_t* out) {
+    if (!c || !out) return set_err("bad args");
+    *out = (int64_t)(c->alloc_bytes + c->snap_bytes);
     return 0;
 }
 
@@ -1186,6 +1196,8 @@ static int ensure_snapshots(dcp_ctx* c, int K) {
     }
     c->snap_cap = (int)cap;
     c->snap_slots = K;
+    c->snap_bytes = (size_t)K * cap * (64 * sizeof(float4) + sizeof(int)) + (size_t)K * (c->B + 1) * sizeof(int) +
+                    (c->snap_prep ? prep_b : 0);
     return 0;
 }
 

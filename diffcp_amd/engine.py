@@ -133,6 +133,12 @@ class Engine:
         vx, ox, cx = cat(vertex_x), cat(grasper_x), cat(cube_x)
         _abi.check(self.lib.dcp_reset(self.ctx, slot, _dp(vx), _dp(ox), _dp(cx)))
 
+    def device_bytes(self) -> int:
+        """Device memory held by the context (all buffers, snapshots included)."""
+        out = ctypes.c_int64()
+        _abi.check(self.lib.dcp_device_bytes(self.ctx, ctypes.byref(out)))
+        return int(out.value)
+
     def rebase_elements(self, slot: int) -> None:
         """Element rest shapes from the vertex positions in ``slot`` (reset_quadrature_particles)."""
         _abi.check(self.lib.dcp_rebase_elements(self.ctx, slot))

@@ -330,6 +330,14 @@ def config_c5(seed: int = 4, substeps: int = 1000) -> Workload:
                     per_element_moduli=True, gt_params=(8e3, 0.25))
 
 
+def precision_stress_instance(n: int = 24, n_grid: int = 64, substeps: int = 120, seed: int = 3) -> Instance:
+    """The precision-stress trajectory of config_c4 on a smaller cloth: 40 %
+    in-plane compression, then a fold over the cube (in-plane singular values
+    down to ~0.1 with near-equal pairs)."""
+    return make_instance(n, n_grid, substeps + 2, seed=seed, E=5e3, nu=0.3, compress=0.4, fold=True,
+                         density=HEAVY_DENSITY, lift=0.08, name="c4_small")
+
+
 CONFIGS = {"c1": config_c1, "c2": config_c2, "c3": config_c3, "c4": config_c4, "c5": config_c5}
 
 

@@ -252,6 +252,12 @@ int dcp_kernel_timing(dcp_ctx* ctx, double* total_ms, int32_t* launches);
 /* Sizes: [B, total points, total elements, slot floats, max_states]. */
 int dcp_sizes(dcp_ctx* ctx, int64_t* out5);
 
+/* Device memory the context holds (bytes): every buffer it allocated,
+ * including the grid and element-prep snapshots of dcp_rollout_grad.  No
+ * counterpart in the reference (Taichi's device_memory_GB pool,
+ * initialization.py:19, is a fixed reservation). */
+int dcp_device_bytes(dcp_ctx* ctx, int64_t* out);
+
 #ifdef __cplusplus
 }
 #endif
