@@ -112,13 +112,14 @@ template <typename T> struct DevBuf {
 #define DCP_PREP_FLOATS 21   // x_e, v_e, k, n (3 each) and three corner forces (9) per element
 
 struct dcp_ctx {
-    int B = 0, NP = 0, NE = 0, n_patches = 0, n_cu = 256;
+    int B = 0, NP = 0, NE = 0, n_patches = 0, n_spatches = 0, n_cu = 256;
     dcp_options opt{};
     hipStream_t stream = nullptr;
     bool own_stream = false;
     std::vector<HostInst> hi;
     std::vector<InstDev> inst_h;
     std::vector<Patch> patches_h;
+    std::vector<SPatch> spatches_h;
     std::vector<int> tri_h;            // internal [3][NE]
     std::vector<int> elem_inst_h;
     std::vector<int> vnat_h;           // per point: natural vertex index (vertices) or index within class
@@ -225,6 +226,9 @@ struct dcp_ctx {
     int grid_patches() const {
         return std::max(1, std::min(n_patches, opt.scatter_wgs > 0 ? opt.scatter_wgs : n_cu * 16));
     }
+    int grid_spatches() const {
+        return std::max(1, std::min(n_spatches, opt.scatter_wgs > 0 ? opt.scatter_wgs : n_cu * 16));
+    }
     int grid_elems() const { return std::max(1, std::min((NE + DCP_WG - 1) / DCP_WG, n_cu * 16)); }
     // B x per workgroups walking the per-instance active block lists
     int grid_gridop() const {
@@ -261,10 +265,10 @@ static hipError_t time_mark(dcp_ctx* c) {
 // whenever the grid snapshot holds the substep.
 static int launch_p2g(dcp_ctx* c, StateView S, int s, int g, const int* skip) {
     if (skip) {
-        hipLaunchKernelGGL(k_p2g, dim3(c->grid_patches()), dim3(DCP_WG), 0, c->stream, c->sim, S, s, g, skip);
+        hipLaunchKernelGGL(k_p2g, dim3(c->grid_spatches()), dim3(DCP_WG), 0, c->stream, c->sim, S, s, g, skip);
         LAUNCH_OK();
     } else {
-        TIMED_LAUNCH(DCP_KERNEL_P2G, k_p2g, dim3(c->grid_patches()), dim3(DCP_WG), 0, c->stream, c->sim, S, s, g, skip);
+        TIMED_LAUNCH(DCP_KERNEL_P2G, k_p2g, dim3(c->grid_spatches()), dim3(DCP_WG), 0, c->stream, c->sim, S, s, g, skip);
     }
     return 0;
 }
@@ -343,7 +347,7 @@ static int launch_adjoint_grid(dcp_ctx* c, int s, int g, StateView S, int snap) 
 // G2P adjoint and grid-update adjoint (they need the substep's grid).
 static int launch_adjoint_head(dcp_ctx* c, StateView S, StateView G1, StateView Gout, int snap, int pending) {
     PrepSlot ps(c, prep_saved(c, snap) ? snap : -1);
-    TIMED_LAUNCH(DCP_KERNEL_G2P_ADJ, k_g2p_adj, dim3(c->grid_patches()), dim3(DCP_WG), 0, c->stream, c->sim, S, G1, Gout,
+    TIMED_LAUNCH(DCP_KERNEL_G2P_ADJ, k_g2p_adj, dim3(c->grid_spatches()), dim3(DCP_WG), 0, c->stream, c->sim, S, G1, Gout,
                  pending);
     TIMED_LAUNCH(DCP_KERNEL_GRIDOP_ADJ, k_gridop_adj, dim3(c->grid_gridop()), dim3(DCP_WG), 0, c->stream, c->sim);
     return 0;
@@ -548,6 +552,34 @@ static int build(dcp_ctx* c, const dcp_instance_desc* d, int B) {
         chunk_bins(b, CLS_ELEMENT, I.e_start, c->hi[b].ebin);
     }
     c->n_patches = (int)c->patches_h.size();
+    // scatter patches: per spatial bin, the bin's vertices and elements
+    // together, k patches of at most ps particles taking equal shares of both
+    // runs (consecutive in the bin's Morton order); grasper and cube patches as
+    // above
+    for (int b = 0; b < B; ++b) {
+        const HostInst& H = c->hi[b];
+        const InstDev& I = c->inst_h[b];
+        int vi = 0, ei = 0;
+        while (vi < H.nv || ei < H.ne) {
+            const uint64_t bin = vi < H.nv && (ei >= H.ne || H.vbin[vi] <= H.ebin[ei]) ? H.vbin[vi] : H.ebin[ei];
+            int v1 = vi, e1 = ei;
+            while (v1 < H.nv && H.vbin[v1] == bin) ++v1;
+            while (e1 < H.ne && H.ebin[e1] == bin) ++e1;
+            const int nv = v1 - vi, nel = e1 - ei;
+            const int k = std::max(1, (nv + nel + ps - 3) / (ps - 2));   // shares of floor boundaries stay <= ps
+            for (int j = 0; j < k; ++j) {
+                const int va = vi + (int)((long long)nv * j / k), vb = vi + (int)((long long)nv * (j + 1) / k);
+                const int ea = ei + (int)((long long)nel * j / k), eb = ei + (int)((long long)nel * (j + 1) / k);
+                if (vb - va + eb - ea == 0) continue;
+                c->spatches_h.push_back(SPatch{b, CLS_VERTEX, I.v_start + va, vb - va, CLS_ELEMENT, I.e_start + ea, eb - ea});
+            }
+            vi = v1; ei = e1;
+        }
+    }
+    for (const Patch& P : c->patches_h)
+        if (P.cls == CLS_GRASPER || P.cls == CLS_CUBE)
+            c->spatches_h.push_back(SPatch{P.inst, P.cls, P.start, P.count, CLS_ELEMENT, 0, 0});
+    c->n_spatches = (int)c->spatches_h.size();
     // 4^3 blocks the initial stencils touch (sizes the grid snapshots)
     c->init_blocks = 0;
     for (int b = 0; b < B; ++b) {
@@ -584,12 +616,13 @@ static int build(dcp_ctx* c, const dcp_instance_desc* d, int B) {
     S.max_active = flags;
     c->slot_floats = 15 * (size_t)np + 12 * (size_t)ne;
     c->max_states = std::max(2, c->opt.max_states);
-    Patch* pd; int *trid, *eid, *iod, *ild; float* trd;
-    if (c->upload(&pd, c->patches_h) || c->upload(&trid, c->tri_h) || c->upload(&eid, c->elem_inst_h) ||
+    Patch* pd; SPatch* spd; int *trid, *eid, *iod, *ild; float* trd;
+    if (c->upload(&pd, c->patches_h) || c->upload(&spd, c->spatches_h) || c->upload(&trid, c->tri_h) ||
+        c->upload(&eid, c->elem_inst_h) ||
         c->upload(&iod, inc_off) || c->upload(&ild, inc_list) || c->upload(&trd, trajh) ||
         c->upload(&c->inst_d, c->inst_h) || c->upload(&c->vnat, c->vnat_h))
         return -1;
-    S.inst = c->inst_d; S.patches = pd; S.tri = trid; S.elem_inst = eid; S.inc_off = iod; S.inc_list = ild; S.traj = trd;
+    S.inst = c->inst_d; S.patches = pd; S.spatches = spd; S.n_spatches = c->n_spatches; S.tri = trid; S.elem_inst = eid; S.inc_off = iod; S.inc_list = ild; S.traj = trd;
     float *Dinv, *Dii, *emod;
     if (c->alloc(&Dinv, 9 * (size_t)ne) || c->alloc(&Dii, 9 * (size_t)ne)) return -1;
     S.Dinv = Dinv; S.Dinvinv = Dii;

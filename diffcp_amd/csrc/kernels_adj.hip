@@ -117,18 +117,21 @@ __global__ __launch_bounds__(DCP_WG, 3) void k_g2p_adj(SimDev sim, StateView S, 
     if (tid < 3) { L.box[tid] = 1 << 30; L.box[3 + tid] = -(1 << 30); }
     if (tid == 0) { cA[DCP_CONTRIB_ZERO] = make_float2(0.f, 0.f); cB[DCP_CONTRIB_ZERO] = 0.f; }   // never overwritten
     __syncthreads();
-    for (int pi = blockIdx.x; pi < sim.n_patches; pi += gridDim.x) {
-        const Patch P = sim.patches[pi];
-        const InstDev I = sim.inst[P.inst];
-        bool tiled = P.count <= DCP_WG;
+    for (int pi = blockIdx.x; pi < sim.n_spatches; pi += gridDim.x) {
+        const SPatch SP = sim.spatches[pi];
+        const InstDev I = sim.inst[SP.inst];
+        bool tiled = SP.count() <= DCP_WG;
         if (tiled) {
-            const bool act = tid < P.count;
+            const bool act = tid < SP.count();
             V3<float> x = {0.f, 0.f, 0.f}, gv, gx1;
             M3<float> gC;
             int b0 = 0, b1 = 0, b2 = 0;
+            Patch Pl{SP.inst, 0, 0, 0};
+            int ql = 0;
             if (act) {
-                x = particle_x(sim, S, P, tid);
-                g2p_adj_inputs(sim, I, G1, P, tid, pending, gv, gx1, gC);
+                SP.lane(tid, Pl, ql);
+                x = particle_x(sim, S, Pl, ql);
+                g2p_adj_inputs(sim, I, G1, Pl, ql, pending, gv, gx1, gC);
                 b0 = stencil_base(x.x, I.inv_dx, I.G);
                 b1 = stencil_base(x.y, I.inv_dx, I.G);
                 b2 = stencil_base(x.z, I.inv_dx, I.G);
@@ -172,14 +175,14 @@ __global__ __launch_bounds__(DCP_WG, 3) void k_g2p_adj(SimDev sim, StateView S, 
                                 cA[(j * 3 + k) * DCP_SORT_MAX + pos] = make_float2(gg[0], gg[1]);
                                 cB[(j * 3 + k) * DCP_SORT_MAX + pos] = gg[2];
                             }
-                        if (i == 2) g2p_adj_store(sim, I, Gout, P, tid, gx1, gfx);
+                        if (i == 2) g2p_adj_store(sim, I, Gout, Pl, ql, gx1, gfx);
                     }
                     __syncthreads();
                     run_sums3(L, cA, cB);
                     __syncthreads();
 #pragma unroll
                     for (int h = 0; h < DCP_NODES; ++h) {
-                        const float3 v = node_sum3(L, B, P.count, cA, cB, nc[h][0], nc[h][1], nc[h][2], i, has[h]);
+                        const float3 v = node_sum3(L, B, SP.count(), cA, cB, nc[h][0], nc[h][1], nc[h][2], i, has[h]);
                         a[h].y += v.x; a[h].z += v.y; a[h].w += v.z;
                     }
                     __syncthreads();
@@ -194,7 +197,10 @@ __global__ __launch_bounds__(DCP_WG, 3) void k_g2p_adj(SimDev sim, StateView S, 
                 }
             }
         }
-        if (!tiled) g2p_adj_spill(sim, S, G1, Gout, P, I, pending);
+        if (!tiled) {
+            g2p_adj_spill(sim, S, G1, Gout, SP.part(0), I, pending);
+            g2p_adj_spill(sim, S, G1, Gout, SP.part(1), I, pending);
+        }
         bin_clear(L);
         if (tid < 3) { L.box[tid] = 1 << 30; L.box[3 + tid] = -(1 << 30); }
         __syncthreads();

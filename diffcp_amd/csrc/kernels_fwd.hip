@@ -140,7 +140,8 @@ __device__ __forceinline__ void p2g_mark_blocks(const SimDev& sim, const InstDev
     }
 }
 
-// P2G of one patch, compute-then-gather.  One particle per lane: the lane
+// P2G of one scatter patch (a bin's vertices and elements together, or a
+// grasper or cube chunk), compute-then-gather.  One particle per lane: the lane
 // builds its particle's record in registers, the patch is binned by base cell
 // (patch_bin: LDS-atomic ranks, one packed scan), then three passes over the
 // stencil's x-layer i: (1) each lane computes its particle's nine
@@ -159,16 +160,20 @@ __global__ __launch_bounds__(DCP_WG, 2) void k_p2g(SimDev sim, StateView S, int 
     if (tid < 3) { L.box[tid] = 1 << 30; L.box[3 + tid] = -(1 << 30); }
     if (tid == 0) contrib[DCP_CONTRIB_ZERO] = make_float4(0.f, 0.f, 0.f, 0.f);   // never overwritten
     __syncthreads();
-    for (int pi = blockIdx.x; pi < sim.n_patches; pi += gridDim.x) {
-        const Patch P = sim.patches[pi];
-        const InstDev I = sim.inst[P.inst];
-        bool tiled = P.count <= DCP_WG;
+    for (int pi = blockIdx.x; pi < sim.n_spatches; pi += gridDim.x) {
+        const SPatch SP = sim.spatches[pi];
+        const Patch P = SP.part(0);      // instance and block marks
+        const InstDev I = sim.inst[SP.inst];
+        bool tiled = SP.count() <= DCP_WG;
         if (tiled) {
-            const bool act = tid < P.count;
+            const bool act = tid < SP.count();
             P2GRec r;
             int b0 = 0, b1 = 0, b2 = 0;
             if (act) {
-                p2g_record(sim, I, S, P, tid, s, g, r);
+                Patch Pl;
+                int ql;
+                SP.lane(tid, Pl, ql);
+                p2g_record(sim, I, S, Pl, ql, s, g, r);
                 b0 = stencil_base(r.x.x, I.inv_dx, I.G);
                 b1 = stencil_base(r.x.y, I.inv_dx, I.G);
                 b2 = stencil_base(r.x.z, I.inv_dx, I.G);
@@ -212,7 +217,7 @@ __global__ __launch_bounds__(DCP_WG, 2) void k_p2g(SimDev sim, StateView S, int 
                     __syncthreads();
 #pragma unroll
                     for (int h = 0; h < DCP_NODES; ++h) {
-                        const float4 v = node_sum(L, B, P.count, contrib, nc[h][0], nc[h][1], nc[h][2], i, has[h]);
+                        const float4 v = node_sum(L, B, SP.count(), contrib, nc[h][0], nc[h][1], nc[h][2], i, has[h]);
                         a[h].x += v.x; a[h].y += v.y; a[h].z += v.z; a[h].w += v.w;
                     }
                     __syncthreads();
@@ -226,7 +231,8 @@ __global__ __launch_bounds__(DCP_WG, 2) void k_p2g(SimDev sim, StateView S, int 
         }
         if (!tiled) {
             if (tid == 0) atomicOr(sim.status, (unsigned)DCP_STATUS_TILE_SPILL);
-            p2g_spill(sim, S, P, I, s, g);
+            p2g_spill(sim, S, SP.part(0), I, s, g);
+            p2g_spill(sim, S, SP.part(1), I, s, g);
         }
         // next patch: empty counts and box
         bin_clear(L);

@@ -35,6 +35,27 @@ struct Patch {                     // a chunk of one particle class of one insta
     int inst, cls, start, count;
 };
 
+// A scatter patch: up to two chunks of one instance that share a spatial bin,
+// lanes [0, count_a) taking chunk a and the next count_b lanes chunk b (the
+// vertices and the elements of one bin together, so that one LDS tile sums
+// both classes' contributions before the flush).  count_b = 0: one chunk.
+struct SPatch {
+    int inst, cls_a, start_a, count_a, cls_b, start_b, count_b;
+    __device__ int count() const { return count_a + count_b; }
+    // the chunk and in-chunk index of lane q (q < count())
+    __device__ void lane(int q, Patch& P, int& ql) const {
+        const bool a = q < count_a;
+        P.inst = inst;
+        P.cls = a ? cls_a : cls_b;
+        P.start = a ? start_a : start_b;
+        P.count = a ? count_a : count_b;
+        ql = a ? q : q - count_a;
+    }
+    __device__ Patch part(int which) const {
+        return which == 0 ? Patch{inst, cls_a, start_a, count_a} : Patch{inst, cls_b, start_b, count_b};
+    }
+};
+
 // Compact copy of one substep's grid for the adjoint: the (m, m v) of every
 // active 4^3 block, instances in order, block ids, blocks per instance, and
 // ok = 1 when every instance fitted in `cap` blocks.  acc == nullptr: none.
@@ -51,6 +72,8 @@ struct SimDev {
     int precision, svd_grad, per_elem, max_active;
     const InstDev* inst;
     const Patch* patches;          // point patches then element patches
+    const SPatch* spatches;        // scatter patches (P2G and the G2P adjoint)
+    int n_spatches;
     const int* tri;                // [3][NE] point indices
     const float* Dinv;             // [9][NE] inverse of the rest d
     const float* Dinvinv;          // [9][NE] inverse(Dinv) (the d <- F @ Dinv^-1 round trip)
