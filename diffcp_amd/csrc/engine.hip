@@ -38,28 +38,25 @@ struct HostInst {
     std::vector<uint64_t> vbin, ebin;   // spatial bin of each internal vertex / element
 };
 
-uint32_t spread10(uint32_t v) {
-    v &= 1023u;
-    v = (v | (v << 16)) & 0x030000FFu;
-    v = (v | (v << 8)) & 0x0300F00Fu;
-    v = (v | (v << 4)) & 0x030C30C3u;
-    v = (v | (v << 2)) & 0x09249249u;
-    return v;
-}
-uint32_t morton(float x, float y, float z) {
-    auto q = [](float a) { int t = (int)std::floor(a * 1024.0f); return (uint32_t)std::min(1023, std::max(0, t)); };
-    return spread10(q(x)) | (spread10(q(y)) << 1) | (spread10(q(z)) << 2);
+// Base cell of a position (trunc(x G - 0.5) per axis, as the scatters key it),
+// row-major in 30 bits: i, then j, then k.
+uint32_t base_cell_key(float x, float y, float z, int G) {
+    auto q = [&](float a) { int t = (int)(a * (float)G - 0.5f); return (uint32_t)std::min(1023, std::max(0, t)); };
+    return (q(x) << 20) | (q(y) << 10) | q(z);
 }
 
-// Spatial bin (cubes of `cells` grid cells) in the high bits, Morton code of
-// the position in the low bits: sorting by this key groups particles by bin,
-// and patches never straddle a bin, so a patch's tile stays compact.
+// Spatial bin (cubes of `cells` grid cells) in the high bits, the row-major
+// base cell in the low bits: sorting by this key groups particles by bin
+// (patches never straddle a bin, so a patch's tile stays compact) and orders
+// a bin's particles as the scatters' per-patch binning orders the tile cells,
+// so a particle's sorted position in its patch stays close to its lane and the
+// scatters' LDS stores at sorted positions spread over the banks.
 uint64_t bin_of(float x, float y, float z, int G, int cells) {
     auto b = [&](float a) { int t = (int)std::floor(a * G / cells); return (uint64_t)std::min(127, std::max(0, t)); };
     return (b(x) * 128 + b(y)) * 128 + b(z);
 }
 uint64_t spatial_key(float x, float y, float z, int G, int cells) {
-    return (bin_of(x, y, z, G, cells) << 32) | morton(x, y, z);
+    return (bin_of(x, y, z, G, cells) << 32) | base_cell_key(x, y, z, G);
 }
 
 // Bin side (grid cells) for which one bin of a cloth sheet holds about one
@@ -434,7 +431,7 @@ static int build(dcp_ctx* c, const dcp_instance_desc* d, int B) {
         H.tri.assign(D.triangles, D.triangles + 3 * H.ne);
         for (int t : H.tri)
             if (t < 0 || t >= H.nv) return set_err("triangle index out of range");
-        // spatial (Morton) order of vertices and element centroids
+        // spatial order (bin, then row-major base cell) of vertices and element centroids
         std::vector<float> xc(3 * (size_t)H.ne);
         for (int e = 0; e < H.ne; ++e)
             for (int d = 0; d < 3; ++d)
@@ -554,7 +551,7 @@ static int build(dcp_ctx* c, const dcp_instance_desc* d, int B) {
     c->n_patches = (int)c->patches_h.size();
     // scatter patches: per spatial bin, the bin's vertices and elements
     // together, k patches of at most ps particles taking equal shares of both
-    // runs (consecutive in the bin's Morton order); grasper and cube patches as
+    // runs (consecutive in the bin order); grasper and cube patches as
     // above
     for (int b = 0; b < B; ++b) {
         const HostInst& H = c->hi[b];

@@ -109,8 +109,8 @@ __device__ __forceinline__ void g2p_adj_spill(const SimDev& sim, const StateView
 // LDS once per patch, so every node read after that is an LDS read.
 __global__ __launch_bounds__(DCP_WG, 3) void k_g2p_adj(SimDev sim, StateView S, StateView G1, StateView Gout, int pending) {
     __shared__ BinLDS L;
-    __shared__ float2 cA[9 * DCP_SORT_MAX + 1];   // contributions, channels 0 and 1
-    __shared__ float cB[9 * DCP_SORT_MAX + 1];    // channel 2
+    __shared__ float2 cA[DCP_CONTRIB_ZERO + 1];   // contributions, channels 0 and 1
+    __shared__ float cB[DCP_CONTRIB_ZERO + 1];    // channel 2
     __shared__ float4 vt[DCP_STILE_CAP];        // grid_v_out over the patch tile
     const int tid = threadIdx.x;
     bin_clear(L);
@@ -172,8 +172,8 @@ __global__ __launch_bounds__(DCP_WG, 3) void k_g2p_adj(SimDev sim, StateView S, 
                                 float4 gvo = vt[key + (i * B.e1 + j) * B.e2 + k];
                                 float gg[3];
                                 g2p_adj_node(I, st, i, j, k, gvo, gv, gC, gg, gfx);
-                                cA[(j * 3 + k) * DCP_SORT_MAX + pos] = make_float2(gg[0], gg[1]);
-                                cB[(j * 3 + k) * DCP_SORT_MAX + pos] = gg[2];
+                                cA[(j * 3 + k) * DCP_SLOT_STRIDE + pos] = make_float2(gg[0], gg[1]);
+                                cB[(j * 3 + k) * DCP_SLOT_STRIDE + pos] = gg[2];
                             }
                         if (i == 2) g2p_adj_store(sim, I, Gout, Pl, ql, gx1, gfx);
                     }

@@ -153,7 +153,7 @@ __device__ __forceinline__ void p2g_mark_blocks(const SimDev& sim, const InstDev
 __global__ __launch_bounds__(DCP_WG, 2) void k_p2g(SimDev sim, StateView S, int s, int g, const int* skip) {
     if (skip && *skip) return;   // adjoint substep whose grid is restored from a snapshot
     __shared__ BinLDS L;
-    __shared__ float4 contrib[9 * DCP_SORT_MAX + 1];
+    __shared__ float4 contrib[DCP_CONTRIB_ZERO + 1];
     __shared__ int tb_touched[DCP_TB_CAP];
     const int tid = threadIdx.x;
     bin_clear(L);
@@ -209,7 +209,7 @@ __global__ __launch_bounds__(DCP_WG, 2) void k_p2g(SimDev sim, StateView S, int 
                             for (int k = 0; k < 3; ++k) {
                                 float mm, m0, m1, m2;
                                 p2g_contrib(I, st, r, i, j, k, mm, m0, m1, m2);
-                                contrib[(j * 3 + k) * DCP_SORT_MAX + pos] = make_float4(mm, m0, m1, m2);
+                                contrib[(j * 3 + k) * DCP_SLOT_STRIDE + pos] = make_float4(mm, m0, m1, m2);
                             }
                     }
                     __syncthreads();

@@ -10,6 +10,10 @@
 #include "kernels.hpp"
 
 #define DCP_SORT_MAX 256           // particles per binned patch (one per lane)
+// Stride of one stencil slot in the contribution arrays: one entry of padding
+// puts the nine slots of a run in different LDS banks (run_sums reads them in
+// adjacent lanes; with a stride of exactly 256 entries all nine hit one bank).
+#define DCP_SLOT_STRIDE (DCP_SORT_MAX + 1)
 #define DCP_STILE_CAP 1024         // cells of a patch tile
 #define DCP_NODES (DCP_STILE_CAP / DCP_WG)   // tile nodes per lane
 static_assert(DCP_STILE_CAP <= 4 * DCP_WG, "patch_bin scans four cells per lane");
@@ -99,7 +103,7 @@ __device__ __forceinline__ int patch_bin(int key, const PatchBox& B, BinLDS& L) 
 }
 
 // Two-stage reduction of one stencil x-layer's contributions.  `contrib`
-// holds one float4 per (slot, sorted position) at [slot * DCP_SORT_MAX +
+// holds one float4 per (slot, sorted position) at [slot * DCP_SLOT_STRIDE +
 // position], slot = (j, k) of the node relative to the particle's base cell.
 //
 // Stage 1 (run_sums): one task per (run, slot) sums the run's entries of that
@@ -109,8 +113,8 @@ __device__ __forceinline__ void add4(float4& a, const float4& b) { a.x += b.x; a
 __device__ __forceinline__ void run_sums(const BinLDS& L, float4* contrib) {
     const int ntask = 9 * L.n_runs;
     for (int t = threadIdx.x; t < ntask; t += DCP_WG) {
-        const int r = t / 9, slot = t - 9 * r;
-        float4* c = contrib + slot * DCP_SORT_MAX + L.run_start[r];
+        const int slot = t / L.n_runs, r = t - slot * L.n_runs;
+        float4* c = contrib + slot * DCP_SLOT_STRIDE + L.run_start[r];
         const int len = L.run_len[r];
         float4 a = c[0];
         int u = 1;
@@ -139,7 +143,7 @@ __device__ __forceinline__ void run_sums(const BinLDS& L, float4* contrib) {
 // node - (i, dj, dk), the run sum of slot (dj, dk): the 18 run bounds are
 // loaded together, then nine unconditional loads (an empty or out-of-box cell
 // reads the zero entry at DCP_CONTRIB_ZERO).  `has` = false contributes nothing.
-#define DCP_CONTRIB_ZERO (9 * DCP_SORT_MAX)    // contrib arrays hold 9 * DCP_SORT_MAX + 1 entries
+#define DCP_CONTRIB_ZERO (9 * DCP_SLOT_STRIDE)    // contrib arrays hold DCP_CONTRIB_ZERO + 1 entries
 __device__ __forceinline__ float4 node_sum(const BinLDS& L, const PatchBox& B, int count, const float4* contrib,
                                            int ci, int cj, int ck, int i, bool has) {
     float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -154,7 +158,7 @@ __device__ __forceinline__ float4 node_sum(const BinLDS& L, const PatchBox& B, i
         const int kb = in ? (bi * B.e1 + bj) * B.e2 + bk : 0;
         const int a = L.off[kb];
         const int e = kb + 1 < B.vol ? L.off[kb + 1] : count;
-        idx[o] = in && e > a ? o * DCP_SORT_MAX + a : DCP_CONTRIB_ZERO;
+        idx[o] = in && e > a ? o * DCP_SLOT_STRIDE + a : DCP_CONTRIB_ZERO;
     }
 #pragma unroll
     for (int o = 0; o < 9; ++o) add4(acc, contrib[idx[o]]);
@@ -168,8 +172,8 @@ __device__ __forceinline__ float4 node_sum(const BinLDS& L, const PatchBox& B, i
 __device__ __forceinline__ void run_sums3(const BinLDS& L, float2* ca, float* cb) {
     const int ntask = 9 * L.n_runs;
     for (int t = threadIdx.x; t < ntask; t += DCP_WG) {
-        const int r = t / 9, slot = t - 9 * r;
-        const int base = slot * DCP_SORT_MAX + L.run_start[r];
+        const int slot = t / L.n_runs, r = t - slot * L.n_runs;
+        const int base = slot * DCP_SLOT_STRIDE + L.run_start[r];
         const int len = L.run_len[r];
         float2* a2 = ca + base;
         float* b1 = cb + base;
@@ -215,7 +219,7 @@ __device__ __forceinline__ float3 node_sum3(const BinLDS& L, const PatchBox& B, 
         const int kb = in ? (bi * B.e1 + bj) * B.e2 + bk : 0;
         const int a = L.off[kb];
         const int e = kb + 1 < B.vol ? L.off[kb + 1] : count;
-        idx[o] = in && e > a ? o * DCP_SORT_MAX + a : DCP_CONTRIB_ZERO;
+        idx[o] = in && e > a ? o * DCP_SLOT_STRIDE + a : DCP_CONTRIB_ZERO;
     }
 #pragma unroll
     for (int o = 0; o < 9; ++o) {
