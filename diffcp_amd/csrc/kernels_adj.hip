@@ -141,20 +141,18 @@ __global__ __launch_bounds__(DCP_WG, 3) void k_g2p_adj(SimDev sim, StateView S, 
             tiled = B.vol <= DCP_STILE_CAP;
             if (tiled) {
                 const int pos = patch_bin(act ? tile_cell(B, b0, b1, b2) : -1, B, L);
-                int nc[DCP_NODES][3];
+                // this lane's tile nodes: tid + h * DCP_WG (vol <= DCP_NODES * DCP_WG), and their planes
+                int nci[DCP_NODES];
                 bool has[DCP_NODES];
+                float4 a[DCP_NODES];
 #pragma unroll
                 for (int h = 0; h < DCP_NODES; ++h) {
                     const int c = tid + h * DCP_WG;
                     has[h] = c < B.vol;
                     const int ck = c % B.e2, t = c / B.e2;
-                    nc[h][0] = t / B.e1; nc[h][1] = t % B.e1; nc[h][2] = ck;
-                }
-                float4 a[DCP_NODES];
-#pragma unroll
-                for (int h = 0; h < DCP_NODES; ++h) {
+                    nci[h] = t / B.e1;
                     a[h] = make_float4(0.f, 0.f, 0.f, 0.f);
-                    if (has[h]) vt[tid + h * DCP_WG] = sim.vout[cell_index(I, nc[h][0] + B.l0, nc[h][1] + B.l1, nc[h][2] + B.l2)];
+                    if (has[h]) vt[c] = sim.vout[cell_index(I, nci[h] + B.l0, t - nci[h] * B.e1 + B.l1, ck + B.l2)];
                 }
                 __syncthreads();
                 // tile index of the particle's base cell; node (i, j, k) is key + (i * e1 + j) * e2 + k
@@ -182,7 +180,7 @@ __global__ __launch_bounds__(DCP_WG, 3) void k_g2p_adj(SimDev sim, StateView S, 
                     __syncthreads();
 #pragma unroll
                     for (int h = 0; h < DCP_NODES; ++h) {
-                        const float3 v = node_sum3(L, B, SP.count(), cA, cB, nc[h][0], nc[h][1], nc[h][2], i, has[h]);
+                        const float3 v = node_sum3(L, B, cA, cB, tid + h * DCP_WG, nci[h], i, has[h]);
                         a[h].y += v.x; a[h].z += v.y; a[h].w += v.z;
                     }
                     __syncthreads();
@@ -190,7 +188,8 @@ __global__ __launch_bounds__(DCP_WG, 3) void k_g2p_adj(SimDev sim, StateView S, 
 #pragma unroll
                 for (int h = 0; h < DCP_NODES; ++h) {
                     if (has[h] && (a[h].y != 0.f || a[h].z != 0.f || a[h].w != 0.f)) {
-                        long long gidx = cell_index(I, nc[h][0] + B.l0, nc[h][1] + B.l1, nc[h][2] + B.l2);
+                        const int c = tid + h * DCP_WG, t = c / B.e2;
+                        long long gidx = cell_index(I, nci[h] + B.l0, t - nci[h] * B.e1 + B.l1, c - t * B.e2 + B.l2);
                         atomicAdd(&sim.gacc[gidx].y, a[h].y); atomicAdd(&sim.gacc[gidx].z, a[h].z);
                         atomicAdd(&sim.gacc[gidx].w, a[h].w);
                     }

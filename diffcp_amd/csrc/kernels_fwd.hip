@@ -185,15 +185,14 @@ __global__ __launch_bounds__(DCP_WG, 2) void k_p2g(SimDev sim, StateView S, int 
                 const TileBlocks TB = tile_blocks(B.l0, B.l1, B.l2, B.e0, B.e1, B.e2);
                 for (int t = tid; t < TB.n && t < DCP_TB_CAP; t += DCP_WG) tb_touched[t] = 0;
                 const int pos = patch_bin(act ? tile_cell(B, b0, b1, b2) : -1, B, L);
-                // this lane's tile nodes: tid + h * DCP_WG (vol <= DCP_NODES * DCP_WG)
-                int nc[DCP_NODES][3];
+                // this lane's tile nodes: tid + h * DCP_WG (vol <= DCP_NODES * DCP_WG), and their planes
+                int nci[DCP_NODES];
                 bool has[DCP_NODES];
 #pragma unroll
                 for (int h = 0; h < DCP_NODES; ++h) {
                     const int c = tid + h * DCP_WG;
                     has[h] = c < B.vol;
-                    const int ck = c % B.e2, t = c / B.e2;
-                    nc[h][0] = t / B.e1; nc[h][1] = t % B.e1; nc[h][2] = ck;
+                    nci[h] = c / (B.e1 * B.e2);
                 }
                 float4 a[DCP_NODES];
 #pragma unroll
@@ -217,14 +216,18 @@ __global__ __launch_bounds__(DCP_WG, 2) void k_p2g(SimDev sim, StateView S, int 
                     __syncthreads();
 #pragma unroll
                     for (int h = 0; h < DCP_NODES; ++h) {
-                        const float4 v = node_sum(L, B, SP.count(), contrib, nc[h][0], nc[h][1], nc[h][2], i, has[h]);
+                        const float4 v = node_sum(L, B, contrib, tid + h * DCP_WG, nci[h], i, has[h]);
                         a[h].x += v.x; a[h].y += v.y; a[h].z += v.z; a[h].w += v.w;
                     }
                     __syncthreads();
                 }
 #pragma unroll
                 for (int h = 0; h < DCP_NODES; ++h)
-                    if (has[h]) p2g_flush_node(sim, I, P, B, TB, tb_touched, nc[h][0], nc[h][1], nc[h][2], a[h]);
+                    if (has[h]) {
+                        const int t = (tid + h * DCP_WG) / B.e2;
+                        p2g_flush_node(sim, I, P, B, TB, tb_touched, nci[h], t - nci[h] * B.e1,
+                                       tid + h * DCP_WG - t * B.e2, a[h]);
+                    }
                 __syncthreads();
                 p2g_mark_blocks(sim, I, P, TB, tb_touched);
             }

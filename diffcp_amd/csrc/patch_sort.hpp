@@ -42,31 +42,42 @@ __device__ __forceinline__ int tile_cell(const PatchBox& B, int i, int j, int k)
 // (count, occupied) words giving every cell's offset and every occupied
 // cell's run index, and the run table.  No permutation is stored: a particle's
 // sorted position is offset(cell) + rank.
+//
+// The offset table is indexed by tile cell and framed for the node sums: a
+// margin of DCP_OFF_MARGIN zero entries before cell 0 and the particle count
+// after cell vol - 1, so that cell c's run is [off(c), off(c + 1)) for every
+// c in [-DCP_OFF_MARGIN, vol).
 // ---------------------------------------------------------------------------
+#define DCP_OFF_MARGIN 256         // >= 2 e2 + 2 for every tile (e2 <= DCP_STILE_CAP / 9)
+static_assert(DCP_OFF_MARGIN >= 2 * (DCP_STILE_CAP / 9) + 2, "node_sum reaches 2 e2 + 2 cells before cell 0");
 struct BinLDS {
     int box[6];
     int wsum[DCP_WG / 64];
     int n_runs;
-    int off[DCP_STILE_CAP];                    // counts, then exclusive offsets
+    int off_mem[DCP_OFF_MARGIN + DCP_STILE_CAP + 1];   // margin | counts, then exclusive offsets | count
     unsigned short run_start[DCP_SORT_MAX];    // run (occupied cell) -> first position
     unsigned short run_len[DCP_SORT_MAX];
+    __device__ int* off() { return off_mem + DCP_OFF_MARGIN; }
+    __device__ const int* off() const { return off_mem + DCP_OFF_MARGIN; }
 };
 
-// Zero the cell counts; the box barrier that follows orders this before use.
+// Zero the margin, the cell counts and the count slot; the box barrier that
+// follows orders this before use.
 __device__ __forceinline__ void bin_clear(BinLDS& L) {
-    for (int c = threadIdx.x; c < DCP_STILE_CAP; c += DCP_WG) L.off[c] = 0;
+    for (int c = threadIdx.x; c < DCP_OFF_MARGIN + DCP_STILE_CAP + 1; c += DCP_WG) L.off_mem[c] = 0;
 }
 
 // `key` < 0 for a lane without a particle.  Returns the sorted position.
 __device__ __forceinline__ int patch_bin(int key, const PatchBox& B, BinLDS& L) {
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    const int rank = key >= 0 ? atomicAdd(&L.off[key], 1) : 0;
+    int* off = L.off();
+    const int rank = key >= 0 ? atomicAdd(&off[key], 1) : 0;
     __syncthreads();
     int v[4], sum = 0;
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
         const int c = tid * 4 + t;
-        const int n = c < B.vol ? L.off[c] : 0;
+        const int n = c < B.vol ? off[c] : 0;
         v[t] = n | (n > 0 ? 1 << 16 : 0);
         sum += v[t];
     }
@@ -89,7 +100,7 @@ __device__ __forceinline__ int patch_bin(int key, const PatchBox& B, BinLDS& L) 
         const int c = tid * 4 + t;
         if (c < B.vol) {
             const int n = v[t] & 0xffff;
-            L.off[c] = run & 0xffff;
+            off[c] = run & 0xffff;
             if (n) {
                 L.run_start[run >> 16] = (unsigned short)(run & 0xffff);
                 L.run_len[run >> 16] = (unsigned short)n;
@@ -97,9 +108,12 @@ __device__ __forceinline__ int patch_bin(int key, const PatchBox& B, BinLDS& L) 
         }
         run += v[t];
     }
-    if (tid == 0) L.n_runs = total >> 16;
+    if (tid == 0) {
+        L.n_runs = total >> 16;
+        off[B.vol] = total & 0xffff;
+    }
     __syncthreads();
-    return key >= 0 ? L.off[key] + rank : -1;
+    return key >= 0 ? off[key] + rank : -1;
 }
 
 // Two-stage reduction of one stencil x-layer's contributions.  `contrib`
@@ -139,27 +153,34 @@ __device__ __forceinline__ void run_sums(const BinLDS& L, float4* contrib) {
     }
 }
 
-// Stage 2 (node_sum): the tile node (ci, cj, ck) adds, for its nine base cells
-// node - (i, dj, dk), the run sum of slot (dj, dk): the 18 run bounds are
-// loaded together, then nine unconditional loads (an empty or out-of-box cell
-// reads the zero entry at DCP_CONTRIB_ZERO).  `has` = false contributes nothing.
+// Stage 2 (node_sum): the tile node c = (ci, cj, ck) adds, for its nine base
+// cells c - (i, dj, dk), the run sum of slot (dj, dk).  No bounds tests: the
+// tile's halo cells (the last two planes, rows and columns of the tile) never
+// hold a base cell, so an offset that leaves the base-cell range in j or k
+// lands on a halo cell of the row or plane before, or in the table's margin,
+// and reads as an empty run; only the plane (ci >= i) is tested.  Per row
+// (dj) the four offsets off(c' - 2 .. c' + 1) bound the three cells' runs.  An
+// empty cell reads the zero entry at DCP_CONTRIB_ZERO.
 #define DCP_CONTRIB_ZERO (9 * DCP_SLOT_STRIDE)    // contrib arrays hold DCP_CONTRIB_ZERO + 1 entries
-__device__ __forceinline__ float4 node_sum(const BinLDS& L, const PatchBox& B, int count, const float4* contrib,
-                                           int ci, int cj, int ck, int i, bool has) {
-    float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
-    const int bi = ci - i;
-    if (!has || bi < 0 || bi > B.e0 - 3) return acc;
-    int idx[9];
+__device__ __forceinline__ void node_runs(const BinLDS& L, const PatchBox& B, int c, int i, int idx[9]) {
+    const int* o0 = L.off() + c - i * B.e1 * B.e2;
 #pragma unroll
-    for (int o = 0; o < 9; ++o) {
-        const int dj = o / 3, dk = o % 3;
-        const int bj = cj - dj, bk = ck - dk;
-        const bool in = bj >= 0 && bj <= B.e1 - 3 && bk >= 0 && bk <= B.e2 - 3;
-        const int kb = in ? (bi * B.e1 + bj) * B.e2 + bk : 0;
-        const int a = L.off[kb];
-        const int e = kb + 1 < B.vol ? L.off[kb + 1] : count;
-        idx[o] = in && e > a ? o * DCP_SLOT_STRIDE + a : DCP_CONTRIB_ZERO;
+    for (int dj = 0; dj < 3; ++dj) {
+        const int* r = o0 - dj * B.e2;
+        const int q[4] = {r[-2], r[-1], r[0], r[1]};
+#pragma unroll
+        for (int dk = 0; dk < 3; ++dk) {
+            const int a = q[2 - dk], e = q[3 - dk];
+            idx[dj * 3 + dk] = e > a ? (dj * 3 + dk) * DCP_SLOT_STRIDE + a : DCP_CONTRIB_ZERO;
+        }
     }
+}
+__device__ __forceinline__ float4 node_sum(const BinLDS& L, const PatchBox& B, const float4* contrib, int c, int ci,
+                                           int i, bool has) {
+    float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (!has || ci < i) return acc;
+    int idx[9];
+    node_runs(L, B, c, i, idx);
 #pragma unroll
     for (int o = 0; o < 9; ++o) add4(acc, contrib[idx[o]]);
     return acc;
@@ -205,22 +226,12 @@ __device__ __forceinline__ void run_sums3(const BinLDS& L, float2* ca, float* cb
     }
 }
 
-__device__ __forceinline__ float3 node_sum3(const BinLDS& L, const PatchBox& B, int count, const float2* ca,
-                                            const float* cb, int ci, int cj, int ck, int i, bool has) {
+__device__ __forceinline__ float3 node_sum3(const BinLDS& L, const PatchBox& B, const float2* ca, const float* cb,
+                                            int c, int ci, int i, bool has) {
     float3 acc = make_float3(0.f, 0.f, 0.f);
-    const int bi = ci - i;
-    if (!has || bi < 0 || bi > B.e0 - 3) return acc;
+    if (!has || ci < i) return acc;
     int idx[9];
-#pragma unroll
-    for (int o = 0; o < 9; ++o) {
-        const int dj = o / 3, dk = o % 3;
-        const int bj = cj - dj, bk = ck - dk;
-        const bool in = bj >= 0 && bj <= B.e1 - 3 && bk >= 0 && bk <= B.e2 - 3;
-        const int kb = in ? (bi * B.e1 + bj) * B.e2 + bk : 0;
-        const int a = L.off[kb];
-        const int e = kb + 1 < B.vol ? L.off[kb + 1] : count;
-        idx[o] = in && e > a ? o * DCP_SLOT_STRIDE + a : DCP_CONTRIB_ZERO;
-    }
+    node_runs(L, B, c, i, idx);
 #pragma unroll
     for (int o = 0; o < 9; ++o) {
         const float2 x = ca[idx[o]];
