@@ -181,9 +181,12 @@ __device__ __forceinline__ float4 node_sum(const BinLDS& L, const PatchBox& B, c
     if (!has || ci < i) return acc;
     int idx[9];
     node_runs(L, B, c, i, idx);
+    float4 x[9];
 #pragma unroll
-    for (int o = 0; o < 9; ++o) add4(acc, contrib[idx[o]]);
-    return acc;
+    for (int o = 0; o < 9; ++o) x[o] = contrib[idx[o]];
+    add4(x[0], x[1]); add4(x[2], x[3]); add4(x[4], x[5]); add4(x[6], x[7]);
+    add4(x[0], x[2]); add4(x[4], x[6]); add4(x[0], x[4]); add4(x[0], x[8]);
+    return x[0];
 }
 
 // The same two stages for three-channel contributions (the G2P adjoint's
@@ -232,11 +235,13 @@ __device__ __forceinline__ float3 node_sum3(const BinLDS& L, const PatchBox& B, 
     if (!has || ci < i) return acc;
     int idx[9];
     node_runs(L, B, c, i, idx);
+    float2 x[9];
+    float y[9];
 #pragma unroll
-    for (int o = 0; o < 9; ++o) {
-        const float2 x = ca[idx[o]];
-        acc.x += x.x; acc.y += x.y; acc.z += cb[idx[o]];
-    }
+    for (int o = 0; o < 9; ++o) { x[o] = ca[idx[o]]; y[o] = cb[idx[o]]; }
+    acc.x = (((x[0].x + x[1].x) + (x[2].x + x[3].x)) + ((x[4].x + x[5].x) + (x[6].x + x[7].x))) + x[8].x;
+    acc.y = (((x[0].y + x[1].y) + (x[2].y + x[3].y)) + ((x[4].y + x[5].y) + (x[6].y + x[7].y))) + x[8].y;
+    acc.z = (((y[0] + y[1]) + (y[2] + y[3])) + ((y[4] + y[5]) + (y[6] + y[7]))) + y[8];
     return acc;
 }
 
