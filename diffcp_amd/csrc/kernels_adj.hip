@@ -370,12 +370,12 @@ __device__ __forceinline__ void p2g_adj_gather(const InstDev& I, const P2GRec& r
 
 // Store the P2G-reverse gradients of a vertex, grasper or cube particle: dL/dx
 // adds to the G2P reverse's, dL/dC and dL/dv (vertices; 0 for the prescribed
-// classes) are written, the vertex force-impulse gradient dL/df = dL/db is
-// kept for the elements' adjoint, and a grasper's dL/dv goes to the two
-// trajectory frames its velocity is the difference of (simulation.py:160-165).
+// classes) are written and the vertex force-impulse gradient dL/df = dL/db is
+// kept for the elements' adjoint.  A grasper's dL/dv reaches the two
+// trajectory frames its velocity is the difference of (simulation.py:160-165)
+// through grasper_traj_grad, summed over the wave first.
 __device__ __forceinline__ void p2g_adj_store_point(const SimDev& sim, const InstDev& I, const StateView& Gout,
-                                                    const Patch& P, int q, int s, int g, const P2GRec& r,
-                                                    const P2GAdj& o) {
+                                                    const Patch& P, int q, const P2GRec& r, const P2GAdj& o) {
     const int p = P.start + q;
     V3<float> prev = Gout.x(p);
     store_v3(Gout.px(0), sim.NP, p, prev + o.gx);
@@ -385,18 +385,28 @@ __device__ __forceinline__ void p2g_adj_store_point(const SimDev& sim, const Ins
         store_v3(sim.g_fv, sim.NP, p, {o.gb[0], o.gb[1], o.gb[2]});
     } else {
         store_v3(Gout.pv(0), sim.NP, p, {0.f, 0.f, 0.f});
-        if (P.cls == CLS_GRASPER && sim.traj_grad) {
-            int frame = grasper_frame(I, s, g);
-            if (frame + 1 < I.n_actions) {
-                int gid = (p - I.o_start) / I.group_size;
-                double* t1 = sim.traj_grad + 3 * ((size_t)I.traj_off + (size_t)(frame + 1) * I.n_grasp + gid);
-                double* t0 = sim.traj_grad + 3 * ((size_t)I.traj_off + (size_t)frame * I.n_grasp + gid);
-                for (int a = 0; a < 3; ++a) {
-                    double gvp = (double)r.m * (double)o.gb[a] / (double)I.interval;
-                    atomicAdd(t1 + a, gvp);
-                    atomicAdd(t0 + a, -gvp);
-                }
-            }
+    }
+}
+
+// Trajectory gradient of a grasper patch (all of whose particles belong to one
+// grasp point: patches are cut per grasp point): each lane's dL/dv = m gb
+// enters frame f + 1 with +1/interval and frame f with -1/interval.  Summed
+// over the wave in FP64, one pair of atomics per wave and component (all of a
+// grasp point's particles add to the same six values).  Called by every lane
+// of the workgroup; `act` lanes contribute.
+__device__ __forceinline__ void grasper_traj_grad(const SimDev& sim, const InstDev& I, const Patch& P, int s, int g,
+                                                  bool act, const P2GRec& r, const P2GAdj& o) {
+    const int frame = grasper_frame(I, s, g);
+    if (!sim.traj_grad || frame + 1 >= I.n_actions) return;
+    const int gid = (P.start - I.o_start) / I.group_size;
+    double* t1 = sim.traj_grad + 3 * ((size_t)I.traj_off + (size_t)(frame + 1) * I.n_grasp + gid);
+    double* t0 = sim.traj_grad + 3 * ((size_t)I.traj_off + (size_t)frame * I.n_grasp + gid);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const double gvp = wave_sum_d(act ? (double)r.m * (double)o.gb[a] / (double)I.interval : 0.0);
+        if ((threadIdx.x & 63) == 0 && gvp != 0.0) {
+            atomicAdd(t1 + a, gvp);
+            atomicAdd(t0 + a, -gvp);
         }
     }
 }
@@ -449,8 +459,8 @@ __global__ __launch_bounds__(DCP_WG, 2) void k_p2g_adj(SimDev sim, StateView S, 
         Stencil st;
         PatchBox B;
         const bool tiled = p2g_adj_stage(sim, I, S, P, s, g, act, r, st, box, gt, B);
+        P2GAdj o;
         if (act) {
-            P2GAdj o;
             if (tiled) {
                 const int key = tile_cell(B, st.base[0], st.base[1], st.base[2]);
                 p2g_adj_gather<false>(I, r, st, [&](int i, int j, int k) { return gt[key + (i * B.e1 + j) * B.e2 + k]; }, o);
@@ -459,8 +469,9 @@ __global__ __launch_bounds__(DCP_WG, 2) void k_p2g_adj(SimDev sim, StateView S, 
                     return sim.gmv[cell_index(I, st.base[0] + i, st.base[1] + j, st.base[2] + k)];
                 }, o);
             }
-            p2g_adj_store_point(sim, I, Gout, P, tid, s, g, r, o);
+            p2g_adj_store_point(sim, I, Gout, P, tid, r, o);
         }
+        if (P.cls == CLS_GRASPER) grasper_traj_grad(sim, I, P, s, g, act, r, o);
         if (tid < 3) { box[tid] = 1 << 30; box[3 + tid] = -(1 << 30); }
         __syncthreads();
     }
