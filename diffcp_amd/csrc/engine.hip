@@ -1,6 +1,7 @@
 // Host engine behind the C-ABI (include/diffcp.h): device memory, internal
 // particle order, patches, launches, checkpointed rollouts and hipGraphs.
 #include "kernels.hpp"
+#include "patch_sort.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -36,6 +37,8 @@ struct HostInst {
     std::vector<float> x0v, x0o, x0c;   // creation positions (natural order)
     std::vector<int> tri;     // natural triangles
     std::vector<uint64_t> vbin, ebin;   // spatial bin of each internal vertex / element
+    std::vector<int> cperm, cinv;       // cube particles: internal -> natural, natural -> internal
+    std::vector<uint64_t> cbin;         // spatial bin of each internal cube particle
 };
 
 // Base cell of a position (trunc(x G - 0.5) per axis, as the scatters key it),
@@ -57,6 +60,20 @@ uint64_t bin_of(float x, float y, float z, int G, int cells) {
 }
 uint64_t spatial_key(float x, float y, float z, int G, int cells) {
     return (bin_of(x, y, z, G, cells) << 32) | base_cell_key(x, y, z, G);
+}
+
+// Cells of the scatter tile over a point set: the box of base cells
+// trunc(x G - 0.5) plus the 2-cell stencil reach per axis.
+long long tile_cells(const std::vector<float>& x, int G) {
+    if (x.size() < 3) return 0;
+    int lo[3] = {1 << 30, 1 << 30, 1 << 30}, hi[3] = {-(1 << 30), -(1 << 30), -(1 << 30)};
+    for (size_t i = 0; i + 2 < x.size(); i += 3)
+        for (int d = 0; d < 3; ++d) {
+            const int b = (int)(x[i + d] * (float)G - 0.5f);
+            lo[d] = std::min(lo[d], b);
+            hi[d] = std::max(hi[d], b);
+        }
+    return (long long)(hi[0] - lo[0] + 3) * (hi[1] - lo[1] + 3) * (hi[2] - lo[2] + 3);
 }
 
 // Bin side (grid cells) for which one bin of a cloth sheet holds about one
@@ -457,6 +474,18 @@ static int build(dcp_ctx* c, const dcp_instance_desc* d, int B) {
         for (int i = 0; i < H.ne; ++i) H.einv[H.eperm[i]] = i;
         H.ebin.resize(H.ne);
         for (int i = 0; i < H.ne; ++i) H.ebin[i] = ecode[H.eperm[i]] >> 32;
+        // the cube's particles in the same spatial order: a cube wider than one
+        // patch tile is cut into per-bin patches instead of one patch whose
+        // tile would overflow
+        std::vector<uint64_t> ccode(H.nc);
+        for (int i = 0; i < H.nc; ++i) ccode[i] = spatial_key(H.x0c[3 * i], H.x0c[3 * i + 1], H.x0c[3 * i + 2], H.G, bcells);
+        H.cperm.resize(H.nc);
+        std::iota(H.cperm.begin(), H.cperm.end(), 0);
+        std::stable_sort(H.cperm.begin(), H.cperm.end(), [&](int a, int bb) { return ccode[a] < ccode[bb]; });
+        H.cinv.resize(H.nc);
+        for (int i = 0; i < H.nc; ++i) H.cinv[H.cperm[i]] = i;
+        H.cbin.resize(H.nc);
+        for (int i = 0; i < H.nc; ++i) H.cbin[i] = ccode[H.cperm[i]] >> 32;
 
         InstDev& I = c->inst_h[b];
         memset(&I, 0, sizeof(I));
@@ -520,7 +549,7 @@ static int build(dcp_ctx* c, const dcp_instance_desc* d, int B) {
         const InstDev& I = c->inst_h[b];
         for (int i = 0; i < H.nv; ++i) c->vnat_h[I.v_start + i] = H.vperm[i];
         for (int i = 0; i < H.no; ++i) c->vnat_h[I.o_start + i] = i;
-        for (int i = 0; i < H.nc; ++i) c->vnat_h[I.c_start + i] = i;
+        for (int i = 0; i < H.nc; ++i) c->vnat_h[I.c_start + i] = H.cperm[i];
     }
     // patches: point patches (vertex, grasper, cube) then element patches
     // particles per patch: the scatters stage up to 256 particle records in LDS
@@ -542,7 +571,9 @@ static int build(dcp_ctx* c, const dcp_instance_desc* d, int B) {
             if (I.n_grasp == 0) { g0 = 0; g1 = I.o_count; }
             if (g1 > g0) chunk(b, CLS_GRASPER, I.o_start + g0, g1 - g0);
         }
-        chunk(b, CLS_CUBE, I.c_start, I.c_count);
+        // one cube patch when its tile fits (the cube moves rigidly), else one per bin
+        if (tile_cells(c->hi[b].x0c, c->hi[b].G) <= DCP_STILE_CAP) chunk(b, CLS_CUBE, I.c_start, I.c_count);
+        else chunk_bins(b, CLS_CUBE, I.c_start, c->hi[b].cbin);
     }
     for (int b = 0; b < B; ++b) {
         const InstDev& I = c->inst_h[b];
@@ -792,7 +823,7 @@ extern "C" int dcp_reset(dcp_ctx* c, int32_t slot, const double* vx, const doubl
         for (int i = 0; i < H.no; ++i)
             for (int k = 0; k < 3; ++k) st[(size_t)k * NP + I.o_start + i] = xo[3 * i + k];
         for (int i = 0; i < H.nc; ++i)
-            for (int k = 0; k < 3; ++k) st[(size_t)k * NP + I.c_start + i] = xc[3 * i + k];
+            for (int k = 0; k < 3; ++k) st[(size_t)k * NP + I.c_start + H.cinv[i]] = xc[3 * i + k];
         for (int ei = 0; ei < H.ne; ++ei) {
             int en = H.eperm[ei], e = I.e_start + ei;
             const float* a = &xv[3 * H.tri[3 * en]];
@@ -959,7 +990,7 @@ static void slot_to_host(const dcp_ctx* c, const std::vector<float>& h, const Ho
         const InstDev& I = c->inst_h[b];
         for (int i = 0; i < H.nv; ++i) put_point(I.v_start + H.vinv[i], nv + i, A.a[0], A.a[1], A.a[2]);
         for (int i = 0; i < H.no; ++i) put_point(I.o_start + i, no + i, A.a[3], A.a[4], A.a[5]);
-        for (int i = 0; i < H.nc; ++i) put_point(I.c_start + i, nc + i, A.a[6], A.a[7], A.a[8]);
+        for (int i = 0; i < H.nc; ++i) put_point(I.c_start + H.cinv[i], nc + i, A.a[6], A.a[7], A.a[8]);
         for (int en0 = 0; en0 < H.ne; ++en0) {
             int e = I.e_start + H.einv[en0];
             if (A.a[9]) for (int k = 0; k < 9; ++k) A.a[9][9 * (ne + en0) + k] = h[15 * (size_t)NP + (size_t)k * NE + e];
@@ -984,7 +1015,7 @@ static void host_to_slot(const dcp_ctx* c, std::vector<float>& h, const HostArra
         const InstDev& I = c->inst_h[b];
         for (int i = 0; i < H.nv; ++i) get_point(I.v_start + H.vinv[i], nv + i, A.a[0], A.a[1], A.a[2]);
         for (int i = 0; i < H.no; ++i) get_point(I.o_start + i, no + i, A.a[3], A.a[4], A.a[5]);
-        for (int i = 0; i < H.nc; ++i) get_point(I.c_start + i, nc + i, A.a[6], A.a[7], A.a[8]);
+        for (int i = 0; i < H.nc; ++i) get_point(I.c_start + H.cinv[i], nc + i, A.a[6], A.a[7], A.a[8]);
         for (int en0 = 0; en0 < H.ne; ++en0) {
             int e = I.e_start + H.einv[en0];
             if (A.a[9]) for (int k = 0; k < 9; ++k) h[15 * (size_t)NP + (size_t)k * NE + e] = A.a[9][9 * (ne + en0) + k];
@@ -1053,7 +1084,7 @@ extern "C" int dcp_binning_keys(dcp_ctx* c, int32_t slot, int32_t* out) {
         const InstDev& I = c->inst_h[b];
         for (int i = 0; i < H.nv; ++i, ++o) for (int k = 0; k < 3; ++k) out[3 * o + k] = h[3 * (size_t)(I.v_start + H.vinv[i]) + k];
         for (int i = 0; i < H.no; ++i, ++o) for (int k = 0; k < 3; ++k) out[3 * o + k] = h[3 * (size_t)(I.o_start + i) + k];
-        for (int i = 0; i < H.nc; ++i, ++o) for (int k = 0; k < 3; ++k) out[3 * o + k] = h[3 * (size_t)(I.c_start + i) + k];
+        for (int i = 0; i < H.nc; ++i, ++o) for (int k = 0; k < 3; ++k) out[3 * o + k] = h[3 * (size_t)(I.c_start + H.cinv[i]) + k];
         for (int en = 0; en < H.ne; ++en, ++o)
             for (int k = 0; k < 3; ++k) out[3 * o + k] = h[3 * (size_t)(c->NP + I.e_start + H.einv[en]) + k];
     }

@@ -104,12 +104,15 @@ __device__ __forceinline__ void p2g_spill(const SimDev& sim, const StateView& S,
                 for (int k = 0; k < 3; ++k) {
                     float mm, m0, m1, m2;
                     p2g_contrib(I, st, r, i, j, k, mm, m0, m1, m2);
-                    int ci = st.base[0] + i, cj = st.base[1] + j, ck = st.base[2] + k;
-                    long long gi = cell_index(I, ci, cj, ck);
+                    long long gi = cell_index(I, st.base[0] + i, st.base[1] + j, st.base[2] + k);
                     atomicAdd(&sim.acc[gi].x, mm); atomicAdd(&sim.acc[gi].y, m0);
                     atomicAdd(&sim.acc[gi].z, m1); atomicAdd(&sim.acc[gi].w, m2);
-                    mark_block(sim, I, P.inst, ci, cj, ck);
                 }
+        // the 4^3 blocks of the 27 nodes: one or two per axis
+        for (int bx = st.base[0] >> 2; bx <= (st.base[0] + 2) >> 2; ++bx)
+            for (int by = st.base[1] >> 2; by <= (st.base[1] + 2) >> 2; ++by)
+                for (int bz = st.base[2] >> 2; bz <= (st.base[2] + 2) >> 2; ++bz)
+                    mark_block(sim, I, P.inst, bx * 4, by * 4, bz * 4);
     }
 }
 
