@@ -296,7 +296,8 @@ __device__ __forceinline__ void grid_velocity(const InstDev& I, float4 c, int i,
 
 // kernels -------------------------------------------------------------------
 template <typename PT> __global__ void k_prep(SimDev sim, StateView S, StateView Snext, int fresh);
-__global__ void k_p2g(SimDev sim, StateView S, int s, int g, const int* skip);
+template <bool DEFER> __global__ void k_p2g(SimDev sim, StateView S, int s, int g, const int* skip);
+__global__ void k_p2g_big(SimDev sim, StateView S, int s, int g, const int* skip);
 __global__ void k_gridop(SimDev sim, int keep, GridSnap save, const int* skip);
 __global__ void k_grid_restore(SimDev sim, GridSnap snap);
 __global__ void k_g2p(SimDev sim, StateView S, StateView Sn);
@@ -306,7 +307,8 @@ __global__ void k_loss_sum(const float* tmin, const int* toff, const int* tcount
 __global__ void k_binning_keys(SimDev sim, StateView S, int* keys);
 __global__ void k_rebase(SimDev sim, StateView S, float* f_old, float* Dinv, float* Dinvinv);
 
-__global__ void k_g2p_adj(SimDev sim, StateView S, StateView G1, StateView Gout, int pending);
+template <bool DEFER> __global__ void k_g2p_adj(SimDev sim, StateView S, StateView G1, StateView Gout, int pending);
+__global__ void k_g2p_adj_big(SimDev sim, StateView S, StateView G1, StateView Gout, int pending);
 __global__ void k_gridop_adj(SimDev sim);
 __global__ void k_p2g_adj(SimDev sim, StateView S, StateView Gout, int s, int g);
 template <typename PT>

@@ -107,6 +107,88 @@ __device__ __forceinline__ void g2p_adj_spill(const SimDev& sim, const StateView
 // tile node adds its nine run sums.  dL/dx of each particle accumulates over
 // the three layers and is stored after the last.  grid_v_out of the tile is staged in
 // LDS once per patch, so every node read after that is an LDS read.
+// G2P reverse, through one LDS tile over box B, of the patch particles whose
+// base cells lie in B (`act` lanes; B is the patch's box, or one x-slab of it
+// in k_g2p_adj_big): node terms, dL/dx stores, and the flush of dL/d grid_v_out.
+__device__ __forceinline__ void g2p_adj_tile(const SimDev& sim, const InstDev& I, const StateView& Gout,
+                                             const PatchBox& B, bool act, const V3<float>& x, const V3<float>& gv,
+                                             const V3<float>& gx1, const M3<float>& gC, const Patch& Pl, int ql,
+                                             int b0, int b1, int b2, BinLDS& L, float2* cA, float* cB, float4* vt) {
+    const int tid = threadIdx.x;
+    const int pos = patch_bin(act ? tile_cell(B, b0, b1, b2) : -1, B, L);
+    // this lane's tile nodes: tid + h * DCP_WG (vol <= DCP_NODES * DCP_WG), and their planes
+    int nci[DCP_NODES];
+    bool has[DCP_NODES];
+    float4 a[DCP_NODES];
+#pragma unroll
+    for (int h = 0; h < DCP_NODES; ++h) {
+        const int c = tid + h * DCP_WG;
+        has[h] = c < B.vol;
+        const int ck = c % B.e2, t = c / B.e2;
+        nci[h] = t / B.e1;
+        a[h] = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (has[h]) vt[c] = sim.vout[cell_index(I, nci[h] + B.l0, t - nci[h] * B.e1 + B.l1, ck + B.l2)];
+    }
+    __syncthreads();
+    // tile index of the particle's base cell; node (i, j, k) is key + (i * e1 + j) * e2 + k
+    const int key = act ? tile_cell(B, b0, b1, b2) : 0;
+    float gfx[3] = {0.f, 0.f, 0.f};   // dL/dfx, summed over the three layers
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        if (act) {
+            Stencil st;
+            make_stencil(x, I.inv_dx, I.G, st, nullptr);
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    float4 gvo = vt[key + (i * B.e1 + j) * B.e2 + k];
+                    float gg[3];
+                    g2p_adj_node(I, st, i, j, k, gvo, gv, gC, gg, gfx);
+                    cA[(j * 3 + k) * DCP_SLOT_STRIDE + pos] = make_float2(gg[0], gg[1]);
+                    cB[(j * 3 + k) * DCP_SLOT_STRIDE + pos] = gg[2];
+                }
+            if (i == 2) g2p_adj_store(sim, I, Gout, Pl, ql, gx1, gfx);
+        }
+        __syncthreads();
+        run_sums3(L, cA, cB);
+        __syncthreads();
+#pragma unroll
+        for (int h = 0; h < DCP_NODES; ++h) {
+            const float3 v = node_sum3(L, B, cA, cB, tid + h * DCP_WG, nci[h], i, has[h]);
+            a[h].y += v.x; a[h].z += v.y; a[h].w += v.z;
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int h = 0; h < DCP_NODES; ++h) {
+        if (has[h] && (a[h].y != 0.f || a[h].z != 0.f || a[h].w != 0.f)) {
+            const int c = tid + h * DCP_WG, t = c / B.e2;
+            long long gidx = cell_index(I, nci[h] + B.l0, t - nci[h] * B.e1 + B.l1, c - t * B.e2 + B.l2);
+            atomicAdd(&sim.gacc[gidx].y, a[h].y); atomicAdd(&sim.gacc[gidx].z, a[h].z);
+            atomicAdd(&sim.gacc[gidx].w, a[h].w);
+        }
+    }
+}
+
+// Lane q's inputs of a scatter patch: position, G2P-reverse inputs, base cell.
+__device__ __forceinline__ void g2p_adj_patch_inputs(const SimDev& sim, const InstDev& I, const StateView& S,
+                                                     const StateView& G1, const SPatch& SP, int q, int pending,
+                                                     Patch& Pl, int& ql, V3<float>& x, V3<float>& gv,
+                                                     V3<float>& gx1, M3<float>& gC, int& b0, int& b1, int& b2) {
+    SP.lane(q, Pl, ql);
+    x = particle_x(sim, S, Pl, ql);
+    g2p_adj_inputs(sim, I, G1, Pl, ql, pending, gv, gx1, gC);
+    b0 = stencil_base(x.x, I.inv_dx, I.G);
+    b1 = stencil_base(x.y, I.inv_dx, I.G);
+    b2 = stencil_base(x.z, I.inv_dx, I.G);
+}
+
+// DEFER: as k_p2g's (an overflowing patch is listed for k_g2p_adj_big, or
+// reversed here per particle with global atomics).  The tile reverse is
+// written out here rather than called as g2p_adj_tile (k_g2p_adj_big's): the
+// shared form compiled to a schedule measured 3 % slower on this kernel.
+template <bool DEFER>
 __global__ __launch_bounds__(DCP_WG, 3) void k_g2p_adj(SimDev sim, StateView S, StateView G1, StateView Gout, int pending) {
     __shared__ BinLDS L;
     __shared__ float2 cA[DCP_CONTRIB_ZERO + 1];   // contributions, channels 0 and 1
@@ -197,6 +279,64 @@ __global__ __launch_bounds__(DCP_WG, 3) void k_g2p_adj(SimDev sim, StateView S, 
             }
         }
         if (!tiled) {
+            if (DEFER) {   // left to k_g2p_adj_big
+                if (tid == 0) sim.big_list[sim.n_spatches + atomicAdd(&sim.big_n[DCP_BIG_G2P_ADJ], 1)] = pi;
+            } else {
+                if (tid == 0) sim.big_n[DCP_BIG_SEEN] = 1;
+                g2p_adj_spill(sim, S, G1, Gout, SP.part(0), I, pending);
+                g2p_adj_spill(sim, S, G1, Gout, SP.part(1), I, pending);
+            }
+        }
+        bin_clear(L);
+        if (tid < 3) { L.box[tid] = 1 << 30; L.box[3 + tid] = -(1 << 30); }
+        __syncthreads();
+    }
+}
+
+// The patches k_g2p_adj left: the same tile reverse over x-slabs of the patch
+// box (as k_p2g_big), else per particle with global atomics.  k_gridop_adj,
+// next on the stream, empties the list.
+__global__ __launch_bounds__(DCP_WG, 2) void k_g2p_adj_big(SimDev sim, StateView S, StateView G1, StateView Gout,
+                                                           int pending) {
+    const int n_big = sim.big_n[DCP_BIG_G2P_ADJ];
+    if ((int)blockIdx.x >= n_big) return;   // the usual case: an empty list
+    __shared__ BinLDS L;
+    __shared__ float2 cA[DCP_CONTRIB_ZERO + 1];
+    __shared__ float cB[DCP_CONTRIB_ZERO + 1];
+    __shared__ float4 vt[DCP_STILE_CAP];
+    const int tid = threadIdx.x;
+    bin_clear(L);
+    if (tid < 3) { L.box[tid] = 1 << 30; L.box[3 + tid] = -(1 << 30); }
+    if (tid == 0) { cA[DCP_CONTRIB_ZERO] = make_float2(0.f, 0.f); cB[DCP_CONTRIB_ZERO] = 0.f; }
+    __syncthreads();
+    for (int bi = blockIdx.x; bi < n_big; bi += gridDim.x) {
+        const int pi = sim.big_list[sim.n_spatches + bi];
+        const SPatch SP = sim.spatches[pi];
+        const InstDev I = sim.inst[SP.inst];
+        bool tiled = SP.count() <= DCP_WG;
+        if (tiled) {
+            const bool act = tid < SP.count();
+            V3<float> x = {0.f, 0.f, 0.f}, gv, gx1;
+            M3<float> gC;
+            int b0 = 0, b1 = 0, b2 = 0;
+            Patch Pl{SP.inst, 0, 0, 0};
+            int ql = 0;
+            if (act) g2p_adj_patch_inputs(sim, I, S, G1, SP, tid, pending, Pl, ql, x, gv, gx1, gC, b0, b1, b2);
+            PatchBox PB;
+            box_reduce(b0, b1, b2, act, L.box, PB);
+            const SlabPlan SPL = slab_plan(PB);
+            tiled = SPL.w >= 1;
+            for (int x0 = 0; tiled && x0 < SPL.planes; x0 += SPL.w) {
+                const PatchBox B = slab_box(PB, x0, SPL);
+                if (x0 > 0) {   // the previous slab's bins and tile are read
+                    bin_clear(L);
+                    __syncthreads();
+                }
+                g2p_adj_tile(sim, I, Gout, B, act && b0 >= B.l0 && b0 < B.l0 + B.e0 - 2, x, gv, gx1, gC, Pl, ql,
+                             b0, b1, b2, L, cA, cB, vt);
+            }
+        }
+        if (!tiled) {
             g2p_adj_spill(sim, S, G1, Gout, SP.part(0), I, pending);
             g2p_adj_spill(sim, S, G1, Gout, SP.part(1), I, pending);
         }
@@ -210,6 +350,8 @@ __global__ __launch_bounds__(DCP_WG, 3) void k_g2p_adj(SimDev sim, StateView S, 
 // grid update adjoint
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(DCP_WG) void k_gridop_adj(SimDev sim) {
+    // k_g2p_adj_big, just before on the stream, has read the overflow list
+    if (blockIdx.x == 0 && threadIdx.x == 0) sim.big_n[DCP_BIG_G2P_ADJ] = 0;
     const int lane = threadIdx.x & 63;
     ActiveWalk w;
     if (!active_walk(sim, sim.inst, w)) return;
@@ -676,3 +818,5 @@ __global__ __launch_bounds__(DCP_WG) void k_loss_adj(SimDev sim, StateView S, St
 
 template __global__ void k_elem_adj<double>(SimDev, StateView, StateView, StateView, int, int, int);
 template __global__ void k_elem_adj<float>(SimDev, StateView, StateView, StateView, int, int, int);
+template __global__ void k_g2p_adj<false>(SimDev, StateView, StateView, StateView, int);
+template __global__ void k_g2p_adj<true>(SimDev, StateView, StateView, StateView, int);

@@ -153,6 +153,82 @@ __device__ __forceinline__ void p2g_mark_blocks(const SimDev& sim, const InstDev
 // owning tile node n adds the nine run sums of the base cells n - (i, j, k).
 // Node sums stay in registers across the passes and are flushed once into the
 // instance grid with float atomics (tiles of neighbouring patches overlap).
+// Lane q's particle record of a scatter patch and its base cell.
+__device__ __forceinline__ void p2g_patch_record(const SimDev& sim, const InstDev& I, const StateView& S,
+                                                 const SPatch& SP, int q, int s, int g, P2GRec& r, int& b0, int& b1,
+                                                 int& b2) {
+    Patch Pl;
+    int ql;
+    SP.lane(q, Pl, ql);
+    p2g_record(sim, I, S, Pl, ql, s, g, r);
+    b0 = stencil_base(r.x.x, I.inv_dx, I.G);
+    b1 = stencil_base(r.x.y, I.inv_dx, I.G);
+    b2 = stencil_base(r.x.z, I.inv_dx, I.G);
+}
+
+// Scatter, through one LDS tile over box B, of the patch particles whose base
+// cells lie in B (`act` lanes; B is the patch's box, or one x-slab of it in
+// k_p2g_big).
+__device__ __forceinline__ void p2g_tile(const SimDev& sim, const InstDev& I, const Patch& P, const PatchBox& B,
+                                         bool act, const P2GRec& r, int b0, int b1, int b2, BinLDS& L,
+                                         float4* contrib, int* tb_touched) {
+    const int tid = threadIdx.x;
+    const TileBlocks TB = tile_blocks(B.l0, B.l1, B.l2, B.e0, B.e1, B.e2);
+    for (int t = tid; t < TB.n && t < DCP_TB_CAP; t += DCP_WG) tb_touched[t] = 0;
+    const int pos = patch_bin(act ? tile_cell(B, b0, b1, b2) : -1, B, L);
+    // this lane's tile nodes: tid + h * DCP_WG (vol <= DCP_NODES * DCP_WG), and their planes
+    int nci[DCP_NODES];
+    bool has[DCP_NODES];
+#pragma unroll
+    for (int h = 0; h < DCP_NODES; ++h) {
+        const int c = tid + h * DCP_WG;
+        has[h] = c < B.vol;
+        nci[h] = c / (B.e1 * B.e2);
+    }
+    float4 a[DCP_NODES];
+#pragma unroll
+    for (int h = 0; h < DCP_NODES; ++h) a[h] = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        if (act) {
+            Stencil st;
+            make_stencil(r.x, I.inv_dx, I.G, st, nullptr);
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    float mm, m0, m1, m2;
+                    p2g_contrib(I, st, r, i, j, k, mm, m0, m1, m2);
+                    contrib[(j * 3 + k) * DCP_SLOT_STRIDE + pos] = make_float4(mm, m0, m1, m2);
+                }
+        }
+        __syncthreads();
+        run_sums(L, contrib);
+        __syncthreads();
+#pragma unroll
+        for (int h = 0; h < DCP_NODES; ++h) {
+            const float4 v = node_sum(L, B, contrib, tid + h * DCP_WG, nci[h], i, has[h]);
+            a[h].x += v.x; a[h].y += v.y; a[h].z += v.z; a[h].w += v.w;
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int h = 0; h < DCP_NODES; ++h)
+        if (has[h]) {
+            const int t = (tid + h * DCP_WG) / B.e2;
+            p2g_flush_node(sim, I, P, B, TB, tb_touched, nci[h], t - nci[h] * B.e1,
+                           tid + h * DCP_WG - t * B.e2, a[h]);
+        }
+    __syncthreads();
+    p2g_mark_blocks(sim, I, P, TB, tb_touched);
+}
+
+// DEFER: a patch whose tile overflows is listed for k_p2g_big; otherwise it is
+// scattered here per particle with global atomics (and big_n[2] notes it, so
+// that the host switches to the listing form).  The tile scatter is written
+// out here rather than called as p2g_tile (k_p2g_big's): the shared form
+// compiled to a schedule measured 2 % slower on this kernel.
+template <bool DEFER>
 __global__ __launch_bounds__(DCP_WG, 2) void k_p2g(SimDev sim, StateView S, int s, int g, const int* skip) {
     if (skip && *skip) return;   // adjoint substep whose grid is restored from a snapshot
     __shared__ BinLDS L;
@@ -236,11 +312,72 @@ __global__ __launch_bounds__(DCP_WG, 2) void k_p2g(SimDev sim, StateView S, int 
             }
         }
         if (!tiled) {
+            if (DEFER) {   // left to k_p2g_big
+                if (tid == 0) sim.big_list[atomicAdd(&sim.big_n[DCP_BIG_P2G], 1)] = pi;
+            } else {
+                if (tid == 0) {
+                    atomicOr(sim.status, (unsigned)DCP_STATUS_TILE_SPILL);
+                    sim.big_n[DCP_BIG_SEEN] = 1;
+                }
+                p2g_spill(sim, S, SP.part(0), I, s, g);
+                p2g_spill(sim, S, SP.part(1), I, s, g);
+            }
+        }
+        // next patch: empty counts and box
+        bin_clear(L);
+        if (tid < 3) { L.box[tid] = 1 << 30; L.box[3 + tid] = -(1 << 30); }
+        __syncthreads();
+    }
+}
+
+// The patches k_p2g left (their tile exceeds the LDS cap: a cloth patch
+// spread by folding, a wide rigid body): the same tile scatter over x-slabs
+// of the patch box, w base planes each; a patch with no slab that fits (or
+// more than DCP_WG particles) scatters per particle with global atomics
+// (status bit DCP_STATUS_TILE_SPILL).  k_gridop, next on the stream, empties
+// the list.
+__global__ __launch_bounds__(DCP_WG, 2) void k_p2g_big(SimDev sim, StateView S, int s, int g, const int* skip) {
+    if (skip && *skip) return;
+    const int n_big = sim.big_n[DCP_BIG_P2G];
+    if ((int)blockIdx.x >= n_big) return;   // the usual case: an empty list
+    __shared__ BinLDS L;
+    __shared__ float4 contrib[DCP_CONTRIB_ZERO + 1];
+    __shared__ int tb_touched[DCP_TB_CAP];
+    const int tid = threadIdx.x;
+    bin_clear(L);
+    if (tid < 3) { L.box[tid] = 1 << 30; L.box[3 + tid] = -(1 << 30); }
+    if (tid == 0) contrib[DCP_CONTRIB_ZERO] = make_float4(0.f, 0.f, 0.f, 0.f);
+    __syncthreads();
+    for (int bi = blockIdx.x; bi < n_big; bi += gridDim.x) {
+        const int pi = sim.big_list[bi];
+        const SPatch SP = sim.spatches[pi];
+        const Patch P = SP.part(0);
+        const InstDev I = sim.inst[SP.inst];
+        bool tiled = SP.count() <= DCP_WG;
+        if (tiled) {
+            const bool act = tid < SP.count();
+            P2GRec r;
+            int b0 = 0, b1 = 0, b2 = 0;
+            if (act) p2g_patch_record(sim, I, S, SP, tid, s, g, r, b0, b1, b2);
+            PatchBox PB;
+            box_reduce(b0, b1, b2, act, L.box, PB);
+            const SlabPlan SPL = slab_plan(PB);
+            tiled = SPL.w >= 1;
+            for (int x0 = 0; tiled && x0 < SPL.planes; x0 += SPL.w) {
+                const PatchBox B = slab_box(PB, x0, SPL);
+                if (x0 > 0) {   // the previous slab's bins and block marks are read
+                    bin_clear(L);
+                    __syncthreads();
+                }
+                p2g_tile(sim, I, P, B, act && b0 >= B.l0 && b0 < B.l0 + B.e0 - 2, r, b0, b1, b2, L, contrib,
+                         tb_touched);
+            }
+        }
+        if (!tiled) {
             if (tid == 0) atomicOr(sim.status, (unsigned)DCP_STATUS_TILE_SPILL);
             p2g_spill(sim, S, SP.part(0), I, s, g);
             p2g_spill(sim, S, SP.part(1), I, s, g);
         }
-        // next patch: empty counts and box
         bin_clear(L);
         if (tid < 3) { L.box[tid] = 1 << 30; L.box[3 + tid] = -(1 << 30); }
         __syncthreads();
@@ -266,6 +403,8 @@ __device__ __forceinline__ void snap_offsets(const SimDev& sim, const int* count
 // the snapshot before being cleared.  `skip` != 0: the adjoint restores this
 // substep's grid from its snapshot instead.
 __global__ __launch_bounds__(DCP_WG) void k_gridop(SimDev sim, int keep, GridSnap save, const int* skip) {
+    // k_p2g_big, just before on the stream, has read the overflow list
+    if (blockIdx.x == 0 && threadIdx.x == 0) sim.big_n[DCP_BIG_P2G] = 0;
     if (skip && *skip) return;
     const int lane = threadIdx.x & 63;
     ActiveWalk w;
@@ -519,3 +658,6 @@ __global__ void k_binning_keys(SimDev sim, StateView S, int* keys) {
 
 template __global__ void k_prep<double>(SimDev, StateView, StateView, int);
 template __global__ void k_prep<float>(SimDev, StateView, StateView, int);
+
+template __global__ void k_p2g<false>(SimDev, StateView, int, int, const int*);
+template __global__ void k_p2g<true>(SimDev, StateView, int, int, const int*);

@@ -245,6 +245,35 @@ __device__ __forceinline__ float3 node_sum3(const BinLDS& L, const PatchBox& B, 
     return acc;
 }
 
+// A patch whose tile exceeds DCP_STILE_CAP cells is scattered in slabs of w
+// base planes along x (tile = w + 2 planes); w < 1 (one plane's tile does not
+// fit) leaves the patch to the per-particle global path.
+struct SlabPlan {
+    int planes, w;
+};
+__device__ __forceinline__ SlabPlan slab_plan(const PatchBox& B) {
+    SlabPlan p;
+    p.planes = B.e0 - 2;
+    p.w = B.vol <= DCP_STILE_CAP ? p.planes : DCP_STILE_CAP / (B.e1 * B.e2) - 2;
+    return p;
+}
+__device__ __forceinline__ PatchBox slab_box(const PatchBox& B, int x0, const SlabPlan& p) {
+    PatchBox S = B;
+    S.l0 = B.l0 + x0;
+    S.e0 = min(p.w, p.planes - x0) + 2;
+    S.vol = S.e0 * S.e1 * S.e2;
+    return S;
+}
+
+// Lists of the patches whose tile exceeds the cap, filled by k_p2g /
+// k_g2p_adj in their listing form, worked off by k_p2g_big / k_g2p_adj_big and
+// emptied by the grid update that follows each (k_gridop / k_gridop_adj):
+// lengths in big_n[0..1], list l at big_list[l * n_spatches ...].  big_n[2]
+// is set once the non-listing form met an overflowing patch.
+#define DCP_BIG_P2G 0
+#define DCP_BIG_G2P_ADJ 1
+#define DCP_BIG_SEEN 2
+
 // Box of the patch's base cells from one candidate per lane.
 __device__ __forceinline__ void box_reduce(int b0, int b1, int b2, bool act, int* box, PatchBox& B) {
     const int big = 1 << 30;

@@ -74,6 +74,8 @@ struct SimDev {
     const Patch* patches;          // point patches then element patches
     const SPatch* spatches;        // scatter patches (P2G and the G2P adjoint)
     int n_spatches;
+    int* big_list;                 // [2][n_spatches]: scatter patches whose tile exceeds the cap
+    int* big_n;                    // [4]: list lengths, finished-workgroup counters
     const int* tri;                // [3][NE] point indices
     const float* Dinv;             // [9][NE] inverse of the rest d
     const float* Dinvinv;          // [9][NE] inverse(Dinv) (the d <- F @ Dinv^-1 round trip)
