@@ -285,7 +285,7 @@ static hipError_t time_mark(dcp_ctx* c) {
 // is made.  Once one has been met (noted on the device and read back after a
 // rollout), the context switches for good to the listing form: the scatters
 // list such patches and k_p2g_big / k_g2p_adj_big, launched after each,
-// scatter them through the LDS tile in x-slabs.
+// scatter them through the LDS tile over sub-boxes that fit it.
 template <bool DEFER>
 static int launch_p2g_form(dcp_ctx* c, StateView S, int s, int g, const int* skip) {
     if (skip) {

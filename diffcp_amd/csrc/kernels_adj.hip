@@ -108,7 +108,7 @@ __device__ __forceinline__ void g2p_adj_spill(const SimDev& sim, const StateView
 // the three layers and is stored after the last.  grid_v_out of the tile is staged in
 // LDS once per patch, so every node read after that is an LDS read.
 // G2P reverse, through one LDS tile over box B, of the patch particles whose
-// base cells lie in B (`act` lanes; B is the patch's box, or one x-slab of it
+// base cells lie in B (`act` lanes; B is the patch's box, or one sub-box of it
 // in k_g2p_adj_big): node terms, dL/dx stores, and the flush of dL/d grid_v_out.
 __device__ __forceinline__ void g2p_adj_tile(const SimDev& sim, const InstDev& I, const StateView& Gout,
                                              const PatchBox& B, bool act, const V3<float>& x, const V3<float>& gv,
@@ -293,9 +293,9 @@ __global__ __launch_bounds__(DCP_WG, 3) void k_g2p_adj(SimDev sim, StateView S, 
     }
 }
 
-// The patches k_g2p_adj left: the same tile reverse over x-slabs of the patch
-// box (as k_p2g_big), else per particle with global atomics.  k_gridop_adj,
-// next on the stream, empties the list.
+// The patches k_g2p_adj left: the same tile reverse over the sub-boxes of the
+// patch box (as k_p2g_big).  k_gridop_adj, next on the stream, empties the
+// list.
 __global__ __launch_bounds__(DCP_WG, 2) void k_g2p_adj_big(SimDev sim, StateView S, StateView G1, StateView Gout,
                                                            int pending) {
     const int n_big = sim.big_n[DCP_BIG_G2P_ADJ];
@@ -324,17 +324,21 @@ __global__ __launch_bounds__(DCP_WG, 2) void k_g2p_adj_big(SimDev sim, StateView
             if (act) g2p_adj_patch_inputs(sim, I, S, G1, SP, tid, pending, Pl, ql, x, gv, gx1, gC, b0, b1, b2);
             PatchBox PB;
             box_reduce(b0, b1, b2, act, L.box, PB);
-            const SlabPlan SPL = slab_plan(PB);
-            tiled = SPL.w >= 1;
-            for (int x0 = 0; tiled && x0 < SPL.planes; x0 += SPL.w) {
-                const PatchBox B = slab_box(PB, x0, SPL);
-                if (x0 > 0) {   // the previous slab's bins and tile are read
-                    bin_clear(L);
-                    __syncthreads();
-                }
-                g2p_adj_tile(sim, I, Gout, B, act && b0 >= B.l0 && b0 < B.l0 + B.e0 - 2, x, gv, gx1, gC, Pl, ql,
-                             b0, b1, b2, L, cA, cB, vt);
-            }
+            const SubPlan SPL = sub_plan(PB);
+            bool first = true;
+            for (int x0 = 0; x0 < SPL.n[0]; x0 += SPL.w[0])
+                for (int y0 = 0; y0 < SPL.n[1]; y0 += SPL.w[1])
+                    for (int z0 = 0; z0 < SPL.n[2]; z0 += SPL.w[2]) {
+                        const PatchBox B = sub_box(PB, SPL, x0, y0, z0);
+                        const bool in = act && in_sub_box(B, b0, b1, b2);
+                        if (!__syncthreads_or(in)) continue;   // no particle in this sub-box
+                        if (!first) {   // the previous sub-box's bins and tile are read
+                            bin_clear(L);
+                            __syncthreads();
+                        }
+                        first = false;
+                        g2p_adj_tile(sim, I, Gout, B, in, x, gv, gx1, gC, Pl, ql, b0, b1, b2, L, cA, cB, vt);
+                    }
         }
         if (!tiled) {
             g2p_adj_spill(sim, S, G1, Gout, SP.part(0), I, pending);

@@ -167,7 +167,7 @@ __device__ __forceinline__ void p2g_patch_record(const SimDev& sim, const InstDe
 }
 
 // Scatter, through one LDS tile over box B, of the patch particles whose base
-// cells lie in B (`act` lanes; B is the patch's box, or one x-slab of it in
+// cells lie in B (`act` lanes; B is the patch's box, or one sub-box of it in
 // k_p2g_big).
 __device__ __forceinline__ void p2g_tile(const SimDev& sim, const InstDev& I, const Patch& P, const PatchBox& B,
                                          bool act, const P2GRec& r, int b0, int b1, int b2, BinLDS& L,
@@ -331,11 +331,11 @@ __global__ __launch_bounds__(DCP_WG, 2) void k_p2g(SimDev sim, StateView S, int 
 }
 
 // The patches k_p2g left (their tile exceeds the LDS cap: a cloth patch
-// spread by folding, a wide rigid body): the same tile scatter over x-slabs
-// of the patch box, w base planes each; a patch with no slab that fits (or
-// more than DCP_WG particles) scatters per particle with global atomics
-// (status bit DCP_STATUS_TILE_SPILL).  k_gridop, next on the stream, empties
-// the list.
+// spread by folding, a wide rigid body): the same tile scatter over sub-boxes
+// of the patch box whose tiles fit (sub_plan), skipping empty ones.  A patch
+// of more than DCP_WG particles (none by construction) would scatter per
+// particle with global atomics (status bit DCP_STATUS_TILE_SPILL).  k_gridop,
+// next on the stream, empties the list.
 __global__ __launch_bounds__(DCP_WG, 2) void k_p2g_big(SimDev sim, StateView S, int s, int g, const int* skip) {
     if (skip && *skip) return;
     const int n_big = sim.big_n[DCP_BIG_P2G];
@@ -361,17 +361,21 @@ __global__ __launch_bounds__(DCP_WG, 2) void k_p2g_big(SimDev sim, StateView S, 
             if (act) p2g_patch_record(sim, I, S, SP, tid, s, g, r, b0, b1, b2);
             PatchBox PB;
             box_reduce(b0, b1, b2, act, L.box, PB);
-            const SlabPlan SPL = slab_plan(PB);
-            tiled = SPL.w >= 1;
-            for (int x0 = 0; tiled && x0 < SPL.planes; x0 += SPL.w) {
-                const PatchBox B = slab_box(PB, x0, SPL);
-                if (x0 > 0) {   // the previous slab's bins and block marks are read
-                    bin_clear(L);
-                    __syncthreads();
-                }
-                p2g_tile(sim, I, P, B, act && b0 >= B.l0 && b0 < B.l0 + B.e0 - 2, r, b0, b1, b2, L, contrib,
-                         tb_touched);
-            }
+            const SubPlan SPL = sub_plan(PB);
+            bool first = true;
+            for (int x0 = 0; x0 < SPL.n[0]; x0 += SPL.w[0])
+                for (int y0 = 0; y0 < SPL.n[1]; y0 += SPL.w[1])
+                    for (int z0 = 0; z0 < SPL.n[2]; z0 += SPL.w[2]) {
+                        const PatchBox B = sub_box(PB, SPL, x0, y0, z0);
+                        const bool in = act && in_sub_box(B, b0, b1, b2);
+                        if (!__syncthreads_or(in)) continue;   // no particle in this sub-box
+                        if (!first) {   // the previous sub-box's bins and block marks are read
+                            bin_clear(L);
+                            __syncthreads();
+                        }
+                        first = false;
+                        p2g_tile(sim, I, P, B, in, r, b0, b1, b2, L, contrib, tb_touched);
+                    }
         }
         if (!tiled) {
             if (tid == 0) atomicOr(sim.status, (unsigned)DCP_STATUS_TILE_SPILL);

@@ -245,24 +245,35 @@ __device__ __forceinline__ float3 node_sum3(const BinLDS& L, const PatchBox& B, 
     return acc;
 }
 
-// A patch whose tile exceeds DCP_STILE_CAP cells is scattered in slabs of w
-// base planes along x (tile = w + 2 planes); w < 1 (one plane's tile does not
-// fit) leaves the patch to the per-particle global path.
-struct SlabPlan {
-    int planes, w;
+// A patch whose tile exceeds DCP_STILE_CAP cells is scattered block by block:
+// its box of base cells is cut into sub-boxes of at most w0 x w1 x w2 base
+// cells whose tiles ((w + 2) per axis) fit the cap, k extents first (whole
+// rows of the box when they fit), then j, then i.  Since 3^3 <= the cap, every
+// box can be cut so.
+struct SubPlan {
+    int n[3], w[3];   // base-cell extents of the box, sub-box extents
 };
-__device__ __forceinline__ SlabPlan slab_plan(const PatchBox& B) {
-    SlabPlan p;
-    p.planes = B.e0 - 2;
-    p.w = B.vol <= DCP_STILE_CAP ? p.planes : DCP_STILE_CAP / (B.e1 * B.e2) - 2;
+__device__ __forceinline__ SubPlan sub_plan(const PatchBox& B) {
+    SubPlan p;
+    p.n[0] = B.e0 - 2; p.n[1] = B.e1 - 2; p.n[2] = B.e2 - 2;
+    p.w[2] = min(p.n[2], DCP_STILE_CAP / 9 - 2);
+    p.w[1] = min(p.n[1], DCP_STILE_CAP / (3 * (p.w[2] + 2)) - 2);
+    p.w[0] = min(p.n[0], DCP_STILE_CAP / ((p.w[1] + 2) * (p.w[2] + 2)) - 2);
     return p;
 }
-__device__ __forceinline__ PatchBox slab_box(const PatchBox& B, int x0, const SlabPlan& p) {
+// sub-box with base-cell offset (x0, y0, z0) inside the patch box
+__device__ __forceinline__ PatchBox sub_box(const PatchBox& B, const SubPlan& p, int x0, int y0, int z0) {
     PatchBox S = B;
-    S.l0 = B.l0 + x0;
-    S.e0 = min(p.w, p.planes - x0) + 2;
+    S.l0 = B.l0 + x0; S.l1 = B.l1 + y0; S.l2 = B.l2 + z0;
+    S.e0 = min(p.w[0], p.n[0] - x0) + 2;
+    S.e1 = min(p.w[1], p.n[1] - y0) + 2;
+    S.e2 = min(p.w[2], p.n[2] - z0) + 2;
     S.vol = S.e0 * S.e1 * S.e2;
     return S;
+}
+__device__ __forceinline__ bool in_sub_box(const PatchBox& S, int b0, int b1, int b2) {
+    return b0 >= S.l0 && b0 < S.l0 + S.e0 - 2 && b1 >= S.l1 && b1 < S.l1 + S.e1 - 2 && b2 >= S.l2 &&
+           b2 < S.l2 + S.e2 - 2;
 }
 
 // Lists of the patches whose tile exceeds the cap, filled by k_p2g /
