@@ -153,6 +153,9 @@ struct dcp_ctx {
     int* toff = nullptr; int* tcount = nullptr; int4* tchunks = nullptr;
     int* vnat = nullptr;
     float* tmin = nullptr; int* targ = nullptr;
+    int loss_parts = 1;                       // vertex parts of the loss minimum (k_loss_min)
+    size_t part_cap = 0;                      // entries of the per-part buffers
+    float* pmin = nullptr; int* pnat = nullptr; int* parg = nullptr;
     double* loss_d = nullptr; double* seed_d = nullptr;
     std::vector<double> ones_h;        // host source of the unit loss seeds
     std::vector<float> traj_h;         // staging of dcp_set_trajectory
@@ -430,11 +433,12 @@ static int launch_fill(dcp_ctx* c, float* p, size_t n) {
 static int launch_loss(dcp_ctx* c, StateView S) {
     if (c->n_targets_total == 0) return set_err("no targets set (dcp_set_targets)");
     int nch = (int)c->tchunk_h.size();
-    hipLaunchKernelGGL(k_loss_min, dim3(std::max(1, std::min(nch, c->n_cu * 8))), dim3(DCP_WG), 0, c->stream, c->sim,
-                       S, c->targets, c->tchunks, nch, c->vnat, c->tmin, c->targ);
+    const int P = c->loss_parts, T = c->n_targets_total;
+    hipLaunchKernelGGL(k_loss_min, dim3(std::max(1, std::min(nch * P, c->n_cu * 16))), dim3(DCP_WG), 0, c->stream,
+                       c->sim, S, c->targets, c->tchunks, nch, P, T, c->vnat, c->pmin, c->pnat, c->parg);
     LAUNCH_OK();
-    hipLaunchKernelGGL(k_loss_sum, dim3(std::min(c->B, 1024)), dim3(DCP_WG), 0, c->stream, c->tmin, c->toff, c->tcount,
-                       c->loss_d, c->B);
+    hipLaunchKernelGGL(k_loss_sum, dim3(std::min(c->B, 1024)), dim3(DCP_WG), 0, c->stream, c->pmin, c->pnat, c->parg,
+                       P, T, c->tmin, c->targ, c->toff, c->tcount, c->loss_d, c->B);
     LAUNCH_OK();
     return 0;
 }
@@ -822,6 +826,19 @@ extern "C" int dcp_set_targets(dcp_ctx* c, const double* t, const int32_t* count
     }
     if (c->tchunks) { /* reallocate chunks each time (small) */ }
     if (c->upload(&c->tchunks, c->tchunk_h)) return -1;
+    // vertex parts of the loss minimum: enough work items for 16 per CU, at
+    // most 8 parts, every part of every instance with targets at least 128
+    // vertices (so the first part of an instance is never empty)
+    int minv = 1 << 30;
+    for (int b = 0; b < c->B; ++b)
+        if (counts[b] > 0) minv = std::min(minv, c->inst_h[b].v_count);
+    const int nch = std::max(1, (int)c->tchunk_h.size());
+    c->loss_parts = std::max(1, std::min({8, (c->n_cu * 16 + nch - 1) / nch, minv / 128}));
+    const size_t need = (size_t)c->loss_parts * std::max(1, tot);
+    if (need > c->part_cap) {
+        if (c->alloc(&c->pmin, need) || c->alloc(&c->pnat, need) || c->alloc(&c->parg, need)) return -1;
+        c->part_cap = need;
+    }
     HIP_OK(hipMemcpy(c->targets, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice));
     HIP_OK(hipMemcpy(c->toff, c->toff_h.data(), c->B * sizeof(int), hipMemcpyHostToDevice));
     HIP_OK(hipMemcpy(c->tcount, c->tcount_h.data(), c->B * sizeof(int), hipMemcpyHostToDevice));

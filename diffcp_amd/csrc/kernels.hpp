@@ -302,8 +302,9 @@ __global__ void k_gridop(SimDev sim, int keep, GridSnap save, const int* skip);
 __global__ void k_grid_restore(SimDev sim, GridSnap snap);
 __global__ void k_g2p(SimDev sim, StateView S, StateView Sn);
 __global__ void k_loss_min(SimDev sim, StateView S, const float* targets, const int4* tchunks, int n_chunks,
-                           const int* vnat, float* tmin, int* targ);
-__global__ void k_loss_sum(const float* tmin, const int* toff, const int* tcount, double* loss, int B);
+                           int n_parts, int n_targets, const int* vnat, float* pmin, int* pnat, int* parg);
+__global__ void k_loss_sum(const float* pmin, const int* pnat, const int* parg, int n_parts, int n_targets,
+                           float* tmin, int* targ, const int* toff, const int* tcount, double* loss, int B);
 __global__ void k_binning_keys(SimDev sim, StateView S, int* keys);
 __global__ void k_rebase(SimDev sim, StateView S, float* f_old, float* Dinv, float* Dinvinv);
 

@@ -557,49 +557,91 @@ __global__ __launch_bounds__(DCP_WG) void k_g2p(SimDev sim, StateView S, StateVi
 // loss: per target the first-minimum vertex (strict <, lowest natural index)
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(DCP_WG) void k_loss_min(SimDev sim, StateView S, const float* __restrict__ targets,
-                                                       const int4* __restrict__ tchunks, int n_chunks,
-                                                       const int* __restrict__ vnat, float* tmin, int* targ) {
-    __shared__ float sx[DCP_WG], sy[DCP_WG], sz[DCP_WG];
-    __shared__ int sn[DCP_WG];
+                                                       const int4* __restrict__ tchunks, int n_chunks, int n_parts,
+                                                       int n_targets, const int* __restrict__ vnat, float* pmin,
+                                                       int* pnat, int* parg) {
+    __shared__ float4 sp[DCP_WG];   // vertex x, y, z and natural index (bits), one 16-byte read per candidate
     const int tid = threadIdx.x;
-    for (int ci = blockIdx.x; ci < n_chunks; ci += gridDim.x) {
+    // work item = (target chunk, vertex part): part v scans the instance's
+    // vertices [v n / P, (v + 1) n / P); k_loss_sum merges the parts
+    for (int wi = blockIdx.x; wi < n_chunks * n_parts; wi += gridDim.x) {
+        const int ci = wi / n_parts, part = wi - ci * n_parts;
         const int4 ch = tchunks[ci];            // (instance, first target, count, total targets)
         const InstDev I = sim.inst[ch.x];
+        const int v0 = (int)((long long)I.v_count * part / n_parts);
+        const int v1 = (int)((long long)I.v_count * (part + 1) / n_parts);
         const int t = ch.y + tid;
         const bool act = tid < ch.z;
         float t0 = 0.f, t1 = 0.f, t2 = 0.f;
         if (act) { t0 = targets[3 * t]; t1 = targets[3 * t + 1]; t2 = targets[3 * t + 2]; }
-        float best = 0.f; int bnat = 0x7fffffff, bidx = -1;
-        for (int base = 0; base < I.v_count; base += DCP_WG) {
+        // The first part takes its first candidate whatever its distance (the
+        // reference's loop starts from the first vertex); the others start
+        // from +inf, so that a NaN distance is never taken there, as it never
+        // is after the first vertex.
+        float best = part == 0 ? 0.f : INFINITY;
+        int bnat = 0x7fffffff, bidx = -1;
+        // thr: squared distances above it give a quotient by 3 strictly above
+        // best (at least 4 ulps above the best's, more than the rounding of
+        // the quotient can close), so only those at or below it are divided
+        // and compared; after a best that small quotients could flush to zero,
+        // every candidate is compared
+        float thr = INFINITY;
+        for (int base = v0; base < v1; base += DCP_WG) {
             __syncthreads();
             int p = base + tid;
-            if (p < I.v_count) {
+            if (p < v1) {
                 int gp = I.v_start + p;
-                sx[tid] = S.px(0)[gp]; sy[tid] = S.px(1)[gp]; sz[tid] = S.px(2)[gp]; sn[tid] = vnat[gp];
+                sp[tid] = make_float4(S.px(0)[gp], S.px(1)[gp], S.px(2)[gp], __int_as_float(vnat[gp]));
             }
             __syncthreads();
-            int m = min(DCP_WG, I.v_count - base);
+            int m = min(DCP_WG, v1 - base);
             if (act) {
                 for (int u = 0; u < m; ++u) {
-                    float d0 = sx[u] - t0, d1 = sy[u] - t1, d2 = sz[u] - t2;
-                    float dn = (d0 * d0 + d1 * d1 + d2 * d2) / 3.0f;
-                    int nat = sn[u];
-                    if (bidx < 0 || dn < best || (dn == best && nat < bnat)) { best = dn; bnat = nat; bidx = I.v_start + base + u; }
+                    const float4 q = sp[u];
+                    float d0 = q.x - t0, d1 = q.y - t1, d2 = q.z - t2;
+                    const float sq = d0 * d0 + d1 * d1 + d2 * d2;
+                    if (!(sq <= thr) && bidx >= 0) continue;
+                    float dn = sq / 3.0f;
+                    int nat = __float_as_int(q.w);
+                    if ((part == 0 && bidx < 0) || dn < best || (dn == best && nat < bnat)) {
+                        best = dn; bnat = nat; bidx = I.v_start + base + u;
+                        thr = sq < 1e-30f ? INFINITY : sq * (1.0f + 0x1p-21f);
+                    }
                 }
             }
         }
-        if (act) { tmin[t] = best; targ[t] = bidx; }
+        if (act) {
+            const size_t o = (size_t)part * n_targets + t;
+            pmin[o] = best; pnat[o] = bnat; parg[o] = bidx;
+        }
     }
 }
 
 // per-instance FP64 sum in a fixed order
-__global__ __launch_bounds__(DCP_WG) void k_loss_sum(const float* tmin, const int* toff, const int* tcount,
-                                                      double* loss, int B) {
+__global__ __launch_bounds__(DCP_WG) void k_loss_sum(const float* pmin, const int* pnat, const int* parg, int n_parts,
+                                                      int n_targets, float* tmin, int* targ, const int* toff,
+                                                      const int* tcount, double* loss, int B) {
     __shared__ double red[DCP_WG];
     for (int b = blockIdx.x; b < B; b += gridDim.x) {
         double acc = 0.0;
         const int n = tcount[b], o = toff[b];
-        for (int q = threadIdx.x; q < n; q += DCP_WG) acc += (double)tmin[o + q];
+        for (int q = threadIdx.x; q < n; q += DCP_WG) {
+            // parts in vertex order: the first minimum of (distance, natural index)
+            const int t = o + q;
+            float best = pmin[t];
+            int bnat = pnat[t], bidx = parg[t];
+            for (int v = 1; v < n_parts; ++v) {
+                const size_t i = (size_t)v * n_targets + t;
+                const float dn = pmin[i];
+                const int nat = pnat[i];
+                if (parg[i] >= 0 && (bidx < 0 || dn < best || (dn == best && nat < bnat))) {
+                    best = dn; bnat = nat; bidx = parg[i];
+                }
+            }
+            tmin[t] = best;
+            targ[t] = bidx;
+            acc += (double)best;
+        }
         red[threadIdx.x] = acc;
         __syncthreads();
         for (int st = DCP_WG / 2; st > 0; st >>= 1) {
