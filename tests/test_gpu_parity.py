@@ -214,3 +214,41 @@ def _gpu_rollout_batch(insts, N, **kw):
     for f in range(N):
         eng.substep(f, 0, f, f + 1)
     return eng
+
+
+def test_loss_ties_take_the_first_vertex():
+    """Exact ties across the loss minimum's vertex parts: 40 vertices get the
+    position of another (lower natural index, anywhere in the mesh), and the
+    targets sit near those positions.  The loss matches brute force and every
+    target's adjoint lands on the lower index, as a single in-order scan with
+    a strict `<` gives (the twin receives exactly nothing)."""
+    from scipy.spatial.distance import cdist
+    inst = scene.config_c1().instances[0]
+    rng = np.random.default_rng(3)
+    eng = _gpu_rollout(inst, 4)
+    st = eng.download_state(4)
+    x = st["vertex_x"].copy()
+    nv = len(x)
+    a = rng.choice(nv // 2, 40, replace=False)
+    b = nv - 1 - rng.choice(nv // 2, 40, replace=False)
+    x[b] = x[a]
+    st["vertex_x"] = x
+    eng.upload_state(4, **st)
+    # targets rounded to FP32 as the engine stores them (the brute force then
+    # differs only by FP32 arithmetic), well inside the mesh spacing
+    targets = (x[a].astype(np.float64) + rng.normal(0, 0.0005, (40, 3))).astype(np.float32).astype(np.float64)
+    eng.set_targets([targets])
+    loss = eng.loss(4)[0]
+    xd = x.astype(np.float64)
+    d = cdist(xd, targets) ** 2 / 3
+    assert abs(loss - d.min(0).mean()) <= 1e-6 * abs(loss)
+    eng.zero_adjoint(0)
+    eng.loss_adjoint(4, 0)
+    g = eng.download_adjoint(0)["vertex_x"]
+    idx = d.argmin(0)                      # numpy: the first minimum
+    assert set(idx.tolist()) <= set(a.tolist())
+    ref = np.zeros_like(xd)
+    for q, p in enumerate(idx):
+        ref[p] += 2.0 * (xd[p] - targets[q]) / 3.0 / len(targets)
+    assert np.all(g[b] == 0)
+    assert _rel(g, ref) < 1e-5
