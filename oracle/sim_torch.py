@@ -23,16 +23,28 @@ the values are those of the reference's dense grid at those cells, and
 untouched cells of the dense grid hold v = 0/(0+1e-9) = 0 which no particle
 reads.
 
-Pins (tests/test_oracle.py): closed forms and invariants of the B-spline
-weights, Gram-Schmidt QR vs numpy, the 2x2 polar vs numpy SVD, the literal SVD
-backward vs finite differences away from degeneracy and its exact zero at
-sigma1 = sigma2, P2G mass/momentum conservation, rigid-motion reproduction of
-G2P, free fall, rest-state zero stress, the brute-force loss, and central
-finite differences of whole rollouts in float64.
+Pins (tests/test_oracle.py, tests/test_oracle_pins.py): closed forms and
+invariants of the B-spline weights, Gram-Schmidt QR vs numpy, the 2x2 polar vs
+numpy SVD, the literal SVD backward vs finite differences away from
+degeneracy and its exact zero at sigma1 = sigma2; hand-derived closed forms
+of ``element_stress`` (upper-triangular F with shear and normal compression,
+non-symmetric in-plane F against numpy's SVD and inverse, frame
+indifference); the element force vectors and the normal impulse isolated on
+a one-triangle scene (per-stencil sums, first and second moments);
+``grid_op`` wall by wall (both sides of every wall, the floor, corners,
+damping); ``g2p_gather`` on affine velocity fields; ``damp_element_C``;
+``update_elements`` (rest, translation, third-order rigid rotation, round
+trip); ``grasper_velocity`` at frame and window boundaries and past the last
+frame; P2G mass/momentum conservation, free fall, the brute-force loss, and
+central finite differences of whole rollouts in float64.  A mutation check
+plants a sign, index or transpose error in each of these functions in memory
+and shows that its pin rejects it.
 
 Parity unpinned (Taichi is not installed, no import possible): Taichi's FP32
-codegen order (FMA contraction, fast-math division), ti.svd's internal
-small-off-diagonal branch, float-atomic summation order.
+codegen order (FMA contraction, fast-math division, including the FP32
+frame index int(f*dt/interval) of simulation.py:160, replaced here by the
+exact integer f // steps_per_frame), ti.svd's internal small-off-diagonal
+branch, float-atomic summation order.
 """
 from __future__ import annotations
 
@@ -334,6 +346,79 @@ def element_stress(F: torch.Tensor, params: Dict[str, torch.Tensor], svd_grad: s
     return P, Q, R, Ft
 
 
+
+# ----------------------------------------------------------------------------
+# grid update, G2P gather and element update (simulation.py:274-430)
+# ----------------------------------------------------------------------------
+
+def grid_op(ijk: torch.Tensor, grid_m: torch.Tensor, grid_v: torch.Tensor, G: int, bound: int,
+            damping, dt: float) -> torch.Tensor:
+    """grid_op, simulation.py:274-302, on the cells ``ijk`` [n,3] (int): v = mv/(m+1e-9),
+    the damping extension v *= (1 - damping dt) (identity at 0), then the wall
+    conditions in the reference's order: x low, x high, z low, z high (all
+    components zeroed when moving outward), the floor j < bound (only v_y
+    zeroed when v_y < 0), y high (all components).  Low walls are i < bound,
+    high walls i > G - bound."""
+    gi, gj, gk = ijk[:, 0], ijk[:, 1], ijk[:, 2]
+    inv_m = 1 / (grid_m + 1e-9)
+    v_out = inv_m[:, None] * grid_v
+    v_out = v_out * (1 - damping * dt)
+    b = bound
+    zero3 = torch.zeros_like(v_out)
+
+    def zero_all(cond):
+        return torch.where(cond[:, None], zero3, v_out)
+    v_out = zero_all((gi < b) & (v_out[:, 0] < 0))                                # :279-282
+    v_out = zero_all((gi > G - b) & (v_out[:, 0] > 0))                            # :283-286
+    v_out = zero_all((gk < b) & (v_out[:, 2] < 0))                                # :287-290
+    v_out = zero_all((gk > G - b) & (v_out[:, 2] > 0))                            # :291-294
+    c5 = (gj < b) & (v_out[:, 1] < 0)                                             # :295-296
+    v_out = torch.stack([v_out[:, 0], torch.where(c5, torch.zeros_like(v_out[:, 1]), v_out[:, 1]),
+                         v_out[:, 2]], -1)
+    v_out = zero_all((gj > G - b) & (v_out[:, 1] > 0))                            # :297-300
+    return v_out
+
+
+def g2p_gather(x: torch.Tensor, gv: torch.Tensor, inv_dx: float):
+    """APIC gather of g2p, simulation.py:304-380: new_v = sum w g and
+    new_C = 4 inv_dx sum w g (i - fx)^T over the 27 stencil nodes, whose grid
+    velocities ``gv`` [N,27,3] are in the reference's i,j,k loop order."""
+    base, fx, w, _ = bspline(x, inv_dx)
+    W = stencil_weights(w)
+    dp = OFFSETS[None].to(x.dtype) - fx[:, None, :]
+    new_v = (W[..., None] * gv).sum(1)
+    new_C = ((4 * W[..., None, None] * (gv[..., :, None] * dp[..., None, :])) * inv_dx).sum(1)
+    return new_v, new_C
+
+
+def damp_element_C(C: torch.Tensor) -> torch.Tensor:
+    """Element affine matrix kept as skew(C) + (1 - 0.95) sym(C), simulation.py:381-383."""
+    C_skew = 0.5 * (C - C.transpose(-1, -2))
+    C_sym = 0.5 * (C + C.transpose(-1, -2))
+    return C_skew + (1 - 0.95) * C_sym
+
+
+def update_elements(vx: torch.Tensor, vv: torch.Tensor, C: torch.Tensor, d: torch.Tensor,
+                    Dinv: torch.Tensor, tri: torch.Tensor, dt: float) -> Dict[str, torch.Tensor]:
+    """update_particle_state, simulation.py:386-430, from the new vertex state
+    (vx, vv), the new element affine matrix C and the old d: element v and x
+    are the vertex means; d = [x2-x1 | x3-x1 | col3 + (dt C + dt^2/2 C C) col3]
+    with col3 the old d's normal column; F = d Dinv; the stored d is the
+    round trip F inverse(Dinv).  The return mapping of :408-428 changes only a
+    local R that is never written back, so it is left out."""
+    x1, x2, x3 = vx[tri[:, 0]], vx[tri[:, 1]], vx[tri[:, 2]]
+    v1, v2, v3 = vv[tri[:, 0]], vv[tri[:, 1]], vv[tri[:, 2]]
+    out = {"qv": (1 / 3) * (v1 + v2 + v3),                                        # :392
+           "qx": (1 / 3) * (x1 + x2 + x3)}                                        # :393
+    col3 = d[:, :, 2]
+    M = dt * C + 0.5 * dt * dt * C @ C
+    col3n = col3 + (M @ col3[..., None])[..., 0]                                  # :397-400
+    new_d = cols(x2 - x1, x3 - x1, col3n)
+    new_F = new_d @ Dinv                                                          # :407
+    out["qd"] = new_F @ inv3(Dinv)                                                # :429
+    out["qF"] = new_F                                                             # :430
+    return out
+
 # ----------------------------------------------------------------------------
 # one substep
 # ----------------------------------------------------------------------------
@@ -439,40 +524,14 @@ def substep(st: Dict[str, torch.Tensor], static, params, K: Consts, f: int, g: i
     gi = uniq // (G * G)
     gj = (uniq // G) % G
     gk = uniq % G
-    inv_m = 1 / (grid_m + 1e-9)
-    v_out = inv_m[:, None] * grid_v
-    gamma = params["damping"]
-    v_out = v_out * (1 - gamma * dt)                  # damping extension: identity at 0
-    b = K.bound
-    zero3 = torch.zeros_like(v_out)
-
-    def zero_all(cond):
-        return torch.where(cond[:, None], zero3, v_out)
-    v_out = zero_all((gi < b) & (v_out[:, 0] < 0))
-    v_out = zero_all((gi > G - b) & (v_out[:, 0] > 0))
-    v_out = zero_all((gk < b) & (v_out[:, 2] < 0))
-    v_out = zero_all((gk > G - b) & (v_out[:, 2] > 0))
-    c5 = (gj < b) & (v_out[:, 1] < 0)
-    v_out = torch.stack([v_out[:, 0], torch.where(c5, torch.zeros_like(v_out[:, 1]), v_out[:, 1]),
-                         v_out[:, 2]], -1)
-    v_out = zero_all((gj > G - b) & (v_out[:, 1] > 0))
+    v_out = grid_op(torch.stack([gi, gj, gk], -1), grid_m, grid_v, G, K.bound, params["damping"], dt)
 
     # ---- g2p (:304-383) ----
-    off = 0
     sizes = [k.shape[0] for k in keys]
     inv_split = torch.split(inv, [s * 27 for s in sizes])
 
-    def gather(idx_flat, n):
-        return v_out[idx_flat].reshape(n, 27, 3)
-
     def g2p_point(x, which):
-        base, fx, w, _ = bspline(x, inv_dx)
-        W = stencil_weights(w)
-        dp = OFFSETS[None].to(dtype) - fx[:, None, :]
-        gv = gather(inv_split[which], x.shape[0])
-        new_v = (W[..., None] * gv).sum(1)
-        new_C = ((4 * W[..., None, None] * (gv[..., :, None] * dp[..., None, :])) * inv_dx).sum(1)
-        return new_v, new_C
+        return g2p_gather(x, v_out[inv_split[which]].reshape(x.shape[0], 27, 3), inv_dx)
 
     new = {}
     cv, cC = g2p_point(st["cx"], 0)
@@ -482,22 +541,10 @@ def substep(st: Dict[str, torch.Tensor], static, params, K: Consts, f: int, g: i
     vv, vC = g2p_point(st["vx"], 2)
     new["vv"], new["vx"], new["vC"] = vv, st["vx"] + dt * vv, vC
     _, qCn = g2p_point(st["qx"], 3)
-    C_skew = 0.5 * (qCn - qCn.transpose(-1, -2))
-    C_sym = 0.5 * (qCn + qCn.transpose(-1, -2))
-    qC1 = C_skew + (1 - 0.95) * C_sym                                             # :381-383
+    qC1 = damp_element_C(qCn)                                                     # :381-383
 
     # ---- update_particle_state (:386-430) ----
-    x1, x2, x3 = new["vx"][tri[:, 0]], new["vx"][tri[:, 1]], new["vx"][tri[:, 2]]
-    v1, v2, v3 = new["vv"][tri[:, 0]], new["vv"][tri[:, 1]], new["vv"][tri[:, 2]]
-    new["qv"] = (1 / 3) * (v1 + v2 + v3)                                         # :392
-    new["qx"] = (1 / 3) * (x1 + x2 + x3)                                         # :393
-    col3 = d[:, :, 2]
-    M = dt * qC1 + 0.5 * dt * dt * qC1 @ qC1
-    col3n = col3 + (M @ col3[..., None])[..., 0]                                  # :397-400
-    new_d = cols(x2 - x1, x3 - x1, col3n)
-    new_F = new_d @ Dinv                                                          # :407
-    new["qd"] = new_F @ inv3(Dinv)                                                # :429
-    new["qF"] = new_F                                                             # :430
+    new.update(update_elements(new["vx"], new["vv"], qC1, d, Dinv, tri, dt))
     new["qC"] = qC1
     if return_grid:
         return new, {"keys": uniq, "m": grid_m, "mv": grid_v, "v_out": v_out, "P": P}
