@@ -72,6 +72,7 @@ SIGNATURES = {
     "dcp_download_adjoint": (c_int, [c_void_p, c_int] + [c_flt_p] * 11),
     "dcp_upload_adjoint": (c_int, [c_void_p, c_int] + [c_flt_p] * 11),
     "dcp_binning_keys": (c_int, [c_void_p, c_int, c_int_p]),
+    "dcp_patch_order": (c_int, [c_void_p, c_int, c_int_p]),
     "dcp_status": (c_int, [c_void_p, ctypes.POINTER(ctypes.c_uint32), c_int]),
     "dcp_rollout_grad": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_dbl_p, c_dbl_p]),
     "dcp_rollout": (c_int, [c_void_p, c_int, c_int, c_int, c_int_p]),

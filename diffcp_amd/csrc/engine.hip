@@ -144,8 +144,8 @@ struct dcp_ctx {
     long long grid_cells = 0;
     int n_flags = 0;
     SimDev sim{};
-    // owned device buffers
-    std::vector<void*> allocs;
+    // owned device buffers and their sizes
+    std::vector<std::pair<void*, size_t>> allocs;
     float* states = nullptr;
     float* adj = nullptr;
     float* scratch = nullptr;
@@ -231,10 +231,28 @@ struct dcp_ctx {
         if (e != hipSuccess) return set_err(std::string("hipMalloc ") + std::to_string(bytes) + " B: " + hipGetErrorString(e));
         e = hipMemset(q, 0, bytes);
         if (e != hipSuccess) return set_err(std::string("hipMemset: ") + hipGetErrorString(e));
-        allocs.push_back(q);
+        allocs.emplace_back(q, bytes);
         alloc_bytes += bytes;
         *p = (T*)q;
         return 0;
+    }
+    // free one buffer from alloc() (null: nothing); the stream must be idle
+    template <typename T> void release(T*& p) {
+        if (!p) return;
+        for (size_t i = 0; i < allocs.size(); ++i)
+            if (allocs[i].first == (void*)p) {
+                (void)hipFree(p);
+                alloc_bytes -= allocs[i].second;
+                allocs.erase(allocs.begin() + (long)i);
+                break;
+            }
+        p = nullptr;
+    }
+    // cached rollout graphs hold kernel arguments (buffer pointers, sizes):
+    // drop them before any of those change
+    void drop_graphs() {
+        for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second);
+        graphs.clear();
     }
     template <typename T> int upload(T** p, const std::vector<T>& h) {
         if (alloc(p, h.size())) return -1;
@@ -743,8 +761,8 @@ extern "C" int dcp_create(const dcp_instance_desc* instances, int32_t n, const d
 
 extern "C" int dcp_destroy(dcp_ctx* c) {
     if (!c) return 0;
-    for (auto& kv : c->graphs) hipGraphExecDestroy(kv.second);
-    for (void* p : c->allocs) hipFree(p);
+    c->drop_graphs();
+    for (auto& a : c->allocs) hipFree(a.first);
     if (c->ev0) hipEventDestroy(c->ev0);
     if (c->ev1) hipEventDestroy(c->ev1);
     if (c->ev_fork) hipEventDestroy(c->ev_fork);
@@ -805,26 +823,45 @@ extern "C" int dcp_set_element_moduli(dcp_ctx* c, const double* m) {
 extern "C" int dcp_set_targets(dcp_ctx* c, const double* t, const int32_t* counts) {
     if (!c || !counts) return set_err("bad args");
     int tot = 0;
+    for (int b = 0; b < c->B; ++b) {
+        if (counts[b] < 0) return set_err("negative target count");
+        tot += counts[b];
+    }
+    if (tot > 0 && !t) return set_err("bad args");
+    std::vector<float> h(3 * (size_t)std::max(1, tot), 0.f);
+    for (int i = 0; i < 3 * tot; ++i) h[i] = (float)t[i];
+    // same counts as the targets in place: only the coordinates change, so
+    // the buffers, chunks and every cached graph stay valid
+    const bool same = c->targets && (int)c->tcount_h.size() == c->B &&
+                      std::equal(c->tcount_h.begin(), c->tcount_h.end(), counts);
+    if (same) {
+        HIP_OK(hipMemcpyAsync(c->targets, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
+        HIP_OK(hipStreamSynchronize(c->stream));
+        return 0;
+    }
+    // new counts: the loss kernels' buffers, chunk table and launch sizes
+    // change, and cached graphs hold the old ones
+    HIP_OK(hipStreamSynchronize(c->stream));
+    c->drop_graphs();
+    c->release(c->targets); c->release(c->tmin); c->release(c->targ); c->release(c->tchunks);
+    c->release(c->pmin); c->release(c->pnat); c->release(c->parg);
+    c->part_cap = 0;
+    c->n_targets_total = 0;
     c->toff_h.assign(c->B, 0);
     c->tcount_h.assign(c->B, 0);
     c->tchunk_h.clear();
+    int off = 0;
     for (int b = 0; b < c->B; ++b) {
-        if (counts[b] < 0) return set_err("negative target count");
-        c->toff_h[b] = tot; c->tcount_h[b] = counts[b];
-        for (int o = 0; o < counts[b]; o += DCP_WG) c->tchunk_h.push_back(make_int4(b, tot + o, std::min(DCP_WG, counts[b] - o), counts[b]));
-        tot += counts[b];
+        c->toff_h[b] = off; c->tcount_h[b] = counts[b];
+        for (int o = 0; o < counts[b]; o += DCP_WG) c->tchunk_h.push_back(make_int4(b, off + o, std::min(DCP_WG, counts[b] - o), counts[b]));
+        off += counts[b];
     }
-    std::vector<float> h(3 * (size_t)std::max(1, tot));
-    for (int i = 0; i < 3 * tot; ++i) h[i] = (float)t[i];
-    if (tot > c->n_targets_total || !c->targets) {
-        if (c->alloc(&c->targets, h.size()) || c->alloc(&c->tmin, (size_t)std::max(1, tot)) ||
-            c->alloc(&c->targ, (size_t)std::max(1, tot)))
-            return -1;
-    }
+    if (c->alloc(&c->targets, h.size()) || c->alloc(&c->tmin, (size_t)std::max(1, tot)) ||
+        c->alloc(&c->targ, (size_t)std::max(1, tot)))
+        return -1;
     if (!c->toff) {
         if (c->alloc(&c->toff, c->B) || c->alloc(&c->tcount, c->B)) return -1;
     }
-    if (c->tchunks) { /* reallocate chunks each time (small) */ }
     if (c->upload(&c->tchunks, c->tchunk_h)) return -1;
     // vertex parts of the loss minimum: enough work items for 16 per CU, at
     // most 8 parts, every part of every instance with targets at least 128
@@ -835,10 +872,8 @@ extern "C" int dcp_set_targets(dcp_ctx* c, const double* t, const int32_t* count
     const int nch = std::max(1, (int)c->tchunk_h.size());
     c->loss_parts = std::max(1, std::min({8, (c->n_cu * 16 + nch - 1) / nch, minv / 128}));
     const size_t need = (size_t)c->loss_parts * std::max(1, tot);
-    if (need > c->part_cap) {
-        if (c->alloc(&c->pmin, need) || c->alloc(&c->pnat, need) || c->alloc(&c->parg, need)) return -1;
-        c->part_cap = need;
-    }
+    if (c->alloc(&c->pmin, need) || c->alloc(&c->pnat, need) || c->alloc(&c->parg, need)) return -1;
+    c->part_cap = need;
     HIP_OK(hipMemcpy(c->targets, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice));
     HIP_OK(hipMemcpy(c->toff, c->toff_h.data(), c->B * sizeof(int), hipMemcpyHostToDevice));
     HIP_OK(hipMemcpy(c->tcount, c->tcount_h.data(), c->B * sizeof(int), hipMemcpyHostToDevice));
@@ -1137,6 +1172,41 @@ extern "C" int dcp_binning_keys(dcp_ctx* c, int32_t slot, int32_t* out) {
     return 0;
 }
 
+extern "C" int dcp_patch_order(dcp_ctx* c, int32_t slot, int32_t* out) {
+    if (!c || !slot_ok(c, slot) || !out) return set_err("dcp_patch_order: bad args");
+    // element positions are the vertex means of the prep (non-fresh form, as dcp_binning_keys)
+    if (launch_prep(c, c->state(slot), c->scratch_view(), 0)) return -1;
+    int* d = nullptr;
+    const size_t n = 4 * (size_t)c->n_spatches * DCP_WG;
+    if (c->alloc(&d, n)) return -1;
+    hipLaunchKernelGGL(k_patch_order, dim3(c->grid_spatches()), dim3(DCP_WG), 0, c->stream, c->sim, c->state(slot), d);
+    hipError_t le = hipGetLastError();
+    std::vector<int> h(n);
+    hipError_t ce = le == hipSuccess ? hipMemcpyAsync(h.data(), d, n * sizeof(int), hipMemcpyDeviceToHost, c->stream) : le;
+    hipError_t se = hipStreamSynchronize(c->stream);
+    c->release(d);
+    if (ce != hipSuccess) return set_err(std::string("dcp_patch_order: ") + hipGetErrorString(ce));
+    if (se != hipSuccess) return set_err(std::string("dcp_patch_order: ") + hipGetErrorString(se));
+    size_t row = 0;
+    for (int pi = 0; pi < c->n_spatches; ++pi) {
+        const SPatch& P = c->spatches_h[pi];
+        const HostInst& H = c->hi[P.inst];
+        const InstDev& I = c->inst_h[P.inst];
+        for (int q = 0; q < P.count_a + P.count_b; ++q, ++row) {
+            const bool a = q < P.count_a;
+            const int cls = a ? P.cls_a : P.cls_b, idx = a ? P.start_a + q : P.start_b + (q - P.count_a);
+            int nat = 0;
+            if (cls == CLS_ELEMENT) nat = H.eperm[idx - I.e_start];
+            else nat = c->vnat_h[idx];
+            int32_t* o = out + 9 * row;
+            const int* k = &h[4 * ((size_t)pi * DCP_WG + q)];
+            o[0] = pi; o[1] = q; o[2] = P.inst; o[3] = cls; o[4] = nat;
+            o[5] = k[0]; o[6] = k[1]; o[7] = k[2]; o[8] = k[3];
+        }
+    }
+    return 0;
+}
+
 extern "C" int dcp_status(dcp_ctx* c, uint32_t* out, int32_t clear) {
     if (!c || !out) return set_err("bad args");
     HIP_OK(hipMemcpyAsync(out, c->sim.status, 4, hipMemcpyDeviceToHost, c->stream));
@@ -1274,8 +1344,7 @@ static int ensure_snapshots(dcp_ctx* c, int K) {
     HIP_OK(hipMemGetInfo(&free_b, &total_b));
     // cached graphs hold the old buffers
     HIP_OK(hipStreamSynchronize(c->stream));
-    for (auto& kv : c->graphs) (void)hipGraphExecDestroy(kv.second);
-    c->graphs.clear();
+    c->drop_graphs();
     c->free_snapshots();
     const size_t per_block = 64 * sizeof(float4) + sizeof(int);
     // element prep slots when they fit in an eighth of the free memory
@@ -1317,8 +1386,7 @@ static int update_overflow_form(dcp_ctx* c) {
     HIP_OK(hipMemcpy(&seen, c->sim.big_n + DCP_BIG_SEEN, sizeof(int), hipMemcpyDeviceToHost));
     if (!seen) return 0;
     HIP_OK(hipStreamSynchronize(c->stream));
-    for (auto& kv : c->graphs) (void)hipGraphExecDestroy(kv.second);
-    c->graphs.clear();
+    c->drop_graphs();
     c->defer_big = true;
     return 0;
 }
@@ -1393,8 +1461,7 @@ extern "C" int dcp_set_kernel_timing(dcp_ctx* c, int32_t kernel, int32_t max_lau
     if (!c || kernel < 0 || kernel > DCP_KERNEL_VERTEX_FINAL || max_launches < 0) return set_err("bad args");
     HIP_OK(hipStreamSynchronize(c->stream));
     // cached graphs hold the previous event nodes (or none): rebuild them
-    for (auto& kv : c->graphs) hipGraphExecDestroy(kv.second);
-    c->graphs.clear();
+    c->drop_graphs();
     c->drop_timing_events();
     c->time_kid = kernel;
     if (kernel == 0) return 0;

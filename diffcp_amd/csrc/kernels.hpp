@@ -306,6 +306,7 @@ __global__ void k_loss_min(SimDev sim, StateView S, const float* targets, const 
 __global__ void k_loss_sum(const float* pmin, const int* pnat, const int* parg, int n_parts, int n_targets,
                            float* tmin, int* targ, const int* toff, const int* tcount, double* loss, int B);
 __global__ void k_binning_keys(SimDev sim, StateView S, int* keys);
+__global__ void k_patch_order(SimDev sim, StateView S, int* out);
 __global__ void k_rebase(SimDev sim, StateView S, float* f_old, float* Dinv, float* Dinvinv);
 
 template <bool DEFER> __global__ void k_g2p_adj(SimDev sim, StateView S, StateView G1, StateView Gout, int pending);

@@ -491,7 +491,10 @@ __global__ __launch_bounds__(DCP_WG) void k_g2p(SimDev sim, StateView S, StateVi
         for (int q = tid; q < P.count; q += DCP_WG) {
             V3<float> x = particle_x(sim, S, P, q);
             Stencil st;
-            make_stencil(x, I.inv_dx, I.G, st, nullptr);
+            // every particle passes here once per substep with the position the
+            // substep's P2G scattered from, so this is where a stencil clamped
+            // at the grid edge (by P2G and G2P alike) raises the status bit
+            make_stencil(x, I.inv_dx, I.G, st, sim.status);
             // new_v = sum w g, new_C = 4 inv_dx sum w g dp^T (simulation.py:317-330),
             // dp = (i, j, k) - fx in cells; w = W0(i) W1(j) W2(k) is separable, so
             // the sums are contracted over k, then j, then i:
@@ -688,6 +691,43 @@ __global__ __launch_bounds__(DCP_WG) void k_rebase(SimDev sim, StateView S, floa
     }
 }
 
+// Base cells and sorted positions of every scatter patch's binning (the
+// stable counting sort patch_bin runs inside k_p2g and k_g2p_adj), for the
+// tests: out[4 (pi DCP_WG + q) + 0..3] = (b0, b1, b2, position) of lane q of
+// scatter patch pi; position -1 when the patch's tile exceeds the cap.
+__global__ __launch_bounds__(DCP_WG) void k_patch_order(SimDev sim, StateView S, int* out) {
+    __shared__ BinLDS L;
+    const int tid = threadIdx.x;
+    for (int pi = blockIdx.x; pi < sim.n_spatches; pi += gridDim.x) {
+        bin_clear(L);
+        if (tid < 3) { L.box[tid] = 1 << 30; L.box[3 + tid] = -(1 << 30); }
+        __syncthreads();
+        const SPatch SP = sim.spatches[pi];
+        if (SP.count() > DCP_WG) continue;   // none by construction (uniform per workgroup)
+        const InstDev I = sim.inst[SP.inst];
+        const bool act = tid < SP.count();
+        int b0 = 0, b1 = 0, b2 = 0;
+        if (act) {
+            Patch Pl;
+            int ql;
+            SP.lane(tid, Pl, ql);
+            const V3<float> x = particle_x(sim, S, Pl, ql);
+            b0 = stencil_base(x.x, I.inv_dx, I.G);
+            b1 = stencil_base(x.y, I.inv_dx, I.G);
+            b2 = stencil_base(x.z, I.inv_dx, I.G);
+        }
+        PatchBox B;
+        box_reduce(b0, b1, b2, act, L.box, B);
+        int pos = -1;
+        if (B.vol <= DCP_STILE_CAP) pos = patch_bin(act ? tile_cell(B, b0, b1, b2) : -1, B, L);
+        if (act) {
+            int* o = out + 4 * ((size_t)pi * DCP_WG + tid);
+            o[0] = b0; o[1] = b1; o[2] = b2; o[3] = pos;
+        }
+        __syncthreads();
+    }
+}
+
 // binning keys of a slot (integer, for the bit-exact tests)
 __global__ void k_binning_keys(SimDev sim, StateView S, int* keys) {
     for (int pi = blockIdx.x; pi < sim.n_patches; pi += gridDim.x) {
@@ -697,7 +737,7 @@ __global__ void k_binning_keys(SimDev sim, StateView S, int* keys) {
             V3<float> x = particle_x(sim, S, P, q);
             int idx = (P.cls == CLS_ELEMENT) ? sim.NP + P.start + q : P.start + q;
 #pragma unroll
-            for (int d = 0; d < 3; ++d) keys[3 * idx + d] = (int)(x[d] * I.inv_dx - 0.5f);
+            for (int d = 0; d < 3; ++d) keys[3 * idx + d] = base_key(x[d], I.inv_dx);
         }
     }
 }

@@ -23,7 +23,7 @@ struct PatchBox {
 };
 
 __device__ __forceinline__ int stencil_base(float x, float inv_dx, int G) {
-    int b = (int)(x * inv_dx - 0.5f);
+    int b = base_key(x, inv_dx);
     if (!(x == x) || b < 0) b = 0;
     if (b > G - 3) b = G - 3;
     return b;
@@ -36,12 +36,21 @@ __device__ __forceinline__ int tile_cell(const PatchBox& B, int i, int j, int k)
 
 // ---------------------------------------------------------------------------
 // Binning of a patch for the compute-then-gather scatters (one particle per
-// lane, P.count <= DCP_WG): per-cell counts with LDS atomics (the returned old
-// count is the particle's rank inside its cell; the order among particles of
-// one cell follows the LDS atomic order), then one block scan of packed
-// (count, occupied) words giving every cell's offset and every occupied
-// cell's run index, and the run table.  No permutation is stored: a particle's
-// sorted position is offset(cell) + rank.
+// lane, P.count <= DCP_WG), stable: a particle's sorted position is the
+// offset of its cell plus the number of lanes before it (in lane order, i.e.
+// particle order) whose particle has the same cell, so the permutation is a
+// stable argsort of the cell keys.
+//   1. per wave, ten ballots over the key bits give each lane the mask of the
+//      wave's lanes with its key; its rank in the wave is the population count
+//      of that mask below the lane;
+//   2. the first lane of each distinct key adds the key's count in its wave
+//      into byte `wave` of the cell's count word (LDS atomics; at most 64 per
+//      byte), so the word holds the four per-wave counts in wave order;
+//   3. after a barrier each lane adds the counts of the earlier waves (the
+//      bytes below its wave) to its rank, and one block scan of packed
+//      (count, occupied) words gives every cell's offset and every occupied
+//      cell's run index, and the run table.
+// No permutation is stored: a particle's sorted position is offset(cell) + rank.
 //
 // The offset table is indexed by tile cell and framed for the node sums: a
 // margin of DCP_OFF_MARGIN zero entries before cell 0 and the particle count
@@ -50,11 +59,13 @@ __device__ __forceinline__ int tile_cell(const PatchBox& B, int i, int j, int k)
 // ---------------------------------------------------------------------------
 #define DCP_OFF_MARGIN 256         // >= 2 e2 + 2 for every tile (e2 <= DCP_STILE_CAP / 9)
 static_assert(DCP_OFF_MARGIN >= 2 * (DCP_STILE_CAP / 9) + 2, "node_sum reaches 2 e2 + 2 cells before cell 0");
+static_assert(DCP_WG == 256, "patch_bin packs one count byte per wave of a 256-lane workgroup");
+static_assert(DCP_STILE_CAP <= 1024, "patch_bin matches 10-bit keys");
 struct BinLDS {
     int box[6];
     int wsum[DCP_WG / 64];
     int n_runs;
-    int off_mem[DCP_OFF_MARGIN + DCP_STILE_CAP + 1];   // margin | counts, then exclusive offsets | count
+    int off_mem[DCP_OFF_MARGIN + DCP_STILE_CAP + 1];   // margin | per-wave counts, then exclusive offsets | count
     unsigned short run_start[DCP_SORT_MAX];    // run (occupied cell) -> first position
     unsigned short run_len[DCP_SORT_MAX];
     __device__ int* off() { return off_mem + DCP_OFF_MARGIN; }
@@ -67,17 +78,47 @@ __device__ __forceinline__ void bin_clear(BinLDS& L) {
     for (int c = threadIdx.x; c < DCP_OFF_MARGIN + DCP_STILE_CAP + 1; c += DCP_WG) L.off_mem[c] = 0;
 }
 
+// Lanes of this wave whose key equals this lane's (keys in [0, 1024), or < 0
+// for lanes without a particle, which match each other only).  Called by
+// every lane of the wave.
+__device__ __forceinline__ unsigned long long same_key_lanes(int key) {
+    unsigned long long m = __ballot(key >= 0);
+    if (key < 0) m = ~m;
+#pragma unroll
+    for (int b = 0; b < 10; ++b) {
+        const unsigned long long set = __ballot((key >> b) & 1);
+        m &= ((key >> b) & 1) ? set : ~set;
+    }
+    return m;
+}
+
+// sum of the count bytes of waves 0 .. w-1 in a packed count word
+__device__ __forceinline__ int bytes_below(unsigned word, int w) {
+    int s = 0;
+#pragma unroll
+    for (int q = 0; q < DCP_WG / 64 - 1; ++q)
+        if (q < w) s += (word >> (8 * q)) & 0xff;
+    return s;
+}
+__device__ __forceinline__ int bytes_sum(unsigned word) {
+    return (int)((word & 0xff) + ((word >> 8) & 0xff) + ((word >> 16) & 0xff) + (word >> 24));
+}
+
 // `key` < 0 for a lane without a particle.  Returns the sorted position.
+// Called by every lane of the workgroup.
 __device__ __forceinline__ int patch_bin(int key, const PatchBox& B, BinLDS& L) {
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     int* off = L.off();
-    const int rank = key >= 0 ? atomicAdd(&off[key], 1) : 0;
+    const unsigned long long same = same_key_lanes(key);
+    int rank = __popcll(same & ((1ull << lane) - 1ull));
+    if (key >= 0 && rank == 0) atomicAdd(&off[key], __popcll(same) << (8 * wv));
     __syncthreads();
+    if (key >= 0) rank += bytes_below((unsigned)off[key], wv);
     int v[4], sum = 0;
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
         const int c = tid * 4 + t;
-        const int n = c < B.vol ? off[c] : 0;
+        const int n = c < B.vol ? bytes_sum((unsigned)off[c]) : 0;
         v[t] = n | (n > 0 ? 1 << 16 : 0);
         sum += v[t];
     }

@@ -221,13 +221,26 @@ struct Stencil {
     float dw[3][3];   // d w / d fx
 };
 
+// Integer base cell trunc(x*inv_dx - 0.5) (ti.cast to i32, simulation.py:140)
+// from a rounded FP32 product and a rounded FP32 difference, as the closed
+// form states it: the product is kept out of a fused multiply-add, which
+// could round across an integer when inv_dx is not a power of two.
+__device__ __forceinline__ float scaled_position(float x, float inv_dx) {
+#pragma clang fp contract(off)
+    return x * inv_dx;
+}
+__device__ __forceinline__ int base_key(float x, float inv_dx) {
+#pragma clang fp contract(off)
+    return (int)(scaled_position(x, inv_dx) - 0.5f);
+}
+
 // base = trunc(x*inv_dx - 0.5) (ti.cast to i32); clamped into [0, G-3] with a status bit.
 __device__ __forceinline__ void make_stencil(V3<float> x, float inv_dx, int G, Stencil& s, unsigned* status) {
     bool out = false;
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-        float xs = x[d] * inv_dx;
-        int b = (int)(xs - 0.5f);
+        float xs = scaled_position(x[d], inv_dx);
+        int b = base_key(x[d], inv_dx);
         if (b < 0 || b > G - 3 || !(xs == xs)) { out = true; b = b < 0 || !(xs == xs) ? 0 : G - 3; }
         s.base[d] = b;
         float f = xs - (float)b;

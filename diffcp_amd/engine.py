@@ -252,6 +252,16 @@ class Engine:
         _abi.check(self.lib.dcp_binning_keys(self.ctx, slot, out.ctypes.data_as(_abi.c_int_p)))
         return out
 
+    PATCH_ORDER_COLUMNS = ("patch", "lane", "instance", "class", "natural", "b0", "b1", "b2", "position")
+
+    def patch_order(self, slot: int) -> np.ndarray:
+        """The scatters' per-patch binning of ``slot``: one row per particle,
+        columns ``PATCH_ORDER_COLUMNS`` (class 0..3 = vertex, grasper, cube, element)."""
+        n = self.n_points + self.n_elements
+        out = np.zeros((n, 9), np.int32)
+        _abi.check(self.lib.dcp_patch_order(self.ctx, slot, out.ctypes.data_as(_abi.c_int_p)))
+        return out
+
     # ------------------------------------------------------------------ rollouts
     def rollout_grad(self, n_substeps: int, g: int = 0, checkpoint_every: int = 0,
                      use_graph: bool = True):

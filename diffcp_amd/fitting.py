@@ -120,6 +120,9 @@ class WindowedFit:
                       cube_x=eng.split(st["cube_x"], "n_cube"))
         loss_mean = np.mean(np.stack(losses), axis=0)
         grad_mean = np.mean(np.nan_to_num(np.stack(grads)), axis=0)
+        # the last window's raw dL/dE, before nan_to_num: what the reference's
+        # NaN stop reads (garment_E.grad[None], parameter_fitting_functions.py:148-150)
+        self.last_window_E_grad = grads[-1][:, 0].copy()
         return loss_mean, grad_mean
 
     def parameters_learning(self, params: np.ndarray, epochs: int, elasticity: bool = True,
@@ -146,7 +149,7 @@ class WindowedFit:
             best[take] = params[take]
             best_loss[take] = loss[take]
             for b in range(B):
-                if not (abs(grads[b, 0]) >= 0):
+                if not (abs(self.last_window_E_grad[b]) >= 0):
                     alive[b] = False
                 if not alive[b]:
                     continue

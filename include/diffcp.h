@@ -199,6 +199,16 @@ int dcp_upload_adjoint(dcp_ctx* ctx, int32_t gslot, const float* vertex_x,
  * instance, instances concatenated).  Used by the bit-exact binning tests. */
 int dcp_binning_keys(dcp_ctx* ctx, int32_t slot, int32_t* keys_out);
 
+/* The scatters' per-patch binning of a slot, for the permutation tests:
+ * out[NP + NE][9] (every point and element particle once, scatter-patch
+ * order) = (patch, lane, instance, class 0..3 = vertex/grasper/cube/element,
+ * natural index within the class and instance, base cell b0, b1, b2, sorted
+ * position in the patch or -1 when the patch's tile exceeds the LDS cap).
+ * The position is a stable counting sort of the base cells in lane order.
+ * Replaces nothing in the reference, which never sorts: its base cells are
+ * the keys of simulation.py:140,155,177,194. */
+int dcp_patch_order(dcp_ctx* ctx, int32_t slot, int32_t* out);
+
 /* Sticky status bits (DCP_STATUS_*); `clear` resets them. */
 int dcp_status(dcp_ctx* ctx, uint32_t* out, int32_t clear);
 

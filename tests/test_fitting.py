@@ -44,3 +44,45 @@ def test_first_epoch_step_has_the_order_of_the_remaining_distance():
         p2, _ = single_parameter_fitting(p, g, lo, hi, 1.0, epochs, 0)
         bound = hi if g < 0 else lo
         assert magnitude_level(abs(p2 - p)) == magnitude_level(abs(bound - p) / epochs)
+
+
+class _FakeEngine:
+    """Two instances; the last window's raw dL/dE of instance 0 is NaN."""
+    B = 2
+
+    def set_params(self, p):
+        self.params = np.array(p)
+
+    def reset(self, slot, **kw):
+        pass
+
+    def set_targets(self, t):
+        pass
+
+    def rollout_grad(self, S, g, K, use_graph=True):
+        grads = np.full((2, 6), 1e-3)
+        if g == 1:
+            grads[0, 0] = np.nan
+        return np.array([1.0, 1.0]), grads, 0.0
+
+    def download_state(self, slot):
+        return {k: np.zeros((2, 3)) for k in ("vertex_x", "grasper_x", "cube_x")}
+
+    def last_final_slot(self):
+        return 0
+
+    def split(self, arr, attr):
+        return [arr[:1], arr[1:]]
+
+
+def test_nan_gradient_of_the_last_window_stops_the_candidate():
+    """parameter_fitting_functions.py:148-150 stops on a NaN garment_E.grad of
+    the last window, which compute_grad's nan_to_num average would hide."""
+    from diffcp_amd.fitting import WindowedFit
+    t = np.zeros((4, 3))
+    drv = WindowedFit(_FakeEngine(), [[t, t], [t, t]], substeps=10)
+    p0 = np.array([[500.0, 0.3, 100.0, 100.0, 0.0, 0.0]] * 2)
+    res = drv.parameters_learning(p0, epochs=3)
+    h = np.stack(res.history)
+    assert (h[:, 0] == p0[0]).all()              # stopped from the first epoch on
+    assert (h[1:, 1, :2] != p0[1, :2]).all()     # the other candidate keeps fitting

@@ -109,3 +109,27 @@ def test_autotune_returns_a_measured_candidate(tmp_path, monkeypatch):
     eng.reset(0)
     loss, grads, ms = eng.rollout_grad(8, 0, 8, use_graph=True)
     assert np.isfinite(loss).all() and np.isfinite(grads).all() and ms > 0
+
+
+def test_set_targets_between_graph_replays():
+    """Cached rollout graphs hold the loss kernels' buffers and sizes: after
+    dcp_set_targets with another target count (here N, 2N, N again) and with
+    new coordinates at the same count, graph replays agree with eager runs."""
+    from diffcp_amd.engine import Engine
+    inst = scene.tiny_instance(n=6, n_grid=32, substeps=12)
+    S = 10
+    rng = np.random.default_rng(0)
+    eng = Engine([inst], max_states=Engine.slots_for(S, S))
+    t1 = inst.vertex_x[rng.choice(inst.n_vertices, 10, replace=False)] + np.array([0.004, 0.012, -0.003])
+    t2 = inst.vertex_x[rng.choice(inst.n_vertices, 20, replace=False)] + np.array([-0.006, 0.01, 0.002])
+    t3 = t2 + np.array([0.002, 0.001, -0.001])
+    for tg in (t1, t2, t3, t1):
+        out = []
+        for graph in (False, True, True):
+            eng.set_targets([tg])
+            eng.reset(0)
+            loss, grads, _ = eng.rollout_grad(S, 0, S, use_graph=graph)
+            out.append((loss.copy(), grads.copy()))
+        for loss, grads in out[1:]:
+            assert np.allclose(loss, out[0][0], rtol=1e-6, atol=0), (len(tg), loss, out[0][0])
+            assert np.allclose(grads[:, :4], out[0][1][:, :4], rtol=1e-4, atol=1e-4 * np.abs(out[0][1][:, :4]).max())

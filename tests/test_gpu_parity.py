@@ -165,11 +165,56 @@ def test_binning_keys_bit_exact():
     exp = np.concatenate([exp_v, exp_o, exp_c, exp_e])
     assert keys.dtype == np.int32 and keys.shape == exp.shape
     assert (keys == exp).all()
-    # the counting-sort order of the binning equals numpy's stable argsort of the linear keys
-    G = inst.n_grid
-    lin = (exp_v[:, 0].astype(np.int64) * G + exp_v[:, 1]) * G + exp_v[:, 2]
-    order = np.argsort(lin, kind="stable")
-    assert (np.diff(lin[order]) >= 0).all()
+
+
+def test_binning_permutation_is_a_stable_argsort():
+    """The scatters' per-patch counting sort, read back through
+    dcp_patch_order: every particle appears once, its base cell equals the
+    bit-exact key, and inside each patch the sorted positions are numpy's
+    stable argsort of the base cells in lane (particle) order."""
+    inst = scene.config_c1().instances[0]
+    N = 30
+    eng = _gpu_rollout(inst, N)
+    keys = eng.binning_keys(N)
+    po = eng.patch_order(N)
+    nv, no, nc = inst.n_vertices, inst.n_grasper, inst.n_cube
+    first = {0: 0, 1: nv, 2: nv + no, 3: nv + no + nc}
+    rows = np.array([first[c] + n for c, n in zip(po[:, 3], po[:, 4])])
+    assert sorted(rows.tolist()) == list(range(len(keys)))
+    assert (po[:, 5:8] == keys[rows]).all()
+    n_patches = 0
+    for p in np.unique(po[:, 0]):
+        sel = po[po[:, 0] == p]
+        sel = sel[np.argsort(sel[:, 1])]
+        assert (sel[:, 1] == np.arange(len(sel))).all()
+        pos = sel[:, 8]
+        assert (pos >= 0).all(), "every patch of this scene fits its LDS tile"
+        lin = (sel[:, 5].astype(np.int64) * 4096 + sel[:, 6]) * 4096 + sel[:, 7]
+        order = np.argsort(lin, kind="stable")
+        expect = np.empty_like(order)
+        expect[order] = np.arange(len(order))
+        assert (pos == expect).all(), p
+        n_patches += 1
+    assert n_patches > 10
+
+
+def test_out_of_grid_particle_sets_the_status_bit():
+    """A vertex moved beyond the last full stencil (base cell > G - 3) is
+    clamped, and the status word reports it (bit 2).  The clamp keeps every
+    grid access inside the instance's grid."""
+    inst = scene.tiny_instance(n=5, n_grid=16, substeps=10)
+    eng = _gpu_rollout(inst, 2)
+    assert eng.status(clear=True) & BAD_STATUS == 0
+    st = eng.download_state(2)
+    st["vertex_x"][0] = [0.99, 0.4, 0.5]        # trunc(0.99 * 16 - 0.5) = 15 > 13
+    eng.upload_state(2, **st)
+    eng.substep(2, 0, 2, 3)
+    assert eng.status() & 2
+    # a clean state clears it again
+    eng.status(clear=True)
+    eng.reset(0)
+    eng.substep(0, 0, 0, 1)
+    assert eng.status() & 2 == 0
 
 
 def test_loss_matches_brute_force():
