@@ -244,3 +244,89 @@ __device__ __forceinline__ void stress_adjoint(const M3<float>& F, const ElemPar
     gparam[2] += gks;
     gparam[3] += gkc;
 }
+
+// ---------------------------------------------------------------------------
+// Live return mapping (optional; simulation.py:408-428 computes it with
+// cf = 0.1 and never writes it back, so it is off by default).  On the R of
+// the Gram-Schmidt QR of the new F, only the third column changes:
+//   ks == 0:       R02 = R12 = 1e-7, R22 = min(R22, 1)
+//   else R22 > 1:  R02 = R12 = 1e-7, R22 = 1
+//   else R22 < 0:  R02 = R12 = 1e-7, R22 = max(R22, -1)
+//   else R22 <= 1: with shear = R02^2 + R12^2, k = ks / kc, when
+//                  k^2 shear - (cf (R22-1)^2)^2 > 0 the pair (R02, R12) is
+//                  scaled by cf (R22-1)^2 / (k sqrt(shear))
+// and F becomes Q R.  cf is the friction parameter.
+// ---------------------------------------------------------------------------
+struct RMState {
+    QRState qr;
+    M3<float> Rm;       // mapped R
+    int branch;         // 0 unchanged, 1 ks == 0, 2 R22 > 1, 3 R22 < 0, 4 shear scaled
+    float k, sq, scale;
+};
+
+__device__ __forceinline__ M3<float> return_map_forward(const M3<float>& F, float ks, float kc, float cf,
+                                                        RMState& S) {
+    qr_forward(F, S.qr);
+    M3<float> R = S.qr.R;
+    const float R02 = R(0, 2), R12 = R(1, 2), R22 = R(2, 2);
+    S.branch = 0;
+    S.k = S.sq = S.scale = 0.f;
+    if (ks == 0.f) {
+        R(0, 2) = 1e-7f; R(1, 2) = 1e-7f; R(2, 2) = fminf(R22, 1.f);
+        S.branch = 1;
+    } else if (R22 > 1.f) {
+        R(0, 2) = 1e-7f; R(1, 2) = 1e-7f; R(2, 2) = 1.f;
+        S.branch = 2;
+    } else if (R22 < 0.f) {
+        R(0, 2) = 1e-7f; R(1, 2) = 1e-7f; R(2, 2) = fmaxf(R22, -1.f);
+        S.branch = 3;
+    } else if (R22 <= 1.f) {
+        const float shear = R02 * R02 + R12 * R12;
+        const float k = ks / kc;
+        const float om = R22 - 1.f;
+        const float c2 = cf * (om * om);
+        if ((k * k) * shear - c2 * c2 > 0.f) {
+            S.sq = sqrtf(shear);
+            S.k = k;
+            S.scale = c2 / (k * S.sq);
+            R(0, 2) = R02 * S.scale;
+            R(1, 2) = R12 * S.scale;
+            S.branch = 4;
+        }
+    }
+    S.Rm = R;
+    return mmul(S.qr.Q, R);
+}
+
+// Given dL/dFm of Fm = Q Rm, accumulate dL/dF and the gradients of ks, kc, cf.
+__device__ __forceinline__ void return_map_adjoint(const M3<float>& F, const RMState& S, float ks, float kc, float cf,
+                                                   const M3<float>& gFm, M3<float>& gF, float& gks, float& gkc,
+                                                   float& gcf) {
+    const M3<float> gQ = mmul(gFm, mT(S.Rm));
+    M3<float> gR = mmul(mT(S.qr.Q), gFm);    // identity on the entries the mapping keeps
+    const float R02 = S.qr.R(0, 2), R12 = S.qr.R(1, 2), R22 = S.qr.R(2, 2);
+    if (S.branch == 1) {
+        gR(0, 2) = 0.f; gR(1, 2) = 0.f;
+        if (!(R22 < 1.f)) gR(2, 2) = 0.f;
+    } else if (S.branch == 2) {
+        gR(0, 2) = 0.f; gR(1, 2) = 0.f; gR(2, 2) = 0.f;
+    } else if (S.branch == 3) {
+        gR(0, 2) = 0.f; gR(1, 2) = 0.f;
+        if (!(R22 > -1.f)) gR(2, 2) = 0.f;
+    } else if (S.branch == 4) {
+        // n = scale u, u = (R02, R12), scale = cf (R22-1)^2 / (k |u|)
+        const float g0 = gR(0, 2), g1 = gR(1, 2);
+        const float gu = g0 * R02 + g1 * R12;               // g . u
+        const float inv_sq2 = 1.f / (S.sq * S.sq);
+        gR(0, 2) = S.scale * (g0 - R02 * gu * inv_sq2);
+        gR(1, 2) = S.scale * (g1 - R12 * gu * inv_sq2);
+        const float om = R22 - 1.f;
+        const float ksq = S.k * S.sq;
+        gR(2, 2) += gu * (2.f * cf * om) / ksq;
+        gcf += gu * (om * om) / ksq;
+        const float gk = -(gu * S.scale) / S.k;            // d n / d k = -n / k
+        gks += gk / kc;
+        gkc += -gk * ks / (kc * kc);
+    }
+    qr_adjoint(F, S.qr, gQ, gR, gF);
+}

@@ -326,12 +326,17 @@ static int launch_p2g(dcp_ctx* c, StateView S, int s, int g, const int* skip) {
     return c->defer_big ? launch_p2g_form<true>(c, S, s, g, skip) : launch_p2g_form<false>(c, S, s, g, skip);
 }
 
-static int launch_prep(dcp_ctx* c, StateView S, StateView Sn, int fresh) {
-    if (c->opt.precision == 1)
-        TIMED_LAUNCH(DCP_KERNEL_PREP, k_prep<float>, dim3(c->grid_elems()), dim3(DCP_WG), 0, c->stream, c->sim, S, Sn, fresh);
-    else
-        TIMED_LAUNCH(DCP_KERNEL_PREP, k_prep<double>, dim3(c->grid_elems()), dim3(DCP_WG), 0, c->stream, c->sim, S, Sn, fresh);
+template <typename PT, bool RM>
+static int launch_prep_form(dcp_ctx* c, StateView S, StateView Sn, int fresh) {
+    TIMED_LAUNCH(DCP_KERNEL_PREP, (k_prep<PT, RM>), dim3(c->grid_elems()), dim3(DCP_WG), 0, c->stream, c->sim, S, Sn,
+                 fresh);
     return 0;
+}
+static int launch_prep(dcp_ctx* c, StateView S, StateView Sn, int fresh) {
+    const bool rm = c->opt.return_mapping != 0;
+    if (c->opt.precision == 1)
+        return rm ? launch_prep_form<float, true>(c, S, Sn, fresh) : launch_prep_form<float, false>(c, S, Sn, fresh);
+    return rm ? launch_prep_form<double, true>(c, S, Sn, fresh) : launch_prep_form<double, false>(c, S, Sn, fresh);
 }
 
 // Points the kernels' element prep arrays at snapshot slot `slot` for the
@@ -413,6 +418,21 @@ static int launch_adjoint_head(dcp_ctx* c, StateView S, StateView G1, StateView 
     return 0;
 }
 
+template <typename PT, bool RM>
+static int launch_elem_adj_form(dcp_ctx* c, StateView S, StateView G1, StateView Gout, int fresh, int s, int g) {
+    TIMED_LAUNCH(DCP_KERNEL_PREP_ADJ, (k_elem_adj<PT, RM>), dim3(c->grid_patches()), dim3(DCP_WG), 0, c->stream, c->sim,
+                 S, G1, Gout, fresh, s, g);
+    return 0;
+}
+static int launch_elem_adj(dcp_ctx* c, StateView S, StateView G1, StateView Gout, int fresh, int s, int g) {
+    const bool rm = c->opt.return_mapping != 0;
+    if (c->opt.precision == 1)
+        return rm ? launch_elem_adj_form<float, true>(c, S, G1, Gout, fresh, s, g)
+                  : launch_elem_adj_form<float, false>(c, S, G1, Gout, fresh, s, g);
+    return rm ? launch_elem_adj_form<double, true>(c, S, G1, Gout, fresh, s, g)
+              : launch_elem_adj_form<double, false>(c, S, G1, Gout, fresh, s, g);
+}
+
 // P2G adjoint of the point classes, the fused element adjoint and, with
 // finalize, the vertex gather of the element corner gradients.  These no
 // longer touch the grid accumulators, so the next adjoint substep's grid can
@@ -422,12 +442,7 @@ static int launch_adjoint_tail(dcp_ctx* c, int s, int g, StateView S, StateView 
     PrepSlot ps(c, prep_saved(c, snap) ? snap : -1);
     const int fresh = (s == 0);
     TIMED_LAUNCH(DCP_KERNEL_P2G_ADJ, k_p2g_adj, dim3(c->grid_patches()), dim3(DCP_WG), 0, c->stream, c->sim, S, Gout, s, g);
-    if (c->opt.precision == 1)
-        TIMED_LAUNCH(DCP_KERNEL_PREP_ADJ, k_elem_adj<float>, dim3(c->grid_patches()), dim3(DCP_WG), 0, c->stream, c->sim, S, G1,
-                     Gout, fresh, s, g);
-    else
-        TIMED_LAUNCH(DCP_KERNEL_PREP_ADJ, k_elem_adj<double>, dim3(c->grid_patches()), dim3(DCP_WG), 0, c->stream, c->sim, S, G1,
-                     Gout, fresh, s, g);
+    if (launch_elem_adj(c, S, G1, Gout, fresh, s, g)) return -1;
     if (finalize)
         TIMED_LAUNCH(DCP_KERNEL_VERTEX_FINAL, k_vertex_final, dim3(c->grid_patches()), dim3(DCP_WG), 0, c->stream, c->sim,
                      Gout);
@@ -688,6 +703,7 @@ static int build(dcp_ctx* c, const dcp_instance_desc* d, int B) {
     SimDev& S = c->sim;
     S.B = B; S.NP = np; S.NE = ne; S.n_patches = c->n_patches;
     S.precision = c->opt.precision; S.svd_grad = c->opt.svd_grad; S.per_elem = c->opt.per_element_moduli;
+    S.return_mapping = c->opt.return_mapping != 0;
     S.max_active = flags;
     c->slot_floats = 15 * (size_t)np + 12 * (size_t)ne;
     c->max_states = std::max(2, c->opt.max_states);
@@ -717,7 +733,7 @@ static int build(dcp_ctx* c, const dcp_instance_desc* d, int B) {
         c->alloc(&S.status, 1) || c->alloc(&S.g_xe, 3 * (size_t)ne) ||
         c->alloc(&S.g_fv, 3 * (size_t)np) ||
         c->alloc(&S.ec_gx, 9 * (size_t)ne) || c->alloc(&S.ec_gv, 3 * (size_t)ne) ||
-        c->alloc(&S.pgrad_patch, 4 * (size_t)c->n_patches) || c->alloc(&S.pgrad_inst, (size_t)DCP_N_PARAMS * B) ||
+        c->alloc(&S.pgrad_patch, (size_t)DCP_PATCH_GRADS * c->n_patches) || c->alloc(&S.pgrad_inst, (size_t)DCP_N_PARAMS * B) ||
         c->alloc(&S.emod_grad, 3 * (size_t)ne) || c->alloc(&S.traj_grad, 3 * (size_t)std::max(1, traj)))
         return -1;
     if (c->alloc(&c->states, c->slot_floats * c->max_states) || c->alloc(&c->adj, c->slot_floats * c->n_adj) ||
@@ -965,7 +981,7 @@ extern "C" int dcp_zero_adjoint(dcp_ctx* c, int32_t gs) {
 
 extern "C" int dcp_zero_param_grads(dcp_ctx* c) {
     if (!c) return set_err("bad args");
-    HIP_OK(hipMemsetAsync(c->sim.pgrad_patch, 0, 4 * sizeof(double) * c->n_patches, c->stream));
+    HIP_OK(hipMemsetAsync(c->sim.pgrad_patch, 0, DCP_PATCH_GRADS * sizeof(double) * c->n_patches, c->stream));
     HIP_OK(hipMemsetAsync(c->sim.pgrad_inst, 0, DCP_N_PARAMS * sizeof(double) * c->B, c->stream));
     HIP_OK(hipMemsetAsync(c->sim.emod_grad, 0, 3 * sizeof(double) * c->NE, c->stream));
     size_t nt = 0;
@@ -994,7 +1010,7 @@ extern "C" int dcp_loss_adjoint(dcp_ctx* c, int32_t slot, int32_t gs, const doub
 }
 
 static int reduce_param_grads(dcp_ctx* c, double* out) {
-    std::vector<double> pp(4 * (size_t)c->n_patches), pi((size_t)DCP_N_PARAMS * c->B);
+    std::vector<double> pp((size_t)DCP_PATCH_GRADS * c->n_patches), pi((size_t)DCP_N_PARAMS * c->B);
     HIP_OK(hipMemcpyAsync(pp.data(), c->sim.pgrad_patch, pp.size() * 8, hipMemcpyDeviceToHost, c->stream));
     HIP_OK(hipMemcpyAsync(pi.data(), c->sim.pgrad_inst, pi.size() * 8, hipMemcpyDeviceToHost, c->stream));
     HIP_OK(hipStreamSynchronize(c->stream));
@@ -1003,7 +1019,9 @@ static int reduce_param_grads(dcp_ctx* c, double* out) {
     for (int p = 0; p < c->n_patches; ++p) {
         const Patch& P = c->patches_h[p];
         if (P.cls != CLS_ELEMENT) continue;
-        for (int k = 0; k < 4; ++k) out[P.inst * DCP_N_PARAMS + k] += pp[4 * (size_t)p + k];
+        // E, nu, shearing, contact, friction (parameter slots 0-3 and 5)
+        for (int k = 0; k < 4; ++k) out[P.inst * DCP_N_PARAMS + k] += pp[DCP_PATCH_GRADS * (size_t)p + k];
+        out[P.inst * DCP_N_PARAMS + 5] += pp[DCP_PATCH_GRADS * (size_t)p + 4];
     }
     return 0;
 }
@@ -1449,6 +1467,32 @@ extern "C" int dcp_rollout(dcp_ctx* c, int32_t S, int32_t g, int32_t use_graph, 
     c->last_final_slot = cur;
     if (final_slot) *final_slot = cur;
     return update_overflow_form(c);
+}
+
+extern "C" int dcp_polar_adjoint(const float* Ft, const float* gR, int32_t n, int32_t precision, int32_t svd_grad,
+                                 float* gF, float* R) {
+    if (!Ft || !gR || !gF || !R || n < 0 || (precision != 0 && precision != 1) || (svd_grad != 0 && svd_grad != 1))
+        return set_err("dcp_polar_adjoint: bad arguments");
+    if (n == 0) return 0;
+    const size_t bytes = 16 * (size_t)n;
+    float* d = nullptr;
+    HIP_OK(hipMalloc(&d, 4 * bytes));
+    float *dF = d, *dg = d + 4 * (size_t)n, *dgF = d + 8 * (size_t)n, *dR = d + 12 * (size_t)n;
+    hipError_t e = hipMemcpy(dF, Ft, bytes, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dg, gR, bytes, hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        const int blocks = (int)std::min<size_t>((n + 255) / 256, 4096);
+        if (precision == 1)
+            hipLaunchKernelGGL(k_polar_adj<float>, dim3(blocks), dim3(256), 0, nullptr, dF, dg, n, svd_grad == 0 ? 0 : 1, dgF, dR);
+        else
+            hipLaunchKernelGGL(k_polar_adj<double>, dim3(blocks), dim3(256), 0, nullptr, dF, dg, n, svd_grad == 0 ? 0 : 1, dgF, dR);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpy(gF, dgF, bytes, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(R, dR, bytes, hipMemcpyDeviceToHost);
+    (void)hipFree(d);
+    if (e != hipSuccess) return set_err(std::string("dcp_polar_adjoint: ") + hipGetErrorString(e));
+    return 0;
 }
 
 extern "C" int dcp_synchronize(dcp_ctx* c) {

@@ -30,6 +30,7 @@ struct ElemKin {
     int i1, i2, i3;
     V3<float> np, xe;
     M3<float> C, M, dn, Dinv, F, d;
+    M3<float> Fr;          // F before the live return mapping (apply_return_map)
 };
 
 __device__ __forceinline__ ElemParams elem_params(const SimDev& sim, const InstDev& I, int e) {
@@ -74,6 +75,17 @@ __device__ __forceinline__ void element_kinematics(const SimDev& sim, const Inst
         const float third = 1.0f / 3.0f;
         K.xe = {third * (x1.x + x2.x + x3.x), third * (x1.y + x2.y + x3.y), third * (x1.z + x2.z + x3.z)};
     }
+}
+
+// Live return mapping of a non-fresh element's F (off by default; see
+// return_map_forward): F = Q return_map(R) with cf = the instance's friction,
+// d = F inverse(Dinv); the unmapped F is kept in K.Fr for the adjoint.
+__device__ __forceinline__ void apply_return_map(const SimDev& sim, const InstDev& I, int e, const ElemParams& prm,
+                                                 ElemKin& K) {
+    RMState rs;
+    K.Fr = K.F;
+    K.F = return_map_forward(K.Fr, prm.ks, prm.kc, I.friction, rs);
+    K.d = mmul(K.F, load_m9(sim.Dinvinv, sim.NE, e));
 }
 
 __device__ __forceinline__ V3<float> particle_x(const SimDev& sim, const StateView& S, const Patch& P, int q) {
@@ -295,7 +307,7 @@ __device__ __forceinline__ void grid_velocity(const InstDev& I, float4 c, int i,
 }
 
 // kernels -------------------------------------------------------------------
-template <typename PT> __global__ void k_prep(SimDev sim, StateView S, StateView Snext, int fresh);
+template <typename PT, bool RM> __global__ void k_prep(SimDev sim, StateView S, StateView Snext, int fresh);
 template <bool DEFER> __global__ void k_p2g(SimDev sim, StateView S, int s, int g, const int* skip);
 __global__ void k_p2g_big(SimDev sim, StateView S, int s, int g, const int* skip);
 __global__ void k_gridop(SimDev sim, int keep, GridSnap save, const int* skip);
@@ -313,9 +325,10 @@ template <bool DEFER> __global__ void k_g2p_adj(SimDev sim, StateView S, StateVi
 __global__ void k_g2p_adj_big(SimDev sim, StateView S, StateView G1, StateView Gout, int pending);
 __global__ void k_gridop_adj(SimDev sim);
 __global__ void k_p2g_adj(SimDev sim, StateView S, StateView Gout, int s, int g);
-template <typename PT>
+template <typename PT, bool RM>
 __global__ void k_elem_adj(SimDev sim, StateView S, StateView G1, StateView Gout, int fresh, int s, int g);
 __global__ void k_vertex_final(SimDev sim, StateView Gout);
+template <typename PT> __global__ void k_polar_adj(const float* Ft, const float* gR, int n, int mode, float* gF, float* R);
 __global__ void k_loss_adj(SimDev sim, StateView S, StateView Gout, const float* targets, const int* toff,
                            const int* tcount, const int* targ, const double* seed);
 __global__ void k_fill_f(float* p, size_t n, float v);

@@ -634,11 +634,12 @@ __global__ __launch_bounds__(DCP_WG, 2) void k_p2g_adj(SimDev sim, StateView S, 
 // element's dL/dC and dL/dn of the state, its corner gradients for the vertex
 // gather, and adds its parameter gradients into acc (or the per-element
 // moduli gradients).
-template <typename PT>
+template <typename PT, bool RM>
 __device__ __forceinline__ void elem_adjoint(const SimDev& sim, const InstDev& I, const StateView& S,
                                              const StateView& G1, const StateView& Gout, int e, int fresh,
                                              const V3<float>& gxe, const V3<float>& gve, const M3<float>& gC0,
-                                             const V3<float>& gn0, const V3<float>& gkv, double acc[4]) {
+                                             const V3<float>& gn0, const V3<float>& gkv,
+                                             double acc[DCP_PATCH_GRADS]) {
     const int NE = sim.NE;
     ElemKin K;
     element_kinematics(sim, I, S, e, fresh, K);
@@ -646,6 +647,7 @@ __device__ __forceinline__ void elem_adjoint(const SimDev& sim, const InstDev& I
     if (sim.per_elem) {
         prm.E = sim.emod[e]; prm.ks = sim.emod[(size_t)NE + e]; prm.kc = sim.emod[(size_t)2 * NE + e];
     }
+    if (RM && !fresh) apply_return_map(sim, I, e, prm, K);
     StressState SS;
     M3<float> P = stress_forward<PT>(K.F, prm, SS);
     const float vd = I.vol_quad * I.dt;
@@ -668,12 +670,20 @@ __device__ __forceinline__ void elem_adjoint(const SimDev& sim, const InstDev& I
     V3<float> gnt = gn0 + G1.n(e);
     gd(0, 2) += gnt.x; gd(1, 2) += gnt.y; gd(2, 2) += gnt.z;
     M3<float> gF = mzero<float>();
-    float gp[4] = {0.f, 0.f, 0.f, 0.f};
+    float gp[DCP_PATCH_GRADS] = {0.f, 0.f, 0.f, 0.f, 0.f};
     stress_adjoint<PT>(K.F, prm, SS, gP, sim.svd_grad, gF, gp);
     M3<float> gdn;
     if (!fresh) {
         M3<float> Dii = load_m9(sim.Dinvinv, NE, e);
         gF = madd(gF, mmul(gd, mT(Dii)));     // d = F Dinv^-1
+        if (RM) {
+            // F = Q return_map(R) of the unmapped F (its QR recomputed here)
+            RMState rs;
+            (void)return_map_forward(K.Fr, prm.ks, prm.kc, I.friction, rs);
+            M3<float> gFr = mzero<float>();
+            return_map_adjoint(K.Fr, rs, prm.ks, prm.kc, I.friction, gF, gFr, gp[2], gp[3], gp[4]);
+            gF = gFr;
+        }
         gdn = mmul(gF, mT(K.Dinv));           // F = d_new Dinv
     } else {
         gdn = gd;                             // F = I is a constant of the fresh state
@@ -705,8 +715,9 @@ __device__ __forceinline__ void elem_adjoint(const SimDev& sim, const InstDev& I
         sim.emod_grad[(size_t)NE + e] += (double)gp[2];
         sim.emod_grad[(size_t)2 * NE + e] += (double)gp[3];
         acc[1] += (double)gp[1];
+        acc[4] += (double)gp[4];
     } else {
-        for (int t = 0; t < 4; ++t) acc[t] += (double)gp[t];
+        for (int t = 0; t < DCP_PATCH_GRADS; ++t) acc[t] += (double)gp[t];
     }
 }
 
@@ -715,12 +726,12 @@ __device__ __forceinline__ void elem_adjoint(const SimDev& sim, const InstDev& I
 // prep adjoint in registers.  Parameter gradients are reduced per patch in
 // FP64 in a fixed order.  Runs after k_p2g_adj (the vertex force-impulse
 // gradients).
-template <typename PT>
+template <typename PT, bool RM>
 __global__ __launch_bounds__(DCP_WG, 2) void k_elem_adj(SimDev sim, StateView S, StateView G1, StateView Gout,
                                                         int fresh, int s, int g) {
     __shared__ int box[6];
     __shared__ float4 gt[DCP_STILE_CAP];
-    __shared__ double red[4][DCP_WG / 64];
+    __shared__ double red[DCP_PATCH_GRADS][DCP_WG / 64];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     if (tid < 3) { box[tid] = 1 << 30; box[3 + tid] = -(1 << 30); }
     __syncthreads();
@@ -733,7 +744,7 @@ __global__ __launch_bounds__(DCP_WG, 2) void k_elem_adj(SimDev sim, StateView S,
         Stencil st;
         PatchBox B;
         const bool tiled = p2g_adj_stage(sim, I, S, P, s, g, act, r, st, box, gt, B);
-        double acc[4] = {0.0, 0.0, 0.0, 0.0};
+        double acc[DCP_PATCH_GRADS] = {0.0, 0.0, 0.0, 0.0, 0.0};
         if (act) {
             P2GAdj o;
             if (tiled) {
@@ -751,17 +762,18 @@ __global__ __launch_bounds__(DCP_WG, 2) void k_elem_adj(SimDev sim, StateView S,
             const V3<float> gve = {r.m * o.gb[0], r.m * o.gb[1], r.m * o.gb[2]};
             const V3<float> gn = {o.gn[0], o.gn[1], o.gn[2]};
             const V3<float> gk = {o.gk[0], o.gk[1], o.gk[2]};
-            elem_adjoint<PT>(sim, I, S, G1, Gout, e, fresh, gxe, gve, mscale(r.m, o.gA), gn, gk, acc);
+            elem_adjoint<PT, RM>(sim, I, S, G1, Gout, e, fresh, gxe, gve, mscale(r.m, o.gA), gn, gk, acc);
         }
-        for (int t = 0; t < 4; ++t) {
+        for (int t = 0; t < DCP_PATCH_GRADS; ++t) {
+            if (!RM && t == 4) continue;   // no friction gradient without the return mapping
             double v = wave_sum_d(acc[t]);
             if (lane == 0) red[t][wv] = v;
         }
         __syncthreads();
-        if (tid < 4) {
+        if (tid < (RM ? DCP_PATCH_GRADS : 4)) {
             double v = 0.0;
             for (int w = 0; w < DCP_WG / 64; ++w) v += red[tid][w];
-            sim.pgrad_patch[(size_t)pi * 4 + tid] += v;
+            sim.pgrad_patch[(size_t)pi * DCP_PATCH_GRADS + tid] += v;
         }
         if (tid < 3) { box[tid] = 1 << 30; box[3 + tid] = -(1 << 30); }
         __syncthreads();
@@ -820,7 +832,29 @@ __global__ __launch_bounds__(DCP_WG) void k_loss_adj(SimDev sim, StateView S, St
     }
 }
 
-template __global__ void k_elem_adj<double>(SimDev, StateView, StateView, StateView, int, int, int);
-template __global__ void k_elem_adj<float>(SimDev, StateView, StateView, StateView, int, int, int);
+// The 2x2 in-plane polar rotation and its adjoint on a batch of matrices,
+// one per thread (the unit entry dcp_polar_adjoint): Ft and gR row-major
+// [n][4]; R = U V^T, gF = d<gR, R>/dFt (mode 0: the reference's clamped SVD
+// backward, 1: the exact polar derivative).
+template <typename PT>
+__global__ void k_polar_adj(const float* Ft, const float* gR, int n, int mode, float* gF, float* R) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const float* a = Ft + 4 * (size_t)i;
+        float c, s;
+        polar2<PT>(a[0], a[1], a[2], a[3], c, s);
+        float g[4] = {gR[4 * (size_t)i], gR[4 * (size_t)i + 1], gR[4 * (size_t)i + 2], gR[4 * (size_t)i + 3]};
+        float out[4] = {0.f, 0.f, 0.f, 0.f};
+        polar2_adj<PT>(a[0], a[1], a[2], a[3], g, mode, out);
+        for (int k = 0; k < 4; ++k) gF[4 * (size_t)i + k] = out[k];
+        R[4 * (size_t)i] = c; R[4 * (size_t)i + 1] = -s; R[4 * (size_t)i + 2] = s; R[4 * (size_t)i + 3] = c;
+    }
+}
+template __global__ void k_polar_adj<double>(const float*, const float*, int, int, float*, float*);
+template __global__ void k_polar_adj<float>(const float*, const float*, int, int, float*, float*);
+
+template __global__ void k_elem_adj<double, false>(SimDev, StateView, StateView, StateView, int, int, int);
+template __global__ void k_elem_adj<float, false>(SimDev, StateView, StateView, StateView, int, int, int);
+template __global__ void k_elem_adj<double, true>(SimDev, StateView, StateView, StateView, int, int, int);
+template __global__ void k_elem_adj<float, true>(SimDev, StateView, StateView, StateView, int, int, int);
 template __global__ void k_g2p_adj<false>(SimDev, StateView, StateView, StateView, int);
 template __global__ void k_g2p_adj<true>(SimDev, StateView, StateView, StateView, int);

@@ -21,7 +21,7 @@
 // ---------------------------------------------------------------------------
 // element prep
 // ---------------------------------------------------------------------------
-template <typename PT>
+template <typename PT, bool RM>
 __global__ __launch_bounds__(DCP_WG) void k_prep(SimDev sim, StateView S, StateView Snext, int fresh) {
     if (blockIdx.x == 0)
         for (int b = threadIdx.x; b < sim.B; b += blockDim.x) sim.n_active[b] = 0;
@@ -30,11 +30,12 @@ __global__ __launch_bounds__(DCP_WG) void k_prep(SimDev sim, StateView S, StateV
         const InstDev I = sim.inst[sim.elem_inst[e]];
         ElemKin K;
         element_kinematics(sim, I, S, e, fresh, K);
+        ElemParams prm = elem_params(sim, I, e);
+        if (RM && !fresh) apply_return_map(sim, I, e, prm, K);
         V3<float> v1 = S.v(K.i1), v2 = S.v(K.i2), v3 = S.v(K.i3);
         const float third = 1.0f / 3.0f;
         V3<float> ve = {third * (v1.x + v2.x + v3.x), third * (v1.y + v2.y + v3.y), third * (v1.z + v2.z + v3.z)};
         V3<float> n = {K.d(0, 2), K.d(1, 2), K.d(2, 2)};
-        ElemParams prm = elem_params(sim, I, e);
         StressState SS;
         // after a rest-shape re-base the first substep's stress keeps the F of
         // the old rest shape (the reference re-bases d and Dinv but not F,
@@ -671,6 +672,7 @@ __global__ __launch_bounds__(DCP_WG) void k_rebase(SimDev sim, StateView S, floa
         const InstDev I = sim.inst[sim.elem_inst[e]];
         ElemKin K;
         element_kinematics(sim, I, S, e, 0, K);
+        if (sim.return_mapping) apply_return_map(sim, I, e, elem_params(sim, I, e), K);
         store_m9(f_old, NE, e, K.F);
         const V3<float> x0 = S.x(K.i1), x1 = S.x(K.i2), x2 = S.x(K.i3);
         const V3<float> d1 = x1 - x0, d2 = x2 - x0;
@@ -742,8 +744,10 @@ __global__ void k_binning_keys(SimDev sim, StateView S, int* keys) {
     }
 }
 
-template __global__ void k_prep<double>(SimDev, StateView, StateView, int);
-template __global__ void k_prep<float>(SimDev, StateView, StateView, int);
+template __global__ void k_prep<double, false>(SimDev, StateView, StateView, int);
+template __global__ void k_prep<float, false>(SimDev, StateView, StateView, int);
+template __global__ void k_prep<double, true>(SimDev, StateView, StateView, int);
+template __global__ void k_prep<float, true>(SimDev, StateView, StateView, int);
 
 template __global__ void k_p2g<false>(SimDev, StateView, int, int, const int*);
 template __global__ void k_p2g<true>(SimDev, StateView, int, int, const int*);

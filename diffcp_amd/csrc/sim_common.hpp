@@ -15,6 +15,7 @@
 #define DCP_TILE_CAP 2048          // LDS tile cells of a P2G patch (4 floats each)
 #define DCP_PATCH_MAX 1024         // particles per patch
 #define DCP_WG 256                 // threads per particle workgroup
+#define DCP_PATCH_GRADS 5          // per-patch parameter gradients: E, nu, shear, contact, friction
 
 enum : int { CLS_VERTEX = 0, CLS_GRASPER = 1, CLS_CUBE = 2, CLS_ELEMENT = 3 };
 
@@ -70,6 +71,7 @@ struct GridSnap {
 struct SimDev {
     int B, NP, NE, n_patches;
     int precision, svd_grad, per_elem, max_active;
+    int return_mapping;            // 1: live element return mapping with cf = friction
     const InstDev* inst;
     const Patch* patches;          // point patches then element patches
     const SPatch* spatches;        // scatter patches (P2G and the G2P adjoint)
@@ -101,7 +103,7 @@ struct SimDev {
     float* g_xe;                   // [3][NE] dL/dx_e of the G2P reverse
     float* g_fv;                   // [3][NP] d/d (vertex force impulse)
     float* ec_gx; float* ec_gv;    // [9][NE] per-corner d/dx; [3][NE] d/dv (one for the three corners)
-    double* pgrad_patch;           // [n_patches][4]  E, nu, shear, contact (element patches)
+    double* pgrad_patch;           // [n_patches][DCP_PATCH_GRADS]  E, nu, shear, contact, friction (element patches)
     double* pgrad_inst;            // [B][DCP_N_PARAMS] (damping, friction via atomics)
     double* emod_grad;             // [3][NE]
     double* traj_grad;             // [frames*grasp][3]

@@ -45,7 +45,7 @@ class Engine:
                  precision: str = "mixed", svd_grad: str = "reference",
                  per_element_moduli: bool = False, patch_size: int = 0, scatter_wgs: int = 0,
                  grid_wgs: int = 0, bin_cells: int = 0, grid_snapshots: bool = True,
-                 stream: Optional[int] = None):
+                 return_mapping: bool = False, stream: Optional[int] = None):
         self.lib = _abi.load()
         self.instances = list(instances)
         self.B = len(self.instances)
@@ -83,7 +83,8 @@ class Engine:
         opt = _abi.Options(max_states=max_states, precision=PRECISION[precision],
                            svd_grad=SVD_GRAD[svd_grad], per_element_moduli=int(per_element_moduli),
                            patch_size=patch_size, scatter_wgs=scatter_wgs, grid_wgs=grid_wgs,
-                           bin_cells=bin_cells, grid_snapshots=_snap_option(grid_snapshots))
+                           bin_cells=bin_cells, grid_snapshots=_snap_option(grid_snapshots),
+                           return_mapping=int(return_mapping))
         ctx = ctypes.c_void_p()
         _abi.check(self.lib.dcp_create(descs, self.B, ctypes.byref(opt), stream, ctypes.byref(ctx)))
         self.ctx = ctx
@@ -315,3 +316,18 @@ class Engine:
         K = checkpoint_every if 0 < checkpoint_every <= n_substeps else n_substeps
         nseg = (n_substeps + K - 1) // K
         return nseg + 1 + K + 1
+
+
+def polar_adjoint(Ft: np.ndarray, gR: np.ndarray, precision: str = "mixed", svd_grad: str = "reference"):
+    """The library's 2x2 polar rotation R and adjoint gF = d<gR, R>/dFt for
+    arrays of shape [n, 2, 2] (FP32; ``precision`` "mixed" computes in FP64)."""
+    lib = _abi.load()
+    F = np.ascontiguousarray(np.asarray(Ft, np.float32).reshape(-1, 2, 2))
+    g = np.ascontiguousarray(np.asarray(gR, np.float32).reshape(-1, 2, 2))
+    if F.shape != g.shape:
+        raise ValueError("Ft and gR must have the same shape")
+    gF = np.zeros_like(F)
+    R = np.zeros_like(F)
+    _abi.check(lib.dcp_polar_adjoint(_fp(F), _fp(g), F.shape[0], PRECISION[precision], SVD_GRAD[svd_grad],
+                                     _fp(gF), _fp(R)))
+    return R, gF

@@ -338,6 +338,30 @@ def precision_stress_instance(n: int = 24, n_grid: int = 64, substeps: int = 120
                          density=HEAVY_DENSITY, lift=0.08, name="c4_small")
 
 
+def large_cloth_reduced_instance(seed: int = 4, substeps: int = 100) -> Instance:
+    """The large-cloth workload's setting (256^3 grid, per-triangle lognormal
+    (E, shearing, contact) with sigma 0.3, areal density 1.0) on a 40x40 cloth
+    of side 0.05 m (vertex spacing 0.33 dx) over a 0.02 m cube."""
+    inst = make_instance(40, 256, substeps, seed=seed, side=0.05, cube_side=0.02, density=HEAVY_DENSITY,
+                         lift=0.02, name="c5_reduced")
+    rng = np.random.default_rng(seed + 17)
+    base = np.array([inst.E, inst.shearing_stiffness, inst.contact_stiffness])
+    inst.element_moduli = base[None, :] * np.exp(rng.normal(0.0, 0.3, size=(inst.n_elements, 3)))
+    return inst
+
+
+def reference_shaped_instance(substeps: int = 620, seed: int = 5) -> Instance:
+    """A synthetic scene shaped like the reference's own (simulation_arguments.json,
+    initialization.py:79-131): a 15^3 grid (dx = 1/15, not a power of two),
+    scene_scaling 0.5 (gravity halved), frames of 300 substeps
+    (operator_update_interval 0.03 s at dt = 1e-4), 150 cube and 2 x 125
+    grasper particles; a 15x15 cloth of side 0.4 m."""
+    inst = make_instance(15, 15, substeps, seed=seed, side=0.4, cube_side=0.15, density=HEAVY_DENSITY,
+                         lift=0.05, steps_per_frame=300, name="reference_shaped")
+    inst.scene_scaling = 0.5
+    return inst
+
+
 CONFIGS = {"c1": config_c1, "c2": config_c2, "c3": config_c3, "c4": config_c4, "c5": config_c5}
 
 

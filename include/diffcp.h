@@ -78,7 +78,9 @@ typedef struct dcp_options {
                                     checkpoint segment for the adjoint instead of recomputing them (memory
                                     permitting); > 0: the same with at most this many 4^3 blocks per substep
                                     (a substep whose grid overflows it is recomputed); -1: always recompute */
-    int32_t reserved;
+    int32_t return_mapping;      /* 0: as the reference, whose element return mapping (simulation.py:408-428)
+                                    is computed and discarded; 1: live: F = Q return_map(R) with cf = the
+                                    friction parameter, and its adjoint (gradients w.r.t. friction) */
 } dcp_options;
 
 /* Build a context: upload B instances, build the internal layout (spatial
@@ -233,6 +235,17 @@ int dcp_last_rollout_ms(dcp_ctx* ctx, double* ms);
  * dcp_rollout_grad / dcp_rollout call: what vertex_x.to_numpy()[max_steps-1]
  * reads for the window carry-over (parameter_fitting_functions.py:40-49). */
 int dcp_last_rollout_final_slot(dcp_ctx* ctx, int32_t* slot);
+
+/* The 2x2 in-plane polar rotation and its adjoint on host arrays, without a
+ * context: for each of n matrices Ft [n][2][2] (row-major) R = U V^T of
+ * its SVD and gF = d<gR, R>/dFt for the given gR [n][2][2], with the
+ * reference's clamped SVD backward (svd_grad 0: exactly zero at
+ * sigma1 = sigma2) or the exact polar derivative (svd_grad 1), computed in
+ * FP64 (precision 0) or FP32 (precision 1) from the FP32 inputs.  Replaces
+ * inplane_svd + backward_inplane_svd / inplane_svd_grad
+ * (simulation.py:39-42, 44-94) for one batch. */
+int dcp_polar_adjoint(const float* Ft, const float* gR, int32_t n, int32_t precision, int32_t svd_grad,
+                      float* gF, float* R);
 
 /* Block until all work queued on the context's stream has finished. */
 int dcp_synchronize(dcp_ctx* ctx);

@@ -34,7 +34,10 @@ LIB_NAMES = {"vertex_x": "vx", "vertex_v": "vv", "vertex_C": "vC", "grasper_x": 
              "element_C": "qC", "element_n": "qn"}
 
 
-def expand(red: Dict[str, torch.Tensor], static, K: O.Consts, fresh: bool) -> Dict[str, torch.Tensor]:
+def expand(red: Dict[str, torch.Tensor], static, K: O.Consts, fresh: bool, params=None) -> Dict[str, torch.Tensor]:
+    """Reference state from the library's; with ``params`` (live return
+    mapping) F is the return-mapped F = Q return_map(R) of the rebuilt one, as
+    update_particle_state would have stored it."""
     tri, Dinv = static["tri"], static["Dinv"]
     vx, vv = red["vx"], red["vv"]
     x1, x2, x3 = vx[tri[:, 0]], vx[tri[:, 1]], vx[tri[:, 2]]
@@ -53,6 +56,13 @@ def expand(red: Dict[str, torch.Tensor], static, K: O.Consts, fresh: bool) -> Di
         col3 = n + (M @ n[..., None])[..., 0]
         dn = O.cols(x2 - x1, x3 - x1, col3)
         F = dn @ Dinv
+        if params is not None:
+            if "elem" in params:
+                ks, kc = params["elem"][:, 1], params["elem"][:, 2]
+            else:
+                ks, kc = params["ks"], params["kc"]
+            Q, R = O.qr_gram_schmidt(F)
+            F = Q @ O.return_map(R, ks, kc, params["friction"])
         d = F @ O.inv3(Dinv)
     st = {k: red[k] for k in POINT_KEYS}
     st.update(qx=qx, qv=qv, qC=C, qF=F, qd=d)
@@ -73,9 +83,9 @@ def initial_reduced(st0: Dict[str, torch.Tensor]) -> Dict[str, torch.Tensor]:
     return out
 
 
-def reduced_substep(red, static, params, K, f, g=0, svd_grad="reference"):
-    cur = expand(red, static, K, fresh=(f == 0))
-    nxt = O.substep(cur, static, params, K, f, g, svd_grad)
+def reduced_substep(red, static, params, K, f, g=0, svd_grad="reference", return_mapping=False):
+    cur = expand(red, static, K, fresh=(f == 0), params=params if return_mapping else None)
+    nxt = O.substep(cur, static, params, K, f, g, svd_grad, return_mapping=return_mapping)
     return reduce(nxt, cur)
 
 

@@ -398,14 +398,50 @@ def damp_element_C(C: torch.Tensor) -> torch.Tensor:
     return C_skew + (1 - 0.95) * C_sym
 
 
+def return_map(R: torch.Tensor, ks, kc, cf) -> torch.Tensor:
+    """The return mapping of update_particle_state, simulation.py:408-428, on
+    the R of the QR of the new F (only the third column changes):
+      ks == 0:          R02 = R12 = 1e-7, R22 = min(R22, 1)
+      else R22 > 1:     R02 = R12 = 1e-7, R22 = 1
+      else R22 < 0:     R02 = R12 = 1e-7, R22 = max(R22, -1)
+      else R22 <= 1:    with shear = R02^2 + R12^2 and k = ks / kc, if
+                        k^2 shear - (cf (R22-1)^2)^2 > 0 the pair (R02, R12)
+                        is scaled by cf (R22-1)^2 / (k sqrt(shear)).
+    The reference computes it with cf = 0.1 and never writes it back (so it
+    changes nothing there); ``update_elements`` applies it only when asked,
+    with cf = the friction parameter."""
+    R02, R12, R22 = R[:, 0, 2], R[:, 1, 2], R[:, 2, 2]
+    one = torch.ones_like(R22)
+    ks, kc, cf = ks * one, kc * one, cf * one
+    b1 = ks == 0
+    b2 = ~b1 & (R22 > 1)
+    b3 = ~b1 & ~b2 & (R22 < 0)
+    b4 = ~b1 & ~b2 & ~b3 & (R22 <= 1)
+    shear = R02 ** 2 + R12 ** 2
+    k = ks / kc
+    act = b4 & ((k ** 2) * shear - ((cf * (R22 - 1) ** 2) ** 2) > 0)
+    safe = torch.where(act, shear, one)
+    scale = cf * ((R22 - 1) ** 2) / (k * torch.sqrt(safe))
+    const = b1 | b2 | b3
+    tiny = torch.full_like(R22, 1e-7)
+    n02 = torch.where(const, tiny, torch.where(act, R02 * scale, R02))
+    n12 = torch.where(const, tiny, torch.where(act, R12 * scale, R12))
+    n22 = torch.where(b1, torch.minimum(R22, one),
+                      torch.where(b2, one, torch.where(b3, torch.maximum(R22, -one), R22)))
+    col3 = torch.stack([n02, n12, n22], -1)
+    return torch.cat([R[:, :, :2], col3[:, :, None]], 2)
+
+
 def update_elements(vx: torch.Tensor, vv: torch.Tensor, C: torch.Tensor, d: torch.Tensor,
-                    Dinv: torch.Tensor, tri: torch.Tensor, dt: float) -> Dict[str, torch.Tensor]:
+                    Dinv: torch.Tensor, tri: torch.Tensor, dt: float, rm=None) -> Dict[str, torch.Tensor]:
     """update_particle_state, simulation.py:386-430, from the new vertex state
     (vx, vv), the new element affine matrix C and the old d: element v and x
     are the vertex means; d = [x2-x1 | x3-x1 | col3 + (dt C + dt^2/2 C C) col3]
     with col3 the old d's normal column; F = d Dinv; the stored d is the
     round trip F inverse(Dinv).  The return mapping of :408-428 changes only a
-    local R that is never written back, so it is left out."""
+    local R that is never written back, so by default it is left out; with
+    rm = (ks, kc, cf) it is applied (F = Q return_map(R)) before F and d are
+    stored."""
     x1, x2, x3 = vx[tri[:, 0]], vx[tri[:, 1]], vx[tri[:, 2]]
     v1, v2, v3 = vv[tri[:, 0]], vv[tri[:, 1]], vv[tri[:, 2]]
     out = {"qv": (1 / 3) * (v1 + v2 + v3),                                        # :392
@@ -415,6 +451,9 @@ def update_elements(vx: torch.Tensor, vv: torch.Tensor, C: torch.Tensor, d: torc
     col3n = col3 + (M @ col3[..., None])[..., 0]                                  # :397-400
     new_d = cols(x2 - x1, x3 - x1, col3n)
     new_F = new_d @ Dinv                                                          # :407
+    if rm is not None:
+        Q, R = qr_gram_schmidt(new_F)                                             # :408
+        new_F = Q @ return_map(R, *rm)
     out["qd"] = new_F @ inv3(Dinv)                                                # :429
     out["qF"] = new_F                                                             # :430
     return out
@@ -441,8 +480,10 @@ def grasper_velocity(static, K: Consts, f: int, g: int, n_op: int, dtype):
 
 
 def substep(st: Dict[str, torch.Tensor], static, params, K: Consts, f: int, g: int = 0,
-            svd_grad: str = "reference", return_grid: bool = False):
-    """forward(s, grad_update), simulation.py:480-488 -> new state dict."""
+            svd_grad: str = "reference", return_grid: bool = False, return_mapping: bool = False):
+    """forward(s, grad_update), simulation.py:480-488 -> new state dict.
+    ``return_mapping``: apply the element return mapping with cf = friction
+    (off in the reference, where it is computed and discarded)."""
     dtype = st["vx"].dtype
     G, dt, dx, inv_dx = K.G, K.dt, K.dx, K.inv_dx
     tri = static["tri"]
@@ -544,7 +585,13 @@ def substep(st: Dict[str, torch.Tensor], static, params, K: Consts, f: int, g: i
     qC1 = damp_element_C(qCn)                                                     # :381-383
 
     # ---- update_particle_state (:386-430) ----
-    new.update(update_elements(new["vx"], new["vv"], qC1, d, Dinv, tri, dt))
+    rm = None
+    if return_mapping:
+        if "elem" in params:
+            rm = (params["elem"][:, 1], params["elem"][:, 2], params["friction"])
+        else:
+            rm = (params["ks"], params["kc"], params["friction"])
+    new.update(update_elements(new["vx"], new["vv"], qC1, d, Dinv, tri, dt, rm))
     new["qC"] = qC1
     if return_grid:
         return new, {"keys": uniq, "m": grid_m, "mv": grid_v, "v_out": v_out, "P": P}
@@ -568,7 +615,7 @@ def compute_loss(vertex_x: torch.Tensor, targets: torch.Tensor):
 
 def rollout(inst, params=None, n_substeps: Optional[int] = None, g: int = 0,
             dtype=torch.float64, svd_grad: str = "reference", keep_states: bool = False,
-            state=None, static=None):
+            state=None, static=None, return_mapping: bool = False):
     """Run ``n_substeps`` forward substeps from the initial state.
 
     Returns (states_list_or_final, static).  The loss of the reference tape is on
@@ -584,16 +631,17 @@ def rollout(inst, params=None, n_substeps: Optional[int] = None, g: int = 0,
     states = [state] if keep_states else None
     st = state
     for f in range(S):
-        st = substep(st, static, params, K, f, g, svd_grad)
+        st = substep(st, static, params, K, f, g, svd_grad, return_mapping=return_mapping)
         if keep_states:
             states.append(st)
     return (states if keep_states else st), static
 
 
 def tape_loss(inst, params, n_substeps: int, targets, g: int = 0, dtype=torch.float64,
-              svd_grad: str = "reference"):
+              svd_grad: str = "reference", return_mapping: bool = False):
     """The loss the reference's Tape records: compute_loss on state S-2."""
-    states, _ = rollout(inst, params, n_substeps, g, dtype, svd_grad, keep_states=True)
+    states, _ = rollout(inst, params, n_substeps, g, dtype, svd_grad, keep_states=True,
+                        return_mapping=return_mapping)
     tgt = torch.as_tensor(np.asarray(targets), dtype=dtype)
     loss, idx = compute_loss(states[n_substeps - 2]["vx"], tgt)
     return loss, states

@@ -17,8 +17,9 @@
 Autograd runs through torch.utils.checkpoint segments (the rollout is
 recomputed segment by segment in the backward pass), so that memory stays
 bounded.  Usage, from the repository root:
-    python tests/golden/make_config_fixtures.py c2|c4|c5
-(c2: about 10 minutes on 8 cores, c4: about 15, c5: about 2)."""
+    python tests/golden/make_config_fixtures.py c2|c2fd|c4|c5
+(c2: about 10 minutes on 8 cores; c2fd, which adds h = 1e-5 differences to
+c2's fixture: 5; c4: about 20; c5: about 1)."""
 import os
 import sys
 import time
@@ -114,6 +115,30 @@ def make_c2():
     return out
 
 
+def refine_c2():
+    """Adds central differences at h = 1e-5 relative to c2_grad_fd.npz (the
+    FD truncation error at 1e-3 and 1e-4 is visible for nu)."""
+    path = os.path.join(HERE, "c2_grad_fd.npz")
+    out = dict(np.load(path))
+    w = scene.config_c2()
+    inst = w.instances[0]
+    S, K = int(out["substeps"]), int(out["checkpoint_every"])
+    fd = []
+    for name in ("E", "nu"):
+        th = float(getattr(inst, name))
+        h = 1e-5 * abs(th)
+        vals = []
+        for sgn in (1.0, -1.0):
+            q = O.make_params(inst)
+            q[name] = torch.tensor(th + sgn * h, dtype=torch.float64)
+            with torch.no_grad():
+                vals.append(float(loss_at(inst, q, S - 2, out["targets"], K)))
+        fd.append((vals[0] - vals[1]) / (2 * h))
+    out["fd_1e-05"] = np.array(fd)
+    np.savez(path, **out)
+    return {k: v for k, v in out.items() if k.startswith("fd") or k == "grad"}
+
+
 def make_c4():
     w = scene.config_c4()
     inst = w.instances[0]
@@ -121,27 +146,17 @@ def make_c4():
     with torch.no_grad():
         _, kept = rollout_segments(inst, O.make_params(inst), S, K, keep=(250, 500))
     targets = gt_targets(inst, S - 2, w.seed * 100)
-    g = grad_and_fd(inst, S, K, targets, rels=())
+    # 5-substep segments: the autograd graph of one substep of this cloth holds
+    # about a gigabyte
+    g = grad_and_fd(inst, S, 5, targets, rels=())
     out = {"x250": kept[250].astype(np.float32), "x500": kept[500].astype(np.float32), "targets": targets,
            "substeps": np.int64(S), "loss": g["loss"], "grad": g["grad"]}
     np.savez_compressed(os.path.join(HERE, "c4_positions.npz"), **out)
     return {k: v for k, v in out.items() if k not in ("x250", "x500", "targets")}
 
 
-def c5_reduced_instance():
-    """The large-cloth workload's setting (256^3 grid, per-triangle lognormal
-    moduli with sigma 0.3, sqrt(S) checkpoints) on a 40x40 cloth of side
-    0.05 m (vertex spacing 0.33 dx), 100 substeps."""
-    inst = scene.make_instance(40, 256, 100, seed=4, side=0.05, cube_side=0.02, density=scene.HEAVY_DENSITY,
-                               lift=0.02, name="c5_reduced")
-    rng = np.random.default_rng(4 + 17)
-    base = np.array([inst.E, inst.shearing_stiffness, inst.contact_stiffness])
-    inst.element_moduli = base[None, :] * np.exp(rng.normal(0.0, 0.3, size=(inst.n_elements, 3)))
-    return inst
-
-
 def make_c5():
-    inst = c5_reduced_instance()
+    inst = scene.large_cloth_reduced_instance()
     S, K = 100, 10
     targets = gt_targets(inst, S - 2, 400, elem_scale=np.array([1.6, 1.0, 1.0]))
     p = O.make_params(inst, requires_grad=True)
@@ -170,5 +185,5 @@ def make_c5():
 if __name__ == "__main__":
     t = time.time()
     which = sys.argv[1]
-    res = {"c2": make_c2, "c4": make_c4, "c5": make_c5}[which]()
+    res = {"c2": make_c2, "c2fd": refine_c2, "c4": make_c4, "c5": make_c5}[which]()
     print(which, {k: v for k, v in res.items() if k != "targets"}, f"{time.time() - t:.0f} s", flush=True)

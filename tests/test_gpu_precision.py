@@ -75,3 +75,48 @@ def test_stress_gradient_vs_fp64(stress, precision, svd_grad):
     assert abs(loss[0] - lo.item()) < 1e-4 * lo.item()
     err = np.abs(g[0, :2] - ref) / np.abs(ref)
     assert (err < 1e-3).all(), (precision, svd_grad, err)
+
+
+def _near_degenerate(delta, n=2000, seed=0):
+    """FP32 2x2 matrices U diag(s1, s2) V^T with s1^2 - s2^2 = delta, s2 in
+    [0.3, 1.2], random rotations U and V (the regime of the 1/(si^2 - sj^2)
+    terms of the SVD backward, simulation.py:71-94)."""
+    rng = np.random.default_rng(seed)
+    s2 = rng.uniform(0.3, 1.2, n)
+    s1 = np.sqrt(s2 ** 2 + delta)
+    a, b = rng.uniform(0, 2 * np.pi, (2, n))
+    rot = lambda t: np.stack([np.stack([np.cos(t), -np.sin(t)], -1), np.stack([np.sin(t), np.cos(t)], -1)], -2)
+    S = np.zeros((n, 2, 2))
+    S[:, 0, 0], S[:, 1, 1] = s1, s2
+    Ft = (rot(a) @ S @ rot(b).transpose(0, 2, 1)).astype(np.float32)
+    gR = rng.standard_normal((n, 2, 2)).astype(np.float32)
+    return Ft, gR
+
+
+@pytest.mark.parametrize("delta", [1e-8, 1e-7, 1e-6, 1e-5])
+def test_polar_adjoint_near_degenerate_stretch(delta):
+    """The library's 2x2 polar adjoint against the oracle's literal SVD
+    backward (the clamped reference formula) and the exact polar derivative,
+    both evaluated in FP64 on the same FP32 inputs.  Bar: 1e-6 relative (to
+    the largest entry) for the FP64 path; the all-FP32 path's error is
+    printed and bounded at 1e-2."""
+    from diffcp_amd.engine import polar_adjoint
+    Ft, gR = _near_degenerate(delta)
+    F64 = torch.as_tensor(Ft, dtype=torch.float64)
+    g64 = torch.as_tensor(gR, dtype=torch.float64)
+    ref = {"reference": O.reference_svd_backward(F64, g64).numpy()}
+    Fg = F64.clone().requires_grad_(True)
+    (O.polar_rotation_closed(Fg) * g64).sum().backward()
+    ref["stable"] = Fg.grad.numpy()
+    R_ref = O.polar_rotation_closed(F64).numpy()
+    for mode in ("reference", "stable"):
+        for precision in ("mixed", "fp32"):
+            R, gF = polar_adjoint(Ft, gR, precision=precision, svd_grad=mode)
+            err = _rel(gF, ref[mode])
+            print(f"delta {delta:.0e} svd_grad {mode:9s} {precision:5s}: gradient rel err {err:.2e}, "
+                  f"rotation rel err {_rel(R, R_ref):.2e}")
+            assert _rel(R, R_ref) < 1e-6
+            if precision == "mixed":
+                assert err < 1e-6, (mode, err)
+            else:
+                assert err < 1e-2, (mode, err)

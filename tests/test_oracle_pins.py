@@ -338,7 +338,35 @@ def pin_grasper(O):
             assert np.abs(v[q] - exp).max() <= 1e-9 * max(1.0, np.abs(exp).max()), (s, g, q, v[q], exp)
 
 
-PINS = {"stress_upper": pin_stress_upper, "stress_inplane": pin_stress_inplane,
+# ----------------------------------------------------------------------------
+# the optional live return mapping (simulation.py:408-428)
+# ----------------------------------------------------------------------------
+def pin_return_map(O):
+    """Branch by branch on literal R's (ks = 100 or 0, kc = 80, cf = 0.5): a
+    compressed element with shear (R02, R12) = (0.3, 0.4), R22 = 0.6 has
+    k = 1.25, k^2 |u|^2 = 0.390625 > (cf (R22-1)^2)^2 = 0.0064, so the shear
+    is scaled by 0.08 / (1.25 * 0.5) = 0.128 to (0.0384, 0.0512)."""
+    base = np.array([[1.1, 0.2, 0.0], [0.0, 0.9, 0.0], [0.0, 0.0, 0.0]])
+    cases = [  # (ks, R02, R12, R22) -> mapped (R02, R12, R22)
+        (100.0, 0.3, 0.4, 0.6, (0.0384, 0.0512, 0.6)),
+        (100.0, 0.01, 0.0, 0.6, (0.01, 0.0, 0.6)),
+        (100.0, 0.3, 0.4, 1.3, (1e-7, 1e-7, 1.0)),
+        (100.0, 0.3, 0.4, 1.0, (0.0, 0.0, 1.0)),       # R22 = 1: the cone has zero width
+        (100.0, 0.3, 0.4, -0.4, (1e-7, 1e-7, -0.4)),
+        (100.0, 0.3, 0.4, -1.5, (1e-7, 1e-7, -1.0)),
+        (0.0, 0.3, 0.4, 0.5, (1e-7, 1e-7, 0.5)),
+        (0.0, 0.3, 0.4, 1.5, (1e-7, 1e-7, 1.0)),
+    ]
+    for ks, r02, r12, r22, exp in cases:
+        R = torch.tensor(base)[None].clone()
+        R[0, 0, 2], R[0, 1, 2], R[0, 2, 2] = r02, r12, r22
+        out = O.return_map(R, _t(ks), _t(80.0), _t(0.5))[0].numpy()
+        want = base.copy()
+        want[0, 2], want[1, 2], want[2, 2] = exp
+        assert np.abs(out - want).max() < 1e-12, (ks, r02, r12, r22, out[:, 2])
+
+
+PINS = {"stress_upper": pin_stress_upper, "return_map": pin_return_map, "stress_inplane": pin_stress_inplane,
         "stress_rotation": pin_stress_rotation, "forces": pin_forces, "grid_op": pin_grid_op,
         "g2p": pin_g2p, "damp": pin_damp, "update": pin_update, "grasper": pin_grasper}
 
@@ -354,7 +382,7 @@ MUTATIONS = [
     ("a12 = ks * R12 * R22", "a12 = ks * R02 * R22", "stress_upper"),
     ("a02 = ks * R02 * R22", "a02 = ks * R12 * R22", "stress_upper"),
     ("(-kc) * ((1 - R22) ** 2)", "(kc) * ((1 - R22) ** 2)", "stress_upper"),
-    ("(R22 <= 1)", "(R22 >= 1)", "stress_upper"),
+    ("a22 = ((R22 <= 1)", "a22 = ((R22 >= 1)", "stress_upper"),
     ("a00 = ks * R02 * R02 ", "a00 = ks * R02 * R02 + E * R[:, 0, 0] * (R[:, 0, 0] - 1) ", "stress_upper"),
     ("la = E * nu / ((1 + nu) * (1 - 2 * nu))", "la = E * nu / ((1 + nu) * (1 - nu))", "stress_upper"),
     ("mu = E / (2 * (1 + nu))", "mu = E / (1 + nu)", "stress_upper"),
@@ -388,6 +416,11 @@ MUTATIONS = [
     ("new_d = cols(x2 - x1, x3 - x1, col3n)", "new_d = cols(x3 - x1, x2 - x1, col3n)", "update"),
     ("\"qv\": (1 / 3) * (v1 + v2 + v3)", "\"qv\": (1 / 3) * (v1 + v2 + v2)", "update"),
     ("out[\"qd\"] = new_F @ inv3(Dinv)", "out[\"qd\"] = new_F @ Dinv", "update"),
+    ("b2 = ~b1 & (R22 > 1)", "b2 = ~b1 & (R22 >= 1)", "return_map"),
+    ("scale = cf * ((R22 - 1) ** 2) / (k * torch.sqrt(safe))", "scale = cf * ((R22 - 1) ** 2) / (k * safe)",
+     "return_map"),
+    ("torch.where(b3, torch.maximum(R22, -one), R22)", "torch.where(b3, R22, R22)", "return_map"),
+    ("    k = ks / kc\n    act", "    k = kc / ks\n    act", "return_map"),
     ("frame = f // K.steps_per_frame + K.frames_per_window * g",
      "frame = (f + 1) // K.steps_per_frame + K.frames_per_window * g", "grasper"),
     ("if frame + 1 < K.n_actions:", "if frame < K.n_actions:", "grasper"),
