@@ -213,6 +213,10 @@ struct dcp_ctx {
     int tev_used = 0;
     bool capturing = false;
     bool defer_big = false;            // listing form of the tile-overflow handling (see launch_p2g)
+    // lean adjoint: every grid snapshot of the segment being walked is known to
+    // hold its substep (read back after the segment's forward), so the adjoint
+    // substeps restore their grids without launching P2G and the grid update
+    bool lean_adjoint = false;
     void drop_timing_events() {
         for (hipEvent_t e : tev) (void)hipEventDestroy(e);
         tev.clear();
@@ -390,6 +394,7 @@ static int launch_adjoint_grid(dcp_ctx* c, int s, int g, StateView S, int snap) 
         // cleared for it
         hipLaunchKernelGGL(k_grid_restore, dim3(c->grid_gridop()), dim3(DCP_WG), 0, c->stream, c->sim, sn);
         LAUNCH_OK();
+        if (c->lean_adjoint && saved) return 0;   // the snapshot is known to hold: nothing to recompute
     }
     if (launch_p2g(c, S, s, g, sn.ok)) return -1;
     if (sn.ok) {
@@ -1282,20 +1287,49 @@ static int record_forward_segment(dcp_ctx* c, const Plan& p, int cseg, int g) {
 }
 
 // adjoint slots ping-pong: gradient of state t lives in adj[(L - t) & 1]
+// Recompute of a segment's states base+1 .. end-1 before its adjoint walk
+// (every segment but the last forward one, whose states are still resident),
+// saving the grids of all its substeps but the last.
+static int record_recompute_segment(dcp_ctx* c, const Plan& p, int cseg, int g) {
+    const int base = cseg * p.K, end = std::min(base + p.K, p.L);
+    const bool grids = c->snap_slots >= p.K;
+    for (int f = base; f + 1 < end; ++f)
+        if (launch_forward(c, f, g, c->state(seg_slot(p, cseg, f)), c->state(seg_slot(p, cseg, f + 1)),
+                           grids ? f - base : -1))
+            return -1;
+    return 0;
+}
+
+// Snapshot slot of adjoint substep f of segment cseg (-1: none): every
+// substep of a resident segment, all but the last one otherwise.
+static int segment_snap(const dcp_ctx* c, const Plan& p, int cseg, int f) {
+    const int base = cseg * p.K, end = std::min(base + p.K, p.L);
+    const bool resident = cseg == p.nseg - 1;
+    return c->snap_slots >= p.K && (resident || f + 1 < end) ? f - base : -1;
+}
+
+// Whether every grid snapshot the adjoint walk of segment cseg reads holds its
+// substep, from the flags the segment's forward left (waits for it).
+static int segment_snapshots_hold(dcp_ctx* c, const Plan& p, int cseg, bool& hold) {
+    hold = false;
+    const int base = cseg * p.K, end = std::min(base + p.K, p.L);
+    if (end <= base || !c->snap_acc || !c->snap_prep) return 0;
+    std::vector<int> ok((size_t)c->snap_slots, 0);
+    HIP_OK(hipMemcpyAsync(ok.data(), c->snap_ok, ok.size() * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIP_OK(hipStreamSynchronize(c->stream));
+    for (int f = base; f < end; ++f) {
+        const int sn = segment_snap(c, p, cseg, f);
+        if (sn >= 0 && !ok[sn]) return 0;
+    }
+    hold = true;
+    return 0;
+}
+
 static int record_backward_segment(dcp_ctx* c, const Plan& p, int cseg, int g) {
     int base = cseg * p.K;
     int end = std::min(base + p.K, p.L);   // adjoint substeps base .. end-1
     if (end <= base) return 0;
-    // recompute states base+1 .. end-1, except in the last forward segment: no
-    // later segment reused its slots, so they still hold those states
-    const bool resident = cseg == p.nseg - 1;
-    const bool grids = c->snap_slots >= p.K;
-    for (int f = base; !resident && f + 1 < end; ++f)
-        if (launch_forward(c, f, g, c->state(seg_slot(p, cseg, f)), c->state(seg_slot(p, cseg, f + 1)),
-                           grids ? f - base : -1))
-            return -1;
-    // grids saved: every substep of a resident segment, all but the last one otherwise
-    auto snap_of = [&](int f) { return grids && (resident || f + 1 < end) ? f - base : -1; };
+    auto snap_of = [&](int f) { return segment_snap(c, p, cseg, f); };
     bool grid_ready = false;   // the grid of substep f was rebuilt on the side stream
     for (int f = end - 1; f >= base; --f) {
         int gin = (p.L - (f + 1)) & 1, gout = (p.L - f) & 1;
@@ -1327,8 +1361,19 @@ static int record_backward_segment(dcp_ctx* c, const Plan& p, int cseg, int g) {
     return 0;
 }
 
+// Graph kinds: forward segment, adjoint walk (full: P2G and the grid update
+// launched on every substep, skipping themselves where the snapshot holds),
+// recompute of a segment's states, lean adjoint walk (no such launches).
+enum { REC_FORWARD = 0, REC_ADJOINT = 1, REC_RECOMPUTE = 2, REC_ADJOINT_LEAN = 3 };
 static int run_recorded(dcp_ctx* c, int kind, int cseg, int g, const Plan& p, int use_graph) {
-    auto rec = [&]() { return kind == 0 ? record_forward_segment(c, p, cseg, g) : record_backward_segment(c, p, cseg, g); };
+    auto rec = [&]() {
+        if (kind == REC_FORWARD) return record_forward_segment(c, p, cseg, g);
+        if (kind == REC_RECOMPUTE) return record_recompute_segment(c, p, cseg, g);
+        c->lean_adjoint = kind == REC_ADJOINT_LEAN;
+        const int r = record_backward_segment(c, p, cseg, g);
+        c->lean_adjoint = false;
+        return r;
+    };
     if (!use_graph) return rec();
     auto key = std::make_tuple(kind, cseg, g, p.S, p.K);
     auto it = c->graphs.find(key);
@@ -1425,12 +1470,16 @@ extern "C" int dcp_rollout_grad(dcp_ctx* c, int32_t S, int32_t g, int32_t K, int
     HIP_OK(hipMemcpyAsync(c->seed_d, c->ones_h.data(), c->B * sizeof(double), hipMemcpyHostToDevice, c->stream));
     if (dcp_zero_param_grads(c)) return -1;
     for (int cs = 0; cs < p.nseg; ++cs)
-        if (run_recorded(c, 0, cs, g, p, use_graph)) return -1;
+        if (run_recorded(c, REC_FORWARD, cs, g, p, use_graph)) return -1;
     HIP_OK(hipMemcpyAsync(loss_out ? loss_out : nullptr, c->loss_d, loss_out ? c->B * sizeof(double) : 0,
                           hipMemcpyDeviceToHost, c->stream));
     int last = (p.L - 1) / p.K;
-    for (int cs = last; cs >= 0; --cs)
-        if (run_recorded(c, 1, cs, g, p, use_graph)) return -1;
+    for (int cs = last; cs >= 0; --cs) {
+        if (cs != p.nseg - 1 && run_recorded(c, REC_RECOMPUTE, cs, g, p, use_graph)) return -1;
+        bool hold = false;
+        if (segment_snapshots_hold(c, p, cs, hold)) return -1;
+        if (run_recorded(c, hold ? REC_ADJOINT_LEAN : REC_ADJOINT, cs, g, p, use_graph)) return -1;
+    }
     HIP_OK(hipEventRecord(c->ev1, c->stream));
     HIP_OK(hipEventSynchronize(c->ev1));
     float ms = 0.f;

@@ -7,7 +7,11 @@
                       1e-4 relative;
   c4_positions.npz    96x96 cloth compressed and folded, 128^3 grid, 500
                       substeps: vertex positions at substeps 250 and 500 and
-                      the gradient w.r.t. E and nu of a loss on state 498;
+                      central differences (h = 1e-4) w.r.t. E and nu of a
+                      loss on state 498;
+  reference_shaped.npz  the 15^3, scene_scaling 0.5, 300-substep-frame
+                      scene: a 320-substep window's loss and gradient by
+                      autograd of the oracle in FP64 and in FP32;
   c5_reduced.npz      40x40 cloth on the 256^3 grid with per-triangle
                       lognormal moduli, 100 substeps: the gradient w.r.t.
                       every triangle's (E, shearing, contact) by autograd and
@@ -17,9 +21,9 @@
 Autograd runs through torch.utils.checkpoint segments (the rollout is
 recomputed segment by segment in the backward pass), so that memory stays
 bounded.  Usage, from the repository root:
-    python tests/golden/make_config_fixtures.py c2|c2fd|c4|c5
+    python tests/golden/make_config_fixtures.py c2|c2fd|c4|c5|refshape
 (c2: about 10 minutes on 8 cores; c2fd, which adds h = 1e-5 differences to
-c2's fixture: 5; c4: about 20; c5: about 1)."""
+c2's fixture: 5; c4: about 20; c5: about 1; refshape: about 12)."""
 import os
 import sys
 import time
@@ -140,17 +144,30 @@ def refine_c2():
 
 
 def make_c4():
+    """Positions from one forward run; the gradient by central differences of
+    forward runs (h = 1e-4 relative; autograd through 500 substeps of this
+    cloth exceeds the memory of the build machine even in checkpoint
+    segments)."""
     w = scene.config_c4()
     inst = w.instances[0]
     S, K = w.substeps, w.checkpoint_every
     with torch.no_grad():
         _, kept = rollout_segments(inst, O.make_params(inst), S, K, keep=(250, 500))
-    targets = gt_targets(inst, S - 2, w.seed * 100)
-    # 5-substep segments: the autograd graph of one substep of this cloth holds
-    # about a gigabyte
-    g = grad_and_fd(inst, S, 5, targets, rels=())
+    targets = gt_targets(inst, S - 2, w.seed * 100) + np.array([0.002, -0.001, 0.0015])
+    with torch.no_grad():
+        loss = float(loss_at(inst, O.make_params(inst), S - 2, targets, K))
+        fd = []
+        for name in ("E", "nu"):
+            th = float(getattr(inst, name))
+            h = 1e-4 * abs(th)
+            vals = []
+            for sgn in (1.0, -1.0):
+                q = O.make_params(inst)
+                q[name] = torch.tensor(th + sgn * h, dtype=torch.float64)
+                vals.append(float(loss_at(inst, q, S - 2, targets, K)))
+            fd.append((vals[0] - vals[1]) / (2 * h))
     out = {"x250": kept[250].astype(np.float32), "x500": kept[500].astype(np.float32), "targets": targets,
-           "substeps": np.int64(S), "loss": g["loss"], "grad": g["grad"]}
+           "substeps": np.int64(S), "loss": np.float64(loss), "fd_0.0001": np.array(fd)}
     np.savez_compressed(os.path.join(HERE, "c4_positions.npz"), **out)
     return {k: v for k, v in out.items() if k not in ("x250", "x500", "targets")}
 
@@ -158,7 +175,9 @@ def make_c4():
 def make_c5():
     inst = scene.large_cloth_reduced_instance()
     S, K = 100, 10
-    targets = gt_targets(inst, S - 2, 400, elem_scale=np.array([1.6, 1.0, 1.0]))
+    # ground-truth samples moved by ~1 mm, so that the loss is far above the
+    # FP32 position resolution of a 1/256 grid
+    targets = gt_targets(inst, S - 2, 400, elem_scale=np.array([1.6, 1.0, 1.0])) + np.array([0.001, 0.0005, -0.001])
     p = O.make_params(inst, requires_grad=True)
     loss = loss_at(inst, p, S - 2, targets, K)
     loss.backward()
@@ -182,8 +201,29 @@ def make_c5():
     return {k: v for k, v in out.items() if k in ("loss", "fd_dir", "ad_dir")}
 
 
+def make_refshape():
+    """The reference-shaped scene (15^3 grid, scene_scaling 0.5, 300-substep
+    frames): targets 4-6 mm from the state of substep 318, the loss of a
+    320-substep window and its gradient w.r.t. E, nu, shearing and contact
+    stiffness by autograd of the oracle in FP64 and in FP32."""
+    inst = scene.reference_shaped_instance()
+    S = 320
+    with torch.no_grad():
+        st, _ = O.rollout(inst, n_substeps=S - 2)
+    targets = st["vx"].numpy() + np.array([0.004, 0.006, -0.003])
+    out = {"targets": targets, "substeps": np.int64(S)}
+    for tag, dt in (("fp64", torch.float64), ("fp32", torch.float32)):
+        p = O.make_params(inst, dtype=dt, requires_grad=True)
+        loss, _ = O.tape_loss(inst, p, S, targets, dtype=dt)
+        loss.backward()
+        out[f"loss_{tag}"] = np.float64(loss.detach())
+        out[f"grad_{tag}"] = np.array([float(p[k].grad) for k in ("E", "nu", "ks", "kc")])
+    np.savez(os.path.join(HERE, "reference_shaped.npz"), **out)
+    return {k: v for k, v in out.items() if k != "targets"}
+
+
 if __name__ == "__main__":
     t = time.time()
     which = sys.argv[1]
-    res = {"c2": make_c2, "c2fd": refine_c2, "c4": make_c4, "c5": make_c5}[which]()
+    res = {"c2": make_c2, "c2fd": refine_c2, "c4": make_c4, "c5": make_c5, "refshape": make_refshape}[which]()
     print(which, {k: v for k, v in res.items() if k != "targets"}, f"{time.time() - t:.0f} s", flush=True)

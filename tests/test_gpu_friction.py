@@ -6,7 +6,8 @@ d loss / d friction, against the oracle run with the same mapping.
 
 Tolerances as in test_gpu_parity.py: 2e-5 relative on positions and normals,
 5e-3 on velocities and affine matrices, 2e-4 on substep state gradients,
-2e-3 on parameter gradients.
+2e-3 on parameter gradients as loss sensitivities (see _params_close; the
+FP32 floor is 1e-5 of the largest sensitivity).
 """
 import numpy as np
 import pytest
@@ -29,6 +30,19 @@ def _engine(insts, **kw):
 def _rel(a, b):
     a, b = np.asarray(a, np.float64), np.asarray(b, np.float64)
     return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-30))
+
+
+def _params_close(g, ref, theta, rtol=2e-3, floor=1e-5):
+    """Parameter gradients g[j] against ref[j], compared as loss sensitivities
+    theta_j g_j (theta_j = max(|parameter|, 1)): within rtol of each, plus
+    floor times the largest sensitivity.  The floor is the FP32 limit: a
+    component whose sensitivity is orders below the others (the shearing
+    stiffness here, ~1e-7 of E's) differs by ~10 % between the FP32 and FP64
+    oracles themselves."""
+    th = np.maximum(np.abs(np.asarray(theta, np.float64)), 1.0)
+    g, ref = np.asarray(g, np.float64) * th, np.asarray(ref, np.float64) * th
+    bar = rtol * np.abs(ref) + floor * np.abs(ref).max()
+    assert (np.abs(g - ref) <= bar).all(), (g / th, ref / th, np.abs(g - ref) / np.maximum(np.abs(ref), 1e-300))
 
 
 def _scene():
@@ -92,10 +106,8 @@ def test_return_mapping_adjoint_substep(svd_grad):
             ref = red[key].grad
             ref = torch.zeros_like(red[key]) if ref is None else ref
             assert _rel(g[name], ref.numpy()) < 2e-4, (f, name, _rel(g[name], ref.numpy()))
-        for j, pn in enumerate(PARAMS):
-            ref = params[pn].grad.item()
-            assert abs(pg[j] - ref) <= 2e-3 * abs(ref) + 1e-12, (f, pn, pg[j], ref)
-        assert abs(params["friction"].grad.item()) > 0
+        _params_close(pg, [params[pn].grad.item() for pn in PARAMS], inst.params())
+        assert abs(params["friction"].grad.item()) > 0 and abs(pg[5]) > 0
 
 
 def test_return_mapping_window_gradient_matches_oracle():
@@ -114,11 +126,7 @@ def test_return_mapping_window_gradient_matches_oracle():
     ref_loss, _ = O.tape_loss(inst, p, S, targets, svd_grad="stable", return_mapping=True)
     ref_loss.backward()
     assert abs(loss[0] - ref_loss.item()) <= 1e-4 * ref_loss.item()
-    for j, pn in enumerate(PARAMS):
-        if pn == "damping":
-            continue
-        ref = p[pn].grad.item()
-        assert abs(grads[0, j] - ref) <= 2e-3 * abs(ref), (pn, grads[0, j], ref)
+    _params_close(grads[0], [p[pn].grad.item() for pn in PARAMS], inst.params())
     assert abs(grads[0, 5]) > 0
     # without the mapping the friction gradient is exactly zero, as in the reference
     eng0 = _engine([inst], max_states=Engine.slots_for(S, 10), svd_grad="stable")
