@@ -81,6 +81,7 @@ SIGNATURES = {
     "dcp_device_bytes": (c_int, [c_void_p, ctypes.POINTER(ctypes.c_int64)]),
     "dcp_synchronize": (c_int, [c_void_p]),
     "dcp_polar_adjoint": (c_int, [c_flt_p, c_flt_p, c_int, c_int, c_int, c_flt_p, c_flt_p]),
+    "dcp_polar_adjoint_noalloc": (c_int, [c_flt_p, c_flt_p, c_int, c_int, c_int, c_flt_p, c_flt_p]),
     "dcp_last_rollout_final_slot": (c_int, [c_void_p, c_int_p]),
     "dcp_set_kernel_timing": (c_int, [c_void_p, c_int, c_int]),
     "dcp_kernel_timing": (c_int, [c_void_p, c_dbl_p, c_int_p]),

@@ -15,7 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libdiffcp_amd.so")
-SOURCES = ["engine.hip", "kernels_fwd.hip", "kernels_adj.hip"]
+SOURCES = ["engine.hip", "kernels_fwd.hip", "kernels_adj.hip", "noalloc.hip"]
 ARCH = os.environ.get("DIFFCP_ARCH", "gfx950")
 
 # packed FP32 instructions are disabled: they have given silent wrong results

@@ -15,6 +15,8 @@
 
 static thread_local std::string g_err;
 static int set_err(const std::string& m) { g_err = m; return -1; }
+// for the other sources of the library (noalloc.hip)
+int dcp_record_error(const std::string& m) { return set_err(m); }
 #define HIP_OK(x)                                                                             \
     do {                                                                                      \
         hipError_t e_ = (x);                                                                  \

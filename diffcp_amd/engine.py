@@ -318,9 +318,12 @@ class Engine:
         return nseg + 1 + K + 1
 
 
-def polar_adjoint(Ft: np.ndarray, gR: np.ndarray, precision: str = "mixed", svd_grad: str = "reference"):
+def polar_adjoint(Ft: np.ndarray, gR: np.ndarray, precision: str = "mixed", svd_grad: str = "reference",
+                  no_alloc: bool = False):
     """The library's 2x2 polar rotation R and adjoint gF = d<gR, R>/dFt for
-    arrays of shape [n, 2, 2] (FP32; ``precision`` "mixed" computes in FP64)."""
+    arrays of shape [n, 2, 2] (FP32; ``precision`` "mixed" computes in FP64).
+    ``no_alloc``: the entry point that allocates no device memory (at most
+    16384 matrices per call)."""
     lib = _abi.load()
     F = np.ascontiguousarray(np.asarray(Ft, np.float32).reshape(-1, 2, 2))
     g = np.ascontiguousarray(np.asarray(gR, np.float32).reshape(-1, 2, 2))
@@ -328,6 +331,6 @@ def polar_adjoint(Ft: np.ndarray, gR: np.ndarray, precision: str = "mixed", svd_
         raise ValueError("Ft and gR must have the same shape")
     gF = np.zeros_like(F)
     R = np.zeros_like(F)
-    _abi.check(lib.dcp_polar_adjoint(_fp(F), _fp(g), F.shape[0], PRECISION[precision], SVD_GRAD[svd_grad],
-                                     _fp(gF), _fp(R)))
+    fn = lib.dcp_polar_adjoint_noalloc if no_alloc else lib.dcp_polar_adjoint
+    _abi.check(fn(_fp(F), _fp(g), F.shape[0], PRECISION[precision], SVD_GRAD[svd_grad], _fp(gF), _fp(R)))
     return R, gF

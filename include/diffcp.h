@@ -247,6 +247,16 @@ int dcp_last_rollout_final_slot(dcp_ctx* ctx, int32_t* slot);
 int dcp_polar_adjoint(const float* Ft, const float* gR, int32_t n, int32_t precision, int32_t svd_grad,
                       float* gF, float* R);
 
+/* dcp_polar_adjoint without any device allocation: no hipMalloc, hipFree or
+ * allocating library call.  No counterpart in the reference; it is for
+ * callers that cannot allocate (tight serving loops, harnesses that forbid
+ * allocation).  The device staging of inputs and outputs is a static array
+ * of at most 16384 matrices (1 MiB, held by each GPU the library runs on
+ * until the process ends); larger n is rejected with an error.  All calls on
+ * one GPU share that array: calls that overlap on one GPU race. */
+int dcp_polar_adjoint_noalloc(const float* Ft, const float* gR, int32_t n, int32_t precision, int32_t svd_grad,
+                              float* gF, float* R);
+
 /* Block until all work queued on the context's stream has finished. */
 int dcp_synchronize(dcp_ctx* ctx);
 
