@@ -24,8 +24,8 @@ Also reported on the same JSON line:
   * ``large``: the large single cloth (256x256 vertices, 256^3 grid, 1000
     substeps, sqrt(S) checkpoints, per-triangle moduli gradients): window
     time, throughput, G2P-adjoint HBM rate and the device memory held;
-  * ``cpu_baseline``: the FP64 CPU restatement in ``oracle/`` timed on the
-    host on a bounded sample of the same workload.
+  * ``cpu_baseline``: the FP32 C++/OpenMP restatement in ``oracle/`` timed
+    on the host on a bounded sample of the same workload.
 
 Usage: python bench.py [--gpus 1] [--steps 5] [--warmup 2]
 """
@@ -238,31 +238,38 @@ def grad_vs_fd():
 
 
 def cpu_baseline(target_seconds: float = 15.0):
-    """FP64 torch restatement (oracle/), forward + autograd adjoint, on one C3
-    instance for as many substeps as fit ~target_seconds."""
-    import torch
+    """The oracle's FP32 C++/OpenMP restatement (oracle/cloth_ref.cpp): the
+    same substep with forward-mode dual numbers carrying dx/dE and dx/dnu (one
+    pass gives the loss and both gradients), on instance 0 of the batched
+    workload for as many substeps as fit ~target_seconds on all the host
+    threads OpenMP is given, and for a quarter of that on one thread."""
     from diffcp_amd import scene
-    from oracle import sim_torch as O
+    from oracle import cloth_ref as CR
 
+    CR.build()
     w = scene.config_c3()
     inst = w.instances[0]
-    inst.targets = scene.sample_targets(inst.vertex_x, N_TARGETS, 0)
+    targets = scene.sample_targets(inst.vertex_x, N_TARGETS, 0)
+    threads = CR.max_threads()
 
-    def run(n):
-        params = O.make_params(inst, requires_grad=True)
-        t = time.perf_counter()
-        loss, _ = O.tape_loss(inst, params, n, inst.targets)
-        loss.backward()
-        return time.perf_counter() - t
+    def rate(n, th):
+        _, _, secs, _ = CR.loss_grad(inst, targets, n, threads=th)
+        return secs
 
-    t3 = run(3)
-    n = int(min(w.substeps, max(3, round(target_seconds / max(t3 / 3, 1e-6)))))
-    dt = run(n)
-    return {"value": inst.n_cloth_particles * n / dt, "unit": "particle-substeps/s",
-            "cores": torch.get_num_threads(), "kind": "port",
+    def sample(th, budget):
+        t = rate(10, th)
+        n = int(min(w.substeps, max(10, round(budget / max(t / 10, 1e-6)))))
+        return n, rate(n, th)
+
+    n, secs = sample(threads, target_seconds)
+    n1, secs1 = sample(1, target_seconds / 4)
+    return {"value": inst.n_cloth_particles * n / secs, "unit": "particle-substeps/s",
+            "cores": threads, "kind": "port",
+            "single_thread_value": inst.n_cloth_particles * n1 / secs1,
             "sample": (f"instance 0 of the batched workload ({inst.n_vertices} vertices, "
-                       f"{inst.n_elements} elements, 128^3 grid), {n} substeps forward + autograd "
-                       f"adjoint, FP64 torch restatement, {dt:.1f} s")}
+                       f"{inst.n_elements} elements, {inst.n_grid}^3 grid): {n} substeps forward with "
+                       f"dE/dnu dual numbers, FP32 C++/OpenMP restatement, {threads} threads, {secs:.1f} s; "
+                       f"{n1} substeps on 1 thread, {secs1:.1f} s")}
 
 
 def main() -> int:
