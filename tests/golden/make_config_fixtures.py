@@ -7,8 +7,8 @@
                       1e-4 relative;
   c4_positions.npz    96x96 cloth compressed and folded, 128^3 grid, 500
                       substeps: vertex positions at substeps 250 and 500 and
-                      central differences (h = 1e-4) w.r.t. E and nu of a
-                      loss on state 498;
+                      central differences (h = 1e-4 and 1e-5) w.r.t. E and
+                      nu of a loss on state 498;
   reference_shaped.npz  the 15^3, scene_scaling 0.5, 300-substep-frame
                       scene: a 320-substep window's loss and gradient by
                       autograd of the oracle in FP64 and in FP32;
@@ -21,9 +21,10 @@
 Autograd runs through torch.utils.checkpoint segments (the rollout is
 recomputed segment by segment in the backward pass), so that memory stays
 bounded.  Usage, from the repository root:
-    python tests/golden/make_config_fixtures.py c2|c2fd|c4|c5|refshape
+    python tests/golden/make_config_fixtures.py c2|c2fd|c4|c4fd|c5|refshape
 (c2: about 10 minutes on 8 cores; c2fd, which adds h = 1e-5 differences to
-c2's fixture: 5; c4: about 20; c5: about 1; refshape: about 12)."""
+c2's fixture: 5; c4: about 30; c4fd, the same for c4: 30; c5: about 1;
+refshape: about 1)."""
 import os
 import sys
 import time
@@ -119,14 +120,8 @@ def make_c2():
     return out
 
 
-def refine_c2():
-    """Adds central differences at h = 1e-5 relative to c2_grad_fd.npz (the
-    FD truncation error at 1e-3 and 1e-4 is visible for nu)."""
-    path = os.path.join(HERE, "c2_grad_fd.npz")
-    out = dict(np.load(path))
-    w = scene.config_c2()
-    inst = w.instances[0]
-    S, K = int(out["substeps"]), int(out["checkpoint_every"])
+def fd_1e5(inst, S, K, targets):
+    """Central differences at h = 1e-5 relative of the loss on state S - 2."""
     fd = []
     for name in ("E", "nu"):
         th = float(getattr(inst, name))
@@ -136,11 +131,30 @@ def refine_c2():
             q = O.make_params(inst)
             q[name] = torch.tensor(th + sgn * h, dtype=torch.float64)
             with torch.no_grad():
-                vals.append(float(loss_at(inst, q, S - 2, out["targets"], K)))
+                vals.append(float(loss_at(inst, q, S - 2, targets, K)))
         fd.append((vals[0] - vals[1]) / (2 * h))
-    out["fd_1e-05"] = np.array(fd)
+    return np.array(fd)
+
+
+def refine_c2():
+    """Adds central differences at h = 1e-5 relative to c2_grad_fd.npz (the
+    FD truncation error at 1e-3 and 1e-4 is visible for nu)."""
+    path = os.path.join(HERE, "c2_grad_fd.npz")
+    out = dict(np.load(path))
+    inst = scene.config_c2().instances[0]
+    out["fd_1e-05"] = fd_1e5(inst, int(out["substeps"]), int(out["checkpoint_every"]), out["targets"])
     np.savez(path, **out)
     return {k: v for k, v in out.items() if k.startswith("fd") or k == "grad"}
+
+
+def refine_c4():
+    """Adds central differences at h = 1e-5 relative to c4_positions.npz."""
+    path = os.path.join(HERE, "c4_positions.npz")
+    out = dict(np.load(path))
+    w = scene.config_c4()
+    out["fd_1e-05"] = fd_1e5(w.instances[0], int(out["substeps"]), w.checkpoint_every, out["targets"])
+    np.savez_compressed(path, **out)
+    return {k: v for k, v in out.items() if k.startswith("fd") or k == "loss"}
 
 
 def make_c4():
@@ -225,5 +239,6 @@ def make_refshape():
 if __name__ == "__main__":
     t = time.time()
     which = sys.argv[1]
-    res = {"c2": make_c2, "c2fd": refine_c2, "c4": make_c4, "c5": make_c5, "refshape": make_refshape}[which]()
+    res = {"c2": make_c2, "c2fd": refine_c2, "c4": make_c4, "c4fd": refine_c4, "c5": make_c5,
+           "refshape": make_refshape}[which]()
     print(which, {k: v for k, v in res.items() if k != "targets"}, f"{time.time() - t:.0f} s", flush=True)

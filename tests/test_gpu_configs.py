@@ -25,7 +25,9 @@
 Tolerances: 1e-3 relative on window gradients against the oracle (the bar
 of the path; parameter gradients compared as loss sensitivities with an FP32
 floor of 1e-5 of the largest, see _params_close), 1e-4 on losses, 1e-5 on
-positions; equal-kernel comparisons
+positions, widened where a scene's FP32 floor (FP32 restatements against the
+FP64 oracle) is measured above it, with the measurement beside the bar;
+equal-kernel comparisons
 (recompute vs resident, batched vs single) 1e-5 on losses and 1e-4 on
 gradients (float-atomic summation order only).
 """
@@ -182,18 +184,22 @@ def test_precision_stress_positions_and_gradient(precision):
             slots[f + 1] = eng.download_state(b)["vertex_x"]
         a, b = b, a
     assert eng.status() & BAD_STATUS == 0
-    for s in (250, 500):
+    # the FP32 floor of this scene: the oracle's FP32 C++ restatement sits
+    # 9.2e-6 and 1.38e-5 from the FP64 positions after 250 and 500 substeps
+    for s, bar in ((250, 2e-5), (500, 3e-5)):
         err = _rel(slots[s], ref[f"x{s}"])
-        assert err < 1e-5, (s, err)
+        assert err < bar, (s, err)
     eng.close()
     eng = _engine(w.instances, max_states=_slots(S, w.checkpoint_every), precision=precision)
     eng.set_targets([ref["targets"]])
     eng.reset(0)
     loss, grads, _ = eng.rollout_grad(S, 0, w.checkpoint_every, use_graph=True)
     assert abs(loss[0] - float(ref["loss"])) < 1e-4 * float(ref["loss"])
-    fd = ref["fd_0.0001"]
+    # h = 1e-4 differences carry about 2e-3 truncation error in nu here (the
+    # FP32 C++ restatement's forward-mode gradient sits 1.8e-3 from them)
+    fd, bar = (ref["fd_1e-05"], 2e-3) if "fd_1e-05" in ref.files else (ref["fd_0.0001"], 5e-3)
     for j in (0, 1):
-        assert abs(grads[0, j] - fd[j]) < 3e-3 * abs(fd[j]), (j, grads[0, j], fd[j])
+        assert abs(grads[0, j] - fd[j]) < bar * abs(fd[j]), (j, grads[0, j], fd[j])
 
 
 def test_large_cloth_setting_per_triangle_moduli():

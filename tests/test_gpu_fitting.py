@@ -1,7 +1,7 @@
 """Windowed fitting on the HIP path: the carry-over of compute_grad against the
 substep-by-substep drop-in module (Tape windows with initial_*.from_numpy +
-initialize_ti_field, parameter_fitting_functions.py:20-49), and a short fit
-that must lower the loss.
+initialize_ti_field, parameter_fitting_functions.py:20-49) and against the
+FP64 oracle window by window, and a short fit that must lower the loss.
 
 Tolerance between the two paths: 1e-5 relative on the loss, 5e-5 on the
 gradients.  The same kernels run; only float-atomic summation order differs,
@@ -62,6 +62,55 @@ def test_compute_grad_carry_over_matches_tape_windows():
     ref = np.mean(gE, axis=0)
     assert np.abs(grads[0, :2] - ref).max() <= 5e-5 * np.abs(ref).max()
     s.engine.close()
+
+
+def test_compute_grad_carry_over_matches_oracle():
+    """Two instances (different E, nu, damping), two windows: window 1 starts
+    from window 0's final positions with zero velocities and a fresh element
+    state (initial_*_x.from_numpy + initialize_ti_field,
+    parameter_fitting_functions.py:40-49) and its own grasp trajectory slice.
+    The FP64 oracle repeats the same: autograd per window, the carried
+    positions constants.  Bars: 1e-4 on the mean loss, 1e-3 on the mean
+    E and nu gradients."""
+    import torch
+
+    from diffcp_amd.engine import Engine
+    from diffcp_amd.fitting import WindowedFit
+    from oracle import sim_torch as O
+
+    insts = [_instance(E=5e3), _instance(E=9e3)]
+    insts[1].nu, insts[1].damping = 0.35, 1.5
+    tg = [[_targets(insts[0], 1), _targets(insts[1], 3)], [_targets(insts[0], 2), _targets(insts[1], 4)]]
+    eng = Engine(insts, max_states=Engine.slots_for(WINDOW, WINDOW))
+    fit = WindowedFit(eng, tg, WINDOW)
+    fit.initialize_objects()
+    loss, grads = fit.compute_grad()
+    eng.close()
+    for b, inst in enumerate(insts):
+        cur = inst.copy()
+        # particle volumes and masses stay those of the creation mesh
+        # (initialization.py:113-131 runs once), not of the carried positions
+        cur.area = inst.surface_area
+        losses, gs = [], []
+        for g in range(2):
+            params = O.make_params(cur, requires_grad=True)
+            l, states = O.tape_loss(cur, params, WINDOW, tg[g][b], g=g)
+            l.backward()
+            losses.append(l.item())
+            gs.append([params["E"].grad.item(), params["nu"].grad.item()])
+            last = states[WINDOW]
+            cur = cur.copy()
+            cur.vertex_x = last["vx"].detach().numpy().copy()
+            cur.grasper_x = last["ox"].detach().numpy().copy()
+            cur.cube_x = last["cx"].detach().numpy().copy()
+        ref_l, ref_g = np.mean(losses), np.mean(gs, axis=0)
+        assert abs(loss[b] - ref_l) <= 1e-4 * abs(ref_l), (b, loss[b], ref_l)
+        # per component, with a floor of 1e-4 of the larger loss sensitivity
+        # (theta dL/dtheta) for a mean that cancels between the windows
+        theta = np.array([inst.E, inst.nu])
+        floor = 1e-4 * np.abs(ref_g * theta).max() / theta
+        for j in (0, 1):
+            assert abs(grads[b, j] - ref_g[j]) <= 1e-3 * abs(ref_g[j]) + floor[j], (b, j, grads[b, j], ref_g[j])
 
 
 def test_short_fit_moves_E_toward_the_ground_truth():

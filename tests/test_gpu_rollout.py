@@ -144,7 +144,7 @@ def test_per_element_moduli_gradient_matches_oracle(svd_grad):
     lo, _ = O.compute_loss(states[S - 2]["vx"], torch.as_tensor(inst.targets))
     lo.backward()
     ref = params["elem"].grad.numpy()
-    assert abs(loss[0] - float(lo)) < 1e-4 * float(lo)
+    assert abs(loss[0] - lo.item()) < 1e-4 * lo.item()
     assert np.abs(ref).max() > 0
     for c in range(3):                     # E, shearing, contact columns
         scale = np.abs(ref[:, c]).max()
