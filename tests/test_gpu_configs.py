@@ -195,11 +195,12 @@ def test_precision_stress_positions_and_gradient(precision):
     eng.reset(0)
     loss, grads, _ = eng.rollout_grad(S, 0, w.checkpoint_every, use_graph=True)
     assert abs(loss[0] - float(ref["loss"])) < 1e-4 * float(ref["loss"])
-    # h = 1e-4 differences carry about 2e-3 truncation error in nu here (the
-    # FP32 C++ restatement's forward-mode gradient sits 1.8e-3 from them)
-    fd, bar = (ref["fd_1e-05"], 2e-3) if "fd_1e-05" in ref.files else (ref["fd_0.0001"], 5e-3)
+    # FP32 floor of this window: the oracle's FP32 C++ restatement (forward-mode
+    # tangents) sits 0.9e-3 (E) and 1.8e-3 (nu) from the FP64 differences; the
+    # h = 1e-4 and 1e-5 differences agree to 7e-5
+    fd = ref["fd_1e-05"] if "fd_1e-05" in ref.files else ref["fd_0.0001"]
     for j in (0, 1):
-        assert abs(grads[0, j] - fd[j]) < bar * abs(fd[j]), (j, grads[0, j], fd[j])
+        assert abs(grads[0, j] - fd[j]) < 5e-3 * abs(fd[j]), (j, grads[0, j], fd[j])
 
 
 def test_large_cloth_setting_per_triangle_moduli():
