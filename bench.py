@@ -186,8 +186,7 @@ def bench_large(reps: int = 2):
     eng.set_kernel_timing(ROOFLINE_KERNEL, max_launches=4 * S + 16)
     eng.rollout_grad(S, 0, K, use_graph=True)
     eng.synchronize()
-    # the status bits of the timed windows only (the first rollouts, in the
-    # in-kernel overflow form, set bit 4 before the context switched forms)
+    # the status bits of the timed windows only
     eng.status(clear=True)
     t0 = time.perf_counter()
     for _ in range(reps):

@@ -214,7 +214,6 @@ struct dcp_ctx {
     std::vector<hipEvent_t> tev;
     int tev_used = 0;
     bool capturing = false;
-    bool defer_big = false;            // listing form of the tile-overflow handling (see launch_p2g)
     // lean adjoint: every grid snapshot of the segment being walked is known to
     // hold its substep (read back after the segment's forward), so the adjoint
     // substeps restore their grids without launching P2G and the grid update
@@ -306,30 +305,21 @@ static hipError_t time_mark(dcp_ctx* c) {
 
 // A launch with a skip flag is left out of the kernel timing: it exits at once
 // whenever the grid snapshot holds the substep.
-// Overflowing patch tiles (a patch spread beyond DCP_STILE_CAP cells) are
-// handled in one of two forms.  Until one has been met, the scatters handle
-// such a patch in-kernel per particle with global atomics, and no extra launch
-// is made.  Once one has been met (noted on the device and read back after a
-// rollout), the context switches for good to the listing form: the scatters
-// list such patches and k_p2g_big / k_g2p_adj_big, launched after each,
-// scatter them through the LDS tile over sub-boxes that fit it.
-template <bool DEFER>
-static int launch_p2g_form(dcp_ctx* c, StateView S, int s, int g, const int* skip) {
+// Overflowing patch tiles (a patch spread beyond DCP_STILE_CAP cells): the
+// scatters list such patches, and k_p2g_big / k_g2p_adj_big, launched after
+// each, scatter them through the LDS tile over sub-boxes that fit it (with an
+// empty list they exit at once).
+static int launch_p2g(dcp_ctx* c, StateView S, int s, int g, const int* skip) {
     if (skip) {
-        hipLaunchKernelGGL(k_p2g<DEFER>, dim3(c->grid_spatches()), dim3(DCP_WG), 0, c->stream, c->sim, S, s, g, skip);
+        hipLaunchKernelGGL(k_p2g, dim3(c->grid_spatches()), dim3(DCP_WG), 0, c->stream, c->sim, S, s, g, skip);
         LAUNCH_OK();
     } else {
-        TIMED_LAUNCH(DCP_KERNEL_P2G, k_p2g<DEFER>, dim3(c->grid_spatches()), dim3(DCP_WG), 0, c->stream, c->sim, S, s, g,
+        TIMED_LAUNCH(DCP_KERNEL_P2G, k_p2g, dim3(c->grid_spatches()), dim3(DCP_WG), 0, c->stream, c->sim, S, s, g,
                      skip);
     }
-    if (DEFER) {
-        hipLaunchKernelGGL(k_p2g_big, dim3(c->grid_big()), dim3(DCP_WG), 0, c->stream, c->sim, S, s, g, skip);
-        LAUNCH_OK();
-    }
+    hipLaunchKernelGGL(k_p2g_big, dim3(c->grid_big()), dim3(DCP_WG), 0, c->stream, c->sim, S, s, g, skip);
+    LAUNCH_OK();
     return 0;
-}
-static int launch_p2g(dcp_ctx* c, StateView S, int s, int g, const int* skip) {
-    return c->defer_big ? launch_p2g_form<true>(c, S, s, g, skip) : launch_p2g_form<false>(c, S, s, g, skip);
 }
 
 template <typename PT, bool RM>
@@ -412,15 +402,10 @@ static int launch_adjoint_grid(dcp_ctx* c, int s, int g, StateView S, int snap) 
 // G2P adjoint and grid-update adjoint (they need the substep's grid).
 static int launch_adjoint_head(dcp_ctx* c, StateView S, StateView G1, StateView Gout, int snap, int pending) {
     PrepSlot ps(c, prep_saved(c, snap) ? snap : -1);
-    if (c->defer_big) {
-        TIMED_LAUNCH(DCP_KERNEL_G2P_ADJ, k_g2p_adj<true>, dim3(c->grid_spatches()), dim3(DCP_WG), 0, c->stream, c->sim, S,
-                     G1, Gout, pending);
-        hipLaunchKernelGGL(k_g2p_adj_big, dim3(c->grid_big()), dim3(DCP_WG), 0, c->stream, c->sim, S, G1, Gout, pending);
-        LAUNCH_OK();
-    } else {
-        TIMED_LAUNCH(DCP_KERNEL_G2P_ADJ, k_g2p_adj<false>, dim3(c->grid_spatches()), dim3(DCP_WG), 0, c->stream, c->sim, S,
-                     G1, Gout, pending);
-    }
+    TIMED_LAUNCH(DCP_KERNEL_G2P_ADJ, k_g2p_adj, dim3(c->grid_spatches()), dim3(DCP_WG), 0, c->stream, c->sim, S, G1,
+                 Gout, pending);
+    hipLaunchKernelGGL(k_g2p_adj_big, dim3(c->grid_big()), dim3(DCP_WG), 0, c->stream, c->sim, S, G1, Gout, pending);
+    LAUNCH_OK();
     TIMED_LAUNCH(DCP_KERNEL_GRIDOP_ADJ, k_gridop_adj, dim3(c->grid_gridop()), dim3(DCP_WG), 0, c->stream, c->sim);
     return 0;
 }
@@ -1442,20 +1427,6 @@ static int ensure_snapshots(dcp_ctx* c, int K) {
     return 0;
 }
 
-// After a finished rollout: switch to the listing form of the tile-overflow
-// handling if the in-kernel form met an overflowing patch (the cached graphs
-// hold the other kernels and are rebuilt).
-static int update_overflow_form(dcp_ctx* c) {
-    if (c->defer_big) return 0;
-    int seen = 0;
-    HIP_OK(hipMemcpy(&seen, c->sim.big_n + DCP_BIG_SEEN, sizeof(int), hipMemcpyDeviceToHost));
-    if (!seen) return 0;
-    HIP_OK(hipStreamSynchronize(c->stream));
-    c->drop_graphs();
-    c->defer_big = true;
-    return 0;
-}
-
 extern "C" int dcp_rollout_grad(dcp_ctx* c, int32_t S, int32_t g, int32_t K, int32_t use_graph, double* loss_out,
                                 double* grads_out) {
     if (!c) return set_err("bad args");
@@ -1489,7 +1460,7 @@ extern "C" int dcp_rollout_grad(dcp_ctx* c, int32_t S, int32_t g, int32_t K, int
     c->last_ms = ms;
     c->last_final_slot = seg_slot(p, p.nseg - 1, p.S);
     if (grads_out && reduce_param_grads(c, grads_out)) return -1;
-    return update_overflow_form(c);
+    return 0;
 }
 
 extern "C" int dcp_rollout(dcp_ctx* c, int32_t S, int32_t g, int32_t use_graph, int32_t* final_slot) {
@@ -1517,7 +1488,7 @@ extern "C" int dcp_rollout(dcp_ctx* c, int32_t S, int32_t g, int32_t use_graph, 
     c->last_ms = ms;
     c->last_final_slot = cur;
     if (final_slot) *final_slot = cur;
-    return update_overflow_form(c);
+    return 0;
 }
 
 extern "C" int dcp_polar_adjoint(const float* Ft, const float* gR, int32_t n, int32_t precision, int32_t svd_grad,

@@ -256,6 +256,31 @@ __device__ __forceinline__ void p2g_contrib_rt(const InstDev& I, const Stencil& 
     }
 }
 
+// y + n x for a stencil offset n in {0, 1, 2} known at compile time after
+// unrolling (no multiply by 0, which IEEE arithmetic cannot fold away)
+__device__ __forceinline__ float add_times(float y, int n, float x) {
+    return n == 0 ? y : (n == 1 ? y + x : y + 2.0f * x);
+}
+
+// A P2G record in the separable form of its node terms.  With the node offset
+// o = (i, j, k) and dpos = (o - fx) dx, the momentum term
+//   w (b + A dpos) = w (u + i ad0 + j ad1 + k ad2),  u = b - dx A fx,
+// where ad_c is column c of dx A, and the element normal-impulse term
+//   (grad w . n) kimp = (W2_k alpha_ij + dW2_k beta_ij) kimp
+// with alpha_ij = dW0_i W1_j nn0 + W0_i dW1_j nn1, beta_ij = W0_i W1_j nn2 and
+// nn = inv_dx n: a node costs a few adds and multiplies instead of the affine
+// product and the weight gradient.  The weights are rebuilt from fx per layer
+// (a few operations) rather than kept in registers across the layer passes.
+// quadratic B-spline weights and their d/dfx along one axis (simulation.py:142,197)
+__device__ __forceinline__ void bspline_axis(float f, float w[3], float dw[3]) {
+    w[0] = 0.5f * (1.5f - f) * (1.5f - f);
+    w[1] = 0.75f - (f - 1.0f) * (f - 1.0f);
+    w[2] = 0.5f * (f - 0.5f) * (f - 0.5f);
+    dw[0] = f - 1.5f;
+    dw[1] = 2.0f * (1.0f - f);
+    dw[2] = f - 0.5f;
+}
+
 __device__ __forceinline__ void block_cell_coords(const InstDev& I, int blk, int lane, int& i, int& j, int& k) {
     int bk = blk % I.nb, t = blk / I.nb;
     int bj = t % I.nb, bi = t / I.nb;
@@ -308,7 +333,7 @@ __device__ __forceinline__ void grid_velocity(const InstDev& I, float4 c, int i,
 
 // kernels -------------------------------------------------------------------
 template <typename PT, bool RM> __global__ void k_prep(SimDev sim, StateView S, StateView Snext, int fresh);
-template <bool DEFER> __global__ void k_p2g(SimDev sim, StateView S, int s, int g, const int* skip);
+__global__ void k_p2g(SimDev sim, StateView S, int s, int g, const int* skip);
 __global__ void k_p2g_big(SimDev sim, StateView S, int s, int g, const int* skip);
 __global__ void k_gridop(SimDev sim, int keep, GridSnap save, const int* skip);
 __global__ void k_grid_restore(SimDev sim, GridSnap snap);
@@ -321,7 +346,7 @@ __global__ void k_binning_keys(SimDev sim, StateView S, int* keys);
 __global__ void k_patch_order(SimDev sim, StateView S, int* out);
 __global__ void k_rebase(SimDev sim, StateView S, float* f_old, float* Dinv, float* Dinvinv);
 
-template <bool DEFER> __global__ void k_g2p_adj(SimDev sim, StateView S, StateView G1, StateView Gout, int pending);
+__global__ void k_g2p_adj(SimDev sim, StateView S, StateView G1, StateView Gout, int pending);
 __global__ void k_g2p_adj_big(SimDev sim, StateView S, StateView G1, StateView Gout, int pending);
 __global__ void k_gridop_adj(SimDev sim);
 __global__ void k_p2g_adj(SimDev sim, StateView S, StateView Gout, int s, int g);

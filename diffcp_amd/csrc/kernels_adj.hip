@@ -46,30 +46,72 @@ __device__ __forceinline__ void g2p_adj_inputs(const SimDev& sim, const InstDev&
     }
 }
 
-// one stencil node of the G2P reverse: dL/d grid_v_out of the node (gg) and the
-// node's share of dL/dfx
-__device__ __forceinline__ void g2p_adj_node(const InstDev& I, const Stencil& st, int i, int j, int k, float4 gvo,
-                                             const V3<float>& gv, const M3<float>& gC, float gg[3], float gfx[3]) {
+// The G2P reverse of one particle in separable form.  With the node offset o
+// and t = gv + 4 inv_dx gC (o - fx) = u + o_0 A_0 + o_1 A_1 + o_2 A_2, where
+// A_b = 4 inv_dx gC(:, b) and u = gv - fx_0 A_0 - fx_1 A_1 - fx_2 A_2:
+//   dL/d grid_v_out(node) = w t,
+//   dL/dfx_d = sum_nodes (d w / d fx_d) (g . t) - A_d . (sum_nodes w g),
+// g the node's grid_v_out.  The first sum runs over k inside each (i, j) row
+// (X = sum_k W2_k g.t, Y = sum_k dW2_k g.t), so a node costs about fifteen
+// operations; the weights are rebuilt from fx per layer.
+struct G2PAdjSep {
+    float u[3], A[3][3];    // A[b][a] = 4 inv_dx gC(a, b)
+    float fx[3];
+    float gw[3], gfx[3];    // sum of w g; dL/dfx without its A . gw term
+};
+
+__device__ __forceinline__ void g2p_adj_separate(const InstDev& I, const Stencil& st, const V3<float>& gv,
+                                                 const M3<float>& gC, G2PAdjSep& q) {
     const float c4 = 4.0f * I.inv_dx;
-    float W0 = st.w[i][0], W1 = st.w[j][1], W2 = st.w[k][2];
-    float w = W0 * W1 * W2;
-    float dp[3] = {(float)i - st.fx[0], (float)j - st.fx[1], (float)k - st.fx[2]};
-    float g[3] = {gvo.x, gvo.y, gvo.z};
-    float gCdp[3], gCtg[3];
+    const float g0[3] = {gv.x, gv.y, gv.z};
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        gCdp[a] = gC(a, 0) * dp[0] + gC(a, 1) * dp[1] + gC(a, 2) * dp[2];
-        gCtg[a] = gC(0, a) * g[0] + gC(1, a) * g[1] + gC(2, a) * g[2];
+#pragma unroll
+        for (int b = 0; b < 3; ++b) q.A[b][a] = c4 * gC(a, b);
+        q.u[a] = g0[a] - (q.A[0][a] * st.fx[0] + q.A[1][a] * st.fx[1] + q.A[2][a] * st.fx[2]);
+        q.fx[a] = st.fx[a];
+        q.gw[a] = 0.f;
+        q.gfx[a] = 0.f;
     }
-    float s1 = c4 * w;
-    gg[0] = w * gv.x + s1 * gCdp[0];
-    gg[1] = w * gv.y + s1 * gCdp[1];
-    gg[2] = w * gv.z + s1 * gCdp[2];
-    float scal = gv.x * g[0] + gv.y * g[1] + gv.z * g[2] + c4 * (g[0] * gCdp[0] + g[1] * gCdp[1] + g[2] * gCdp[2]);
-    float dW0 = st.dw[i][0] * W1 * W2, dW1 = W0 * st.dw[j][1] * W2, dW2 = W0 * W1 * st.dw[k][2];
-    gfx[0] += dW0 * scal - s1 * gCtg[0];
-    gfx[1] += dW1 * scal - s1 * gCtg[1];
-    gfx[2] += dW2 * scal - s1 * gCtg[2];
+}
+
+// Layer i: the nine nodes' grid_v_out from fetch(j, k) (a float4, xyz used),
+// their dL/d grid_v_out handed to emit(j, k, g0, g1, g2).
+template <typename Fetch, typename Emit>
+__device__ __forceinline__ void g2p_adj_layer(G2PAdjSep& q, int i, Fetch fetch, Emit emit) {
+    float w0[3], d0[3], w1[3], d1[3], w2[3], d2[3];
+    bspline_axis(q.fx[0], w0, d0);
+    bspline_axis(q.fx[1], w1, d1);
+    bspline_axis(q.fx[2], w2, d2);
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        const float w01 = w0[i] * w1[j];
+        float tij[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) tij[a] = add_times(add_times(q.u[a], i, q.A[0][a]), j, q.A[1][a]);
+        float X = 0.f, Y = 0.f;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const float4 g = fetch(j, k);
+            const float t0 = add_times(tij[0], k, q.A[2][0]);
+            const float t1 = add_times(tij[1], k, q.A[2][1]);
+            const float t2 = add_times(tij[2], k, q.A[2][2]);
+            const float w = w01 * w2[k];
+            emit(j, k, w * t0, w * t1, w * t2);
+            const float scal = g.x * t0 + g.y * t1 + g.z * t2;
+            X += w2[k] * scal;
+            Y += d2[k] * scal;
+            q.gw[0] += w * g.x; q.gw[1] += w * g.y; q.gw[2] += w * g.z;
+        }
+        q.gfx[0] += d0[i] * w1[j] * X;
+        q.gfx[1] += w0[i] * d1[j] * X;
+        q.gfx[2] += w01 * Y;
+    }
+}
+
+__device__ __forceinline__ void g2p_adj_finish(const G2PAdjSep& q, float gfx[3]) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) gfx[d] = q.gfx[d] - (q.A[d][0] * q.gw[0] + q.A[d][1] * q.gw[1] + q.A[d][2] * q.gw[2]);
 }
 
 __device__ __forceinline__ void g2p_adj_store(const SimDev& sim, const InstDev& I, const StateView& Gout, const Patch& P,
@@ -87,15 +129,19 @@ __device__ __forceinline__ void g2p_adj_spill(const SimDev& sim, const StateView
         V3<float> gv, gx1;
         M3<float> gC;
         g2p_adj_inputs(sim, I, G1, P, q, pending, gv, gx1, gC);
-        float gfx[3] = {0.f, 0.f, 0.f};
+        G2PAdjSep sep;
+        g2p_adj_separate(I, st, gv, gC, sep);
+#pragma unroll
         for (int i = 0; i < 3; ++i)
-            for (int j = 0; j < 3; ++j)
-                for (int k = 0; k < 3; ++k) {
+            g2p_adj_layer(
+                sep, i,
+                [&](int j, int k) { return sim.vout[cell_index(I, st.base[0] + i, st.base[1] + j, st.base[2] + k)]; },
+                [&](int j, int k, float g0, float g1, float g2) {
                     long long gi = cell_index(I, st.base[0] + i, st.base[1] + j, st.base[2] + k);
-                    float gg[3];
-                    g2p_adj_node(I, st, i, j, k, sim.vout[gi], gv, gC, gg, gfx);
-                    atomicAdd(&sim.gacc[gi].y, gg[0]); atomicAdd(&sim.gacc[gi].z, gg[1]); atomicAdd(&sim.gacc[gi].w, gg[2]);
-                }
+                    atomicAdd(&sim.gacc[gi].y, g0); atomicAdd(&sim.gacc[gi].z, g1); atomicAdd(&sim.gacc[gi].w, g2);
+                });
+        float gfx[3];
+        g2p_adj_finish(sep, gfx);
         g2p_adj_store(sim, I, Gout, P, q, gx1, gfx);
     }
 }
@@ -132,23 +178,26 @@ __device__ __forceinline__ void g2p_adj_tile(const SimDev& sim, const InstDev& I
     __syncthreads();
     // tile index of the particle's base cell; node (i, j, k) is key + (i * e1 + j) * e2 + k
     const int key = act ? tile_cell(B, b0, b1, b2) : 0;
-    float gfx[3] = {0.f, 0.f, 0.f};   // dL/dfx, summed over the three layers
+    G2PAdjSep sep;
+    if (act) {
+        Stencil st;
+        make_stencil(x, I.inv_dx, I.G, st, nullptr);
+        g2p_adj_separate(I, st, gv, gC, sep);
+    }
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
         if (act) {
-            Stencil st;
-            make_stencil(x, I.inv_dx, I.G, st, nullptr);
-#pragma unroll
-            for (int j = 0; j < 3; ++j)
-#pragma unroll
-                for (int k = 0; k < 3; ++k) {
-                    float4 gvo = vt[key + (i * B.e1 + j) * B.e2 + k];
-                    float gg[3];
-                    g2p_adj_node(I, st, i, j, k, gvo, gv, gC, gg, gfx);
-                    cA[(j * 3 + k) * DCP_SLOT_STRIDE + pos] = make_float2(gg[0], gg[1]);
-                    cB[(j * 3 + k) * DCP_SLOT_STRIDE + pos] = gg[2];
-                }
-            if (i == 2) g2p_adj_store(sim, I, Gout, Pl, ql, gx1, gfx);
+            g2p_adj_layer(
+                sep, i, [&](int j, int k) { return vt[key + (i * B.e1 + j) * B.e2 + k]; },
+                [&](int j, int k, float g0, float g1, float g2) {
+                    cA[(j * 3 + k) * DCP_SLOT_STRIDE + pos] = make_float2(g0, g1);
+                    cB[(j * 3 + k) * DCP_SLOT_STRIDE + pos] = g2;
+                });
+            if (i == 2) {
+                float gfx[3];
+                g2p_adj_finish(sep, gfx);
+                g2p_adj_store(sim, I, Gout, Pl, ql, gx1, gfx);
+            }
         }
         __syncthreads();
         run_sums3(L, cA, cB);
@@ -184,11 +233,10 @@ __device__ __forceinline__ void g2p_adj_patch_inputs(const SimDev& sim, const In
     b2 = stencil_base(x.z, I.inv_dx, I.G);
 }
 
-// DEFER: as k_p2g's (an overflowing patch is listed for k_g2p_adj_big, or
-// reversed here per particle with global atomics).  The tile reverse is
-// written out here rather than called as g2p_adj_tile (k_g2p_adj_big's): the
-// shared form compiled to a schedule measured 3 % slower on this kernel.
-template <bool DEFER>
+// As in k_p2g, a patch whose tile overflows is listed for k_g2p_adj_big.  The
+// tile reverse is written out here rather than called as g2p_adj_tile
+// (k_g2p_adj_big's): the shared form compiled to a schedule measured 3 %
+// slower on this kernel.
 __global__ __launch_bounds__(DCP_WG, 3) void k_g2p_adj(SimDev sim, StateView S, StateView G1, StateView Gout, int pending) {
     __shared__ BinLDS L;
     __shared__ float2 cA[DCP_CONTRIB_ZERO + 1];   // contributions, channels 0 and 1
@@ -239,23 +287,26 @@ __global__ __launch_bounds__(DCP_WG, 3) void k_g2p_adj(SimDev sim, StateView S, 
                 __syncthreads();
                 // tile index of the particle's base cell; node (i, j, k) is key + (i * e1 + j) * e2 + k
                 const int key = act ? tile_cell(B, b0, b1, b2) : 0;
-                float gfx[3] = {0.f, 0.f, 0.f};   // dL/dfx, summed over the three layers
+                G2PAdjSep sep;
+                if (act) {
+                    Stencil st;
+                    make_stencil(x, I.inv_dx, I.G, st, nullptr);
+                    g2p_adj_separate(I, st, gv, gC, sep);
+                }
 #pragma unroll
                 for (int i = 0; i < 3; ++i) {
                     if (act) {
-                        Stencil st;
-                        make_stencil(x, I.inv_dx, I.G, st, nullptr);
-#pragma unroll
-                        for (int j = 0; j < 3; ++j)
-#pragma unroll
-                            for (int k = 0; k < 3; ++k) {
-                                float4 gvo = vt[key + (i * B.e1 + j) * B.e2 + k];
-                                float gg[3];
-                                g2p_adj_node(I, st, i, j, k, gvo, gv, gC, gg, gfx);
-                                cA[(j * 3 + k) * DCP_SLOT_STRIDE + pos] = make_float2(gg[0], gg[1]);
-                                cB[(j * 3 + k) * DCP_SLOT_STRIDE + pos] = gg[2];
-                            }
-                        if (i == 2) g2p_adj_store(sim, I, Gout, Pl, ql, gx1, gfx);
+                        g2p_adj_layer(
+                            sep, i, [&](int j, int k) { return vt[key + (i * B.e1 + j) * B.e2 + k]; },
+                            [&](int j, int k, float g0, float g1, float g2) {
+                                cA[(j * 3 + k) * DCP_SLOT_STRIDE + pos] = make_float2(g0, g1);
+                                cB[(j * 3 + k) * DCP_SLOT_STRIDE + pos] = g2;
+                            });
+                        if (i == 2) {
+                            float gfx[3];
+                            g2p_adj_finish(sep, gfx);
+                            g2p_adj_store(sim, I, Gout, Pl, ql, gx1, gfx);
+                        }
                     }
                     __syncthreads();
                     run_sums3(L, cA, cB);
@@ -278,15 +329,8 @@ __global__ __launch_bounds__(DCP_WG, 3) void k_g2p_adj(SimDev sim, StateView S, 
                 }
             }
         }
-        if (!tiled) {
-            if (DEFER) {   // left to k_g2p_adj_big
-                if (tid == 0) sim.big_list[sim.n_spatches + atomicAdd(&sim.big_n[DCP_BIG_G2P_ADJ], 1)] = pi;
-            } else {
-                if (tid == 0) sim.big_n[DCP_BIG_SEEN] = 1;
-                g2p_adj_spill(sim, S, G1, Gout, SP.part(0), I, pending);
-                g2p_adj_spill(sim, S, G1, Gout, SP.part(1), I, pending);
-            }
-        }
+        if (!tiled && tid == 0)   // left to k_g2p_adj_big
+            sim.big_list[sim.n_spatches + atomicAdd(&sim.big_n[DCP_BIG_G2P_ADJ], 1)] = pi;
         bin_clear(L);
         if (tid < 3) { L.box[tid] = 1 << 30; L.box[3 + tid] = -(1 << 30); }
         __syncthreads();
@@ -856,5 +900,3 @@ template __global__ void k_elem_adj<double, false>(SimDev, StateView, StateView,
 template __global__ void k_elem_adj<float, false>(SimDev, StateView, StateView, StateView, int, int, int);
 template __global__ void k_elem_adj<double, true>(SimDev, StateView, StateView, StateView, int, int, int);
 template __global__ void k_elem_adj<float, true>(SimDev, StateView, StateView, StateView, int, int, int);
-template __global__ void k_g2p_adj<false>(SimDev, StateView, StateView, StateView, int);
-template __global__ void k_g2p_adj<true>(SimDev, StateView, StateView, StateView, int);

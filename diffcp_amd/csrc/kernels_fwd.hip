@@ -224,12 +224,9 @@ __device__ __forceinline__ void p2g_tile(const SimDev& sim, const InstDev& I, co
     p2g_mark_blocks(sim, I, P, TB, tb_touched);
 }
 
-// DEFER: a patch whose tile overflows is listed for k_p2g_big; otherwise it is
-// scattered here per particle with global atomics (and big_n[2] notes it, so
-// that the host switches to the listing form).  The tile scatter is written
-// out here rather than called as p2g_tile (k_p2g_big's): the shared form
-// compiled to a schedule measured 2 % slower on this kernel.
-template <bool DEFER>
+// A patch whose tile overflows is listed for k_p2g_big.  The tile scatter is
+// written out here rather than called as p2g_tile (k_p2g_big's): the shared
+// form compiled to a schedule measured 2 % slower on this kernel.
 __global__ __launch_bounds__(DCP_WG, 2) void k_p2g(SimDev sim, StateView S, int s, int g, const int* skip) {
     if (skip && *skip) return;   // adjoint substep whose grid is restored from a snapshot
     __shared__ BinLDS L;
@@ -312,18 +309,7 @@ __global__ __launch_bounds__(DCP_WG, 2) void k_p2g(SimDev sim, StateView S, int 
                 p2g_mark_blocks(sim, I, P, TB, tb_touched);
             }
         }
-        if (!tiled) {
-            if (DEFER) {   // left to k_p2g_big
-                if (tid == 0) sim.big_list[atomicAdd(&sim.big_n[DCP_BIG_P2G], 1)] = pi;
-            } else {
-                if (tid == 0) {
-                    atomicOr(sim.status, (unsigned)DCP_STATUS_TILE_SPILL);
-                    sim.big_n[DCP_BIG_SEEN] = 1;
-                }
-                p2g_spill(sim, S, SP.part(0), I, s, g);
-                p2g_spill(sim, S, SP.part(1), I, s, g);
-            }
-        }
+        if (!tiled && tid == 0) sim.big_list[atomicAdd(&sim.big_n[DCP_BIG_P2G], 1)] = pi;   // left to k_p2g_big
         // next patch: empty counts and box
         bin_clear(L);
         if (tid < 3) { L.box[tid] = 1 << 30; L.box[3 + tid] = -(1 << 30); }
@@ -749,5 +735,3 @@ template __global__ void k_prep<float, false>(SimDev, StateView, StateView, int)
 template __global__ void k_prep<double, true>(SimDev, StateView, StateView, int);
 template __global__ void k_prep<float, true>(SimDev, StateView, StateView, int);
 
-template __global__ void k_p2g<false>(SimDev, StateView, int, int, const int*);
-template __global__ void k_p2g<true>(SimDev, StateView, int, int, const int*);

@@ -318,13 +318,11 @@ __device__ __forceinline__ bool in_sub_box(const PatchBox& S, int b0, int b1, in
 }
 
 // Lists of the patches whose tile exceeds the cap, filled by k_p2g /
-// k_g2p_adj in their listing form, worked off by k_p2g_big / k_g2p_adj_big and
-// emptied by the grid update that follows each (k_gridop / k_gridop_adj):
-// lengths in big_n[0..1], list l at big_list[l * n_spatches ...].  big_n[2]
-// is set once the non-listing form met an overflowing patch.
+// k_g2p_adj, worked off by k_p2g_big / k_g2p_adj_big and emptied by the grid
+// update that follows each (k_gridop / k_gridop_adj): lengths in big_n[0..1],
+// list l at big_list[l * n_spatches ...].
 #define DCP_BIG_P2G 0
 #define DCP_BIG_G2P_ADJ 1
-#define DCP_BIG_SEEN 2
 
 // Box of the patch's base cells from one candidate per lane.
 __device__ __forceinline__ void box_reduce(int b0, int b1, int b2, bool act, int* box, PatchBox& B) {

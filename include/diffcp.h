@@ -35,7 +35,7 @@ typedef struct dcp_ctx dcp_ctx;
 /* status bits reported by dcp_status() */
 #define DCP_STATUS_NONFINITE   1u   /* NaN/Inf in a particle velocity or grid value */
 #define DCP_STATUS_OUT_OF_GRID 2u   /* a particle stencil left [0, n_grid) (clamped) */
-#define DCP_STATUS_TILE_SPILL  4u   /* a patch tile overflowed LDS (slow path used) */
+#define DCP_STATUS_TILE_SPILL  4u   /* a patch of more than 256 particles used global atomics */
 
 /* One cloth scene.  Replaces the module globals of
  * src/initialization.py:21-193 and the fields of
