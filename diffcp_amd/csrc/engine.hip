@@ -725,7 +725,7 @@ static int build(dcp_ctx* c, const dcp_instance_desc* d, int B) {
         c->alloc(&S.status, 1) || c->alloc(&S.g_xe, 3 * (size_t)ne) ||
         c->alloc(&S.g_fv, 3 * (size_t)np) ||
         c->alloc(&S.ec_gx, 9 * (size_t)ne) || c->alloc(&S.ec_gv, 3 * (size_t)ne) ||
-        c->alloc(&S.pgrad_patch, (size_t)DCP_PATCH_GRADS * c->n_patches) || c->alloc(&S.pgrad_inst, (size_t)DCP_N_PARAMS * B) ||
+        c->alloc(&S.pgrad_patch, (size_t)DCP_PATCH_GRADS * (DCP_WG / 64) * c->n_patches) || c->alloc(&S.pgrad_inst, (size_t)DCP_N_PARAMS * B) ||
         c->alloc(&S.emod_grad, 3 * (size_t)ne) || c->alloc(&S.traj_grad, 3 * (size_t)std::max(1, traj)))
         return -1;
     if (c->alloc(&c->states, c->slot_floats * c->max_states) || c->alloc(&c->adj, c->slot_floats * c->n_adj) ||
@@ -973,7 +973,8 @@ extern "C" int dcp_zero_adjoint(dcp_ctx* c, int32_t gs) {
 
 extern "C" int dcp_zero_param_grads(dcp_ctx* c) {
     if (!c) return set_err("bad args");
-    HIP_OK(hipMemsetAsync(c->sim.pgrad_patch, 0, DCP_PATCH_GRADS * sizeof(double) * c->n_patches, c->stream));
+    HIP_OK(hipMemsetAsync(c->sim.pgrad_patch, 0, DCP_PATCH_GRADS * (DCP_WG / 64) * sizeof(double) * c->n_patches,
+                          c->stream));
     HIP_OK(hipMemsetAsync(c->sim.pgrad_inst, 0, DCP_N_PARAMS * sizeof(double) * c->B, c->stream));
     HIP_OK(hipMemsetAsync(c->sim.emod_grad, 0, 3 * sizeof(double) * c->NE, c->stream));
     size_t nt = 0;
@@ -1002,7 +1003,8 @@ extern "C" int dcp_loss_adjoint(dcp_ctx* c, int32_t slot, int32_t gs, const doub
 }
 
 static int reduce_param_grads(dcp_ctx* c, double* out) {
-    std::vector<double> pp((size_t)DCP_PATCH_GRADS * c->n_patches), pi((size_t)DCP_N_PARAMS * c->B);
+    const int nw = DCP_WG / 64;
+    std::vector<double> pp((size_t)DCP_PATCH_GRADS * nw * c->n_patches), pi((size_t)DCP_N_PARAMS * c->B);
     HIP_OK(hipMemcpyAsync(pp.data(), c->sim.pgrad_patch, pp.size() * 8, hipMemcpyDeviceToHost, c->stream));
     HIP_OK(hipMemcpyAsync(pi.data(), c->sim.pgrad_inst, pi.size() * 8, hipMemcpyDeviceToHost, c->stream));
     HIP_OK(hipStreamSynchronize(c->stream));
@@ -1011,9 +1013,12 @@ static int reduce_param_grads(dcp_ctx* c, double* out) {
     for (int p = 0; p < c->n_patches; ++p) {
         const Patch& P = c->patches_h[p];
         if (P.cls != CLS_ELEMENT) continue;
-        // E, nu, shearing, contact, friction (parameter slots 0-3 and 5)
-        for (int k = 0; k < 4; ++k) out[P.inst * DCP_N_PARAMS + k] += pp[DCP_PATCH_GRADS * (size_t)p + k];
-        out[P.inst * DCP_N_PARAMS + 5] += pp[DCP_PATCH_GRADS * (size_t)p + 4];
+        // E, nu, shearing, contact, friction (parameter slots 0-3 and 5), waves in order
+        for (int w = 0; w < nw; ++w) {
+            const double* q = &pp[DCP_PATCH_GRADS * ((size_t)p * nw + w)];
+            for (int k = 0; k < 4; ++k) out[P.inst * DCP_N_PARAMS + k] += q[k];
+            out[P.inst * DCP_N_PARAMS + 5] += q[4];
+        }
     }
     return 0;
 }

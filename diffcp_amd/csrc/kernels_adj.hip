@@ -604,9 +604,11 @@ __device__ __forceinline__ void grasper_traj_grad(const SimDev& sim, const InstD
 // One patch's particle records, their base-cell box and the patch tile of
 // dL/d(m, m v) in LDS (when the box fits the tile cap).  Returns whether the
 // tile was staged; the caller's lane holds particle tid when `act`.
+// `next_box`, when given, is reset here for the next patch (after every lane
+// has read this patch's box at the barrier of box_reduce).
 __device__ __forceinline__ bool p2g_adj_stage(const SimDev& sim, const InstDev& I, const StateView& S,
                                               const Patch& P, int s, int g, bool act, P2GRec& r, Stencil& st,
-                                              int* box, float4* gt, PatchBox& B) {
+                                              int* box, float4* gt, PatchBox& B, int* next_box = nullptr) {
     const int tid = threadIdx.x;
     if (act) {
         p2g_record(sim, I, S, P, tid, s, g, r);
@@ -615,6 +617,7 @@ __device__ __forceinline__ bool p2g_adj_stage(const SimDev& sim, const InstDev& 
         st.base[0] = st.base[1] = st.base[2] = 0;
     }
     box_reduce(st.base[0], st.base[1], st.base[2], act, box, B);
+    if (next_box && tid < 3) { next_box[tid] = 1 << 30; next_box[3 + tid] = -(1 << 30); }
     const bool tiled = B.vol <= DCP_STILE_CAP;
     if (tiled) {
 #pragma unroll
@@ -773,12 +776,15 @@ __device__ __forceinline__ void elem_adjoint(const SimDev& sim, const InstDev& I
 template <typename PT, bool RM>
 __global__ __launch_bounds__(DCP_WG, 2) void k_elem_adj(SimDev sim, StateView S, StateView G1, StateView Gout,
                                                         int fresh, int s, int g) {
-    __shared__ int box[6];
+    // two boxes, alternating by patch: the one of the next patch is reset
+    // while this patch's tile is staged, so that no barrier is needed at the
+    // end of a patch (the next patch's box barrier orders the tile's reuse)
+    __shared__ int box[2][6];
     __shared__ float4 gt[DCP_STILE_CAP];
-    __shared__ double red[DCP_PATCH_GRADS][DCP_WG / 64];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    if (tid < 3) { box[tid] = 1 << 30; box[3 + tid] = -(1 << 30); }
+    if (tid < 6) { box[tid / 3][tid % 3] = 1 << 30; box[tid / 3][3 + tid % 3] = -(1 << 30); }
     __syncthreads();
+    int par = 0;
     for (int pi = blockIdx.x; pi < sim.n_patches; pi += gridDim.x) {
         const Patch P = sim.patches[pi];
         if (P.cls != CLS_ELEMENT) continue;
@@ -787,7 +793,8 @@ __global__ __launch_bounds__(DCP_WG, 2) void k_elem_adj(SimDev sim, StateView S,
         P2GRec r;
         Stencil st;
         PatchBox B;
-        const bool tiled = p2g_adj_stage(sim, I, S, P, s, g, act, r, st, box, gt, B);
+        const bool tiled = p2g_adj_stage(sim, I, S, P, s, g, act, r, st, box[par], gt, B, box[par ^ 1]);
+        par ^= 1;
         double acc[DCP_PATCH_GRADS] = {0.0, 0.0, 0.0, 0.0, 0.0};
         if (act) {
             P2GAdj o;
@@ -808,19 +815,14 @@ __global__ __launch_bounds__(DCP_WG, 2) void k_elem_adj(SimDev sim, StateView S,
             const V3<float> gk = {o.gk[0], o.gk[1], o.gk[2]};
             elem_adjoint<PT, RM>(sim, I, S, G1, Gout, e, fresh, gxe, gve, mscale(r.m, o.gA), gn, gk, acc);
         }
+        // per (patch, wave) sums, one add per launch to each slot: the same
+        // order every run (the host sums the waves and patches in order)
+        double* pg = sim.pgrad_patch + ((size_t)pi * (DCP_WG / 64) + wv) * DCP_PATCH_GRADS;
         for (int t = 0; t < DCP_PATCH_GRADS; ++t) {
             if (!RM && t == 4) continue;   // no friction gradient without the return mapping
-            double v = wave_sum_d(acc[t]);
-            if (lane == 0) red[t][wv] = v;
+            const double v = wave_sum_d(acc[t]);
+            if (lane == 0) atomicAdd(pg + t, v);
         }
-        __syncthreads();
-        if (tid < (RM ? DCP_PATCH_GRADS : 4)) {
-            double v = 0.0;
-            for (int w = 0; w < DCP_WG / 64; ++w) v += red[tid][w];
-            sim.pgrad_patch[(size_t)pi * DCP_PATCH_GRADS + tid] += v;
-        }
-        if (tid < 3) { box[tid] = 1 << 30; box[3 + tid] = -(1 << 30); }
-        __syncthreads();
     }
 }
 

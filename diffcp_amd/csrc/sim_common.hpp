@@ -103,7 +103,7 @@ struct SimDev {
     float* g_xe;                   // [3][NE] dL/dx_e of the G2P reverse
     float* g_fv;                   // [3][NP] d/d (vertex force impulse)
     float* ec_gx; float* ec_gv;    // [9][NE] per-corner d/dx; [3][NE] d/dv (one for the three corners)
-    double* pgrad_patch;           // [n_patches][DCP_PATCH_GRADS]  E, nu, shear, contact, friction (element patches)
+    double* pgrad_patch;           // [n_patches][waves][DCP_PATCH_GRADS]  E, nu, shear, contact, friction (element patches)
     double* pgrad_inst;            // [B][DCP_N_PARAMS] (damping, friction via atomics)
     double* emod_grad;             // [3][NE]
     double* traj_grad;             // [frames*grasp][3]
