@@ -28,24 +28,28 @@ class RolloutLoss(torch.autograd.Function):
     @staticmethod
     def forward(ctx, params: torch.Tensor, element_moduli: Optional[torch.Tensor], engine: Engine,
                 n_substeps: int, g: int, checkpoint_every: int, use_graph: bool):
-        p = params.detach().to("cpu", torch.float64).contiguous().numpy()
-        if p.shape != (engine.B, 6):
+        # the library reads and writes the tensors' memory directly (host or
+        # device): no numpy copies
+        if tuple(params.shape) != (engine.B, 6):
             raise ValueError(f"params must be [{engine.B}, 6] (E, nu, shearing, contact, damping, friction)")
-        engine.set_params(p)
+        engine.set_params(params.detach().to(torch.float64).contiguous())
         if element_moduli is not None:
-            engine.set_element_moduli(element_moduli.detach().to("cpu", torch.float64).contiguous().numpy())
-        loss, grads, _ = engine.rollout_grad(n_substeps, g, checkpoint_every, use_graph)
-        if not (np.isfinite(loss).all() and np.isfinite(grads).all()):
-            # the reference's caller sanitises with nan_to_num (parameter_fitting_functions.py:54-57)
-            grads = np.nan_to_num(grads)
+            engine.set_element_moduli(element_moduli.detach().to(torch.float64).contiguous())
+        loss = torch.empty(engine.B, dtype=torch.float64, device=params.device)
+        grads = torch.empty(engine.B, 6, dtype=torch.float64, device=params.device)
+        engine.rollout_grad(n_substeps, g, checkpoint_every, use_graph, loss_out=loss, grads_out=grads)
+        # the reference's caller sanitises with nan_to_num (parameter_fitting_functions.py:54-57)
+        grads = torch.nan_to_num(grads)
         ctx.has_moduli = element_moduli is not None
-        saved = [torch.from_numpy(grads).to(params.device)]
+        saved = [grads]
         if element_moduli is not None:
-            saved.append(torch.from_numpy(engine.element_moduli_grads()).to(element_moduli.device))
+            ne = sum(i.n_elements for i in engine.instances)
+            gm = torch.empty(ne, 3, dtype=torch.float64, device=element_moduli.device)
+            saved.append(engine.element_moduli_grads(out=gm))
         ctx.save_for_backward(*saved)
         ctx.elem_inst = (np.repeat(np.arange(engine.B), [i.n_elements for i in engine.instances])
                          if element_moduli is not None else None)
-        return torch.from_numpy(loss).to(params.device)
+        return loss
 
     @staticmethod
     def backward(ctx, grad_loss: torch.Tensor):

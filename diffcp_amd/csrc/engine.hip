@@ -179,6 +179,8 @@ struct dcp_ctx {
     // element, used once by the next forward substep's stress
     float* f_old = nullptr;
     bool rebase_pending = false;
+    // natural particle -> slot index for caller state arrays in device memory
+    int* io_perm = nullptr;
     void use_prep(int slot) {
         float* b = slot >= 0 && snap_prep ? snap_prep + (size_t)slot * DCP_PREP_FLOATS * NE : prep_work;
         sim.xe = b; sim.ve = b + (size_t)3 * NE; sim.ke = b + (size_t)6 * NE; sim.nd = b + (size_t)9 * NE;
@@ -742,6 +744,50 @@ static int build(dcp_ctx* c, const dcp_instance_desc* d, int B) {
     return 0;
 }
 
+// ---------------------------------------------------------------------------
+// caller arrays in host or device memory
+// ---------------------------------------------------------------------------
+// Every array argument of the C-ABI may be host memory or device memory of
+// the context's GPU (a torch tensor's data_ptr, say), told apart by HIP's
+// unified addressing.  Device arrays of a state slot go through the
+// k_state_export / k_state_import kernels; the other inputs are read on the
+// host (parameters, targets, positions for the rest shapes: small, and
+// reordered or preprocessed there), so device inputs are first copied in;
+// results computed on the host or in device buffers are copied to the
+// caller's memory, whichever it is.
+static bool is_device_ptr(const void* p) {
+    if (!p) return false;
+    hipPointerAttribute_t a;
+    if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+        (void)hipGetLastError();   // plain host memory the runtime does not know
+        return false;
+    }
+    return a.type == hipMemoryTypeDevice;
+}
+
+// the caller's input of n elements as host memory (tmp holds a copy of a device array)
+template <typename T>
+static int host_in(dcp_ctx* c, const T*& p, size_t n, std::vector<T>& tmp) {
+    if (!p || !is_device_ptr(p)) return 0;
+    tmp.resize(std::max<size_t>(n, 1));
+    HIP_OK(hipMemcpyAsync(tmp.data(), p, n * sizeof(T), hipMemcpyDeviceToHost, c->stream));
+    HIP_OK(hipStreamSynchronize(c->stream));
+    p = tmp.data();
+    return 0;
+}
+
+// host result -> the caller's memory (host or device)
+static int put_out(dcp_ctx* c, void* dst, const void* src, size_t bytes) {
+    if (!dst || !bytes) return 0;
+    if (is_device_ptr(dst)) {
+        HIP_OK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+        HIP_OK(hipStreamSynchronize(c->stream));
+    } else {
+        std::memcpy(dst, src, bytes);
+    }
+    return 0;
+}
+
 extern "C" int dcp_create(const dcp_instance_desc* instances, int32_t n, const dcp_options* options, void* stream,
                           dcp_ctx** out) {
     if (!instances || n <= 0 || !out) return set_err("dcp_create: bad arguments");
@@ -797,6 +843,8 @@ extern "C" int dcp_device_bytes(dcp_ctx* c, int64_t* out) {
 
 extern "C" int dcp_set_params(dcp_ctx* c, const double* p) {
     if (!c || !p) return set_err("bad args");
+    std::vector<double> tmp;
+    if (host_in(c, p, (size_t)c->B * DCP_N_PARAMS, tmp)) return -1;
     for (int b = 0; b < c->B; ++b) {
         const double* q = p + (size_t)b * DCP_N_PARAMS;
         if (!(q[1] > -1.0 && q[1] < 0.5)) return set_err("nu must lie in (-1, 0.5)");
@@ -813,6 +861,8 @@ extern "C" int dcp_set_params(dcp_ctx* c, const double* p) {
 extern "C" int dcp_set_element_moduli(dcp_ctx* c, const double* m) {
     if (!c || !m) return set_err("bad args");
     if (!c->opt.per_element_moduli) return set_err("context built without per_element_moduli");
+    std::vector<double> tmp;
+    if (host_in(c, m, 3 * (size_t)c->NE, tmp)) return -1;
     std::vector<float> h(3 * (size_t)c->NE);
     size_t nat = 0;
     for (int b = 0; b < c->B; ++b) {
@@ -830,12 +880,16 @@ extern "C" int dcp_set_element_moduli(dcp_ctx* c, const double* m) {
 
 extern "C" int dcp_set_targets(dcp_ctx* c, const double* t, const int32_t* counts) {
     if (!c || !counts) return set_err("bad args");
+    std::vector<int32_t> ctmp;
+    if (host_in(c, counts, (size_t)c->B, ctmp)) return -1;
     int tot = 0;
     for (int b = 0; b < c->B; ++b) {
         if (counts[b] < 0) return set_err("negative target count");
         tot += counts[b];
     }
     if (tot > 0 && !t) return set_err("bad args");
+    std::vector<double> ttmp;
+    if (host_in(c, t, 3 * (size_t)tot, ttmp)) return -1;
     std::vector<float> h(3 * (size_t)std::max(1, tot), 0.f);
     for (int i = 0; i < 3 * tot; ++i) h[i] = (float)t[i];
     // same counts as the targets in place: only the coordinates change, so
@@ -892,6 +946,12 @@ extern "C" int dcp_set_targets(dcp_ctx* c, const double* t, const int32_t* count
 // initialize_ti_field (simulation.py:447-477) in FP32 on the host
 extern "C" int dcp_reset(dcp_ctx* c, int32_t slot, const double* vx, const double* ox, const double* cx) {
     if (!c || !slot_ok(c, slot)) return set_err("dcp_reset: bad slot");
+    std::vector<double> vx_h, ox_h, cx_h;   // copies of device inputs
+    {
+        size_t tv = 0, to = 0, tc = 0;
+        for (const HostInst& H : c->hi) { tv += H.nv; to += H.no; tc += H.nc; }
+        if (host_in(c, vx, 3 * tv, vx_h) || host_in(c, ox, 3 * to, ox_h) || host_in(c, cx, 3 * tc, cx_h)) return -1;
+    }
     const int NP = c->NP, NE = c->NE;
     std::vector<float> st(c->slot_floats, 0.f);
     std::vector<float> Dinv(9 * (size_t)NE), Dii(9 * (size_t)NE);
@@ -987,7 +1047,7 @@ extern "C" int dcp_loss(dcp_ctx* c, int32_t slot, double* out) {
     if (!c || !slot_ok(c, slot)) return set_err("dcp_loss: bad slot");
     if (launch_loss(c, c->state(slot))) return -1;
     if (out) {
-        HIP_OK(hipMemcpyAsync(out, c->loss_d, c->B * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIP_OK(hipMemcpyAsync(out, c->loss_d, c->B * sizeof(double), hipMemcpyDefault, c->stream));
         HIP_OK(hipStreamSynchronize(c->stream));
     }
     return 0;
@@ -995,14 +1055,16 @@ extern "C" int dcp_loss(dcp_ctx* c, int32_t slot, double* out) {
 
 extern "C" int dcp_loss_adjoint(dcp_ctx* c, int32_t slot, int32_t gs, const double* seed) {
     if (!c || !slot_ok(c, slot) || !gslot_ok(c, gs)) return set_err("dcp_loss_adjoint: bad slot");
-    if (seed) HIP_OK(hipMemcpyAsync(c->seed_d, seed, c->B * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    if (seed) HIP_OK(hipMemcpyAsync(c->seed_d, seed, c->B * sizeof(double), hipMemcpyDefault, c->stream));
     if (launch_loss(c, c->state(slot))) return -1;
     if (launch_loss_adj(c, c->state(slot), c->adjv(gs))) return -1;
     HIP_OK(hipStreamSynchronize(c->stream));
     return 0;
 }
 
-static int reduce_param_grads(dcp_ctx* c, double* out) {
+static int reduce_param_grads(dcp_ctx* c, double* out_caller) {
+    std::vector<double> outv((size_t)DCP_N_PARAMS * c->B);
+    double* out = outv.data();
     const int nw = DCP_WG / 64;
     std::vector<double> pp((size_t)DCP_PATCH_GRADS * nw * c->n_patches), pi((size_t)DCP_N_PARAMS * c->B);
     HIP_OK(hipMemcpyAsync(pp.data(), c->sim.pgrad_patch, pp.size() * 8, hipMemcpyDeviceToHost, c->stream));
@@ -1020,7 +1082,7 @@ static int reduce_param_grads(dcp_ctx* c, double* out) {
             out[P.inst * DCP_N_PARAMS + 5] += q[4];
         }
     }
-    return 0;
+    return put_out(c, out_caller, out, outv.size() * sizeof(double));
 }
 
 extern "C" int dcp_param_grads(dcp_ctx* c, double* out) {
@@ -1030,7 +1092,7 @@ extern "C" int dcp_param_grads(dcp_ctx* c, double* out) {
 
 extern "C" int dcp_element_moduli_grads(dcp_ctx* c, double* out) {
     if (!c || !out) return set_err("bad args");
-    std::vector<double> h(3 * (size_t)c->NE);
+    std::vector<double> h(3 * (size_t)c->NE), o(3 * (size_t)c->NE);
     HIP_OK(hipMemcpyAsync(h.data(), c->sim.emod_grad, h.size() * 8, hipMemcpyDeviceToHost, c->stream));
     HIP_OK(hipStreamSynchronize(c->stream));
     size_t nat = 0;
@@ -1039,17 +1101,19 @@ extern "C" int dcp_element_moduli_grads(dcp_ctx* c, double* out) {
         const InstDev& I = c->inst_h[b];
         for (int en = 0; en < H.ne; ++en) {
             int e = I.e_start + H.einv[en];
-            for (int k = 0; k < 3; ++k) out[3 * (nat + en) + k] = h[(size_t)k * c->NE + e];
+            for (int k = 0; k < 3; ++k) o[3 * (nat + en) + k] = h[(size_t)k * c->NE + e];
         }
         nat += H.ne;
     }
-    return 0;
+    return put_out(c, out, o.data(), o.size() * sizeof(double));
 }
 
 extern "C" int dcp_set_trajectory(dcp_ctx* c, const double* traj) {
     if (!c || !traj) return set_err("bad args");
     size_t nt = 0;
     for (auto& I : c->inst_h) nt += (size_t)I.n_actions * I.n_grasp;
+    std::vector<double> tmp;
+    if (host_in(c, traj, 3 * nt, tmp)) return -1;
     c->traj_h.resize(3 * nt);
     for (size_t i = 0; i < 3 * nt; ++i) c->traj_h[i] = (float)traj[i];
     HIP_OK(hipMemcpyAsync((void*)c->sim.traj, c->traj_h.data(), 3 * nt * sizeof(float), hipMemcpyHostToDevice, c->stream));
@@ -1061,7 +1125,7 @@ extern "C" int dcp_trajectory_grads(dcp_ctx* c, double* out) {
     if (!c || !out) return set_err("bad args");
     size_t nt = 0;
     for (auto& I : c->inst_h) nt += (size_t)I.n_actions * I.n_grasp;
-    HIP_OK(hipMemcpyAsync(out, c->sim.traj_grad, 3 * nt * 8, hipMemcpyDeviceToHost, c->stream));
+    HIP_OK(hipMemcpyAsync(out, c->sim.traj_grad, 3 * nt * 8, hipMemcpyDefault, c->stream));
     HIP_OK(hipStreamSynchronize(c->stream));
     return 0;
 }
@@ -1121,7 +1185,49 @@ static void host_to_slot(const dcp_ctx* c, std::vector<float>& h, const HostArra
     }
 }
 
+// 1: every given array is device memory, 0: every one host memory, -1: a mix
+static int arrays_kind(const HostArrays& A) {
+    int dev = 0, host = 0;
+    for (float* a : A.a) {
+        if (!a) continue;
+        if (is_device_ptr(a)) ++dev; else ++host;
+    }
+    return dev && host ? -1 : (dev ? 1 : 0);
+}
+
+static int state_io(dcp_ctx* c, const HostArrays& A, StateIO& io) {
+    if (!c->io_perm) {   // natural particle -> slot index, built on first use
+        std::vector<int> perm;
+        for (int cls = 0; cls < 4; ++cls)
+            for (int b = 0; b < c->B; ++b) {
+                const HostInst& H = c->hi[b];
+                const InstDev& I = c->inst_h[b];
+                if (cls == 0) for (int i = 0; i < H.nv; ++i) perm.push_back(I.v_start + H.vinv[i]);
+                if (cls == 1) for (int i = 0; i < H.no; ++i) perm.push_back(I.o_start + i);
+                if (cls == 2) for (int i = 0; i < H.nc; ++i) perm.push_back(I.c_start + H.cinv[i]);
+                if (cls == 3) for (int i = 0; i < H.ne; ++i) perm.push_back(I.e_start + H.einv[i]);
+            }
+        if (c->upload(&c->io_perm, perm)) return -1;
+    }
+    for (int k = 0; k < 11; ++k) io.a[k] = A.a[k];
+    io.perm = c->io_perm;
+    io.n[0] = io.n[1] = io.n[2] = io.n[3] = 0;
+    for (const HostInst& H : c->hi) { io.n[0] += H.nv; io.n[1] += H.no; io.n[2] += H.nc; io.n[3] += H.ne; }
+    return 0;
+}
+
 static int download_to(dcp_ctx* c, const float* dev, const HostArrays& A) {
+    const int kind = arrays_kind(A);
+    if (kind < 0) return set_err("state arrays: all host or all device memory");
+    if (kind == 1) {
+        StateIO io;
+        if (state_io(c, A, io)) return -1;
+        const int n = io.n[0] + io.n[1] + io.n[2] + io.n[3];
+        hipLaunchKernelGGL(k_state_export, dim3((n + 255) / 256), dim3(256), 0, c->stream, dev, c->NP, c->NE, io);
+        LAUNCH_OK();
+        HIP_OK(hipStreamSynchronize(c->stream));
+        return 0;
+    }
     std::vector<float> h(c->slot_floats);
     HIP_OK(hipMemcpyAsync(h.data(), dev, h.size() * 4, hipMemcpyDeviceToHost, c->stream));
     HIP_OK(hipStreamSynchronize(c->stream));
@@ -1130,6 +1236,17 @@ static int download_to(dcp_ctx* c, const float* dev, const HostArrays& A) {
 }
 
 static int upload_from(dcp_ctx* c, float* dev, const HostArrays& A) {
+    const int kind = arrays_kind(A);
+    if (kind < 0) return set_err("state arrays: all host or all device memory");
+    if (kind == 1) {
+        StateIO io;
+        if (state_io(c, A, io) || launch_fill(c, dev, c->slot_floats)) return -1;
+        const int n = io.n[0] + io.n[1] + io.n[2] + io.n[3];
+        hipLaunchKernelGGL(k_state_import, dim3((n + 255) / 256), dim3(256), 0, c->stream, dev, c->NP, c->NE, io);
+        LAUNCH_OK();
+        HIP_OK(hipStreamSynchronize(c->stream));
+        return 0;
+    }
     std::vector<float> h(c->slot_floats, 0.f);
     host_to_slot(c, h, A);
     HIP_OK(hipMemcpyAsync(dev, h.data(), h.size() * 4, hipMemcpyHostToDevice, c->stream));
@@ -1450,7 +1567,7 @@ extern "C" int dcp_rollout_grad(dcp_ctx* c, int32_t S, int32_t g, int32_t K, int
     for (int cs = 0; cs < p.nseg; ++cs)
         if (run_recorded(c, REC_FORWARD, cs, g, p, use_graph)) return -1;
     HIP_OK(hipMemcpyAsync(loss_out ? loss_out : nullptr, c->loss_d, loss_out ? c->B * sizeof(double) : 0,
-                          hipMemcpyDeviceToHost, c->stream));
+                          hipMemcpyDefault, c->stream));
     int last = (p.L - 1) / p.K;
     for (int cs = last; cs >= 0; --cs) {
         if (cs != p.nseg - 1 && run_recorded(c, REC_RECOMPUTE, cs, g, p, use_graph)) return -1;

@@ -333,6 +333,14 @@ __device__ __forceinline__ void grid_velocity(const InstDev& I, float4 c, int i,
 
 // kernels -------------------------------------------------------------------
 template <typename PT, bool RM> __global__ void k_prep(SimDev sim, StateView S, StateView Snext, int fresh);
+// caller arrays of a state slot in device memory (k_state_export / k_state_import)
+struct StateIO {
+    float* a[11];      // vertex x v C, grasper x v C, cube x v C, element C, element normal column
+    const int* perm;   // natural particle (vertices, graspers, cubes, elements) -> slot index
+    int n[4];          // vertices, graspers, cubes, elements (all instances)
+};
+__global__ void k_state_export(const float* slot, int NP, int NE, StateIO io);
+__global__ void k_state_import(float* slot, int NP, int NE, StateIO io);
 __global__ void k_p2g(SimDev sim, StateView S, int s, int g, const int* skip);
 __global__ void k_p2g_big(SimDev sim, StateView S, int s, int g, const int* skip);
 __global__ void k_gridop(SimDev sim, int keep, GridSnap save, const int* skip);

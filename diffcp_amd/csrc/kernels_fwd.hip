@@ -735,3 +735,59 @@ template __global__ void k_prep<float, false>(SimDev, StateView, StateView, int)
 template __global__ void k_prep<double, true>(SimDev, StateView, StateView, int);
 template __global__ void k_prep<float, true>(SimDev, StateView, StateView, int);
 
+
+// ---------------------------------------------------------------------------
+// state slot <-> caller arrays in device memory
+// ---------------------------------------------------------------------------
+// One thread per particle in natural order (vertices, graspers, cubes, then
+// elements of every instance); perm holds the slot index of each.  The
+// caller's arrays are row-major per particle: x, v [n][3], C [n][3][3],
+// element C [ne][3][3] and normal column [ne][3]; a null array is skipped
+// (export) or left zero (import, after the slot is cleared).
+__global__ void k_state_export(const float* slot, int NP, int NE, StateIO io) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n_pts = io.n[0] + io.n[1] + io.n[2];
+    if (t >= n_pts + io.n[3]) return;
+    if (t < n_pts) {
+        const int cls = t < io.n[0] ? 0 : (t < io.n[0] + io.n[1] ? 1 : 2);
+        const int i = t - (cls > 0 ? io.n[0] : 0) - (cls > 1 ? io.n[1] : 0);
+        const int p = io.perm[t];
+        float* x = io.a[3 * cls];
+        float* v = io.a[3 * cls + 1];
+        float* C = io.a[3 * cls + 2];
+        for (int k = 0; k < 3; ++k) {
+            if (x) x[3 * (size_t)i + k] = slot[(size_t)k * NP + p];
+            if (v) v[3 * (size_t)i + k] = slot[(size_t)(3 + k) * NP + p];
+        }
+        if (C) for (int k = 0; k < 9; ++k) C[9 * (size_t)i + k] = slot[(size_t)(6 + k) * NP + p];
+    } else {
+        const int i = t - n_pts, e = io.perm[t];
+        const float* base = slot + 15 * (size_t)NP;
+        if (io.a[9]) for (int k = 0; k < 9; ++k) io.a[9][9 * (size_t)i + k] = base[(size_t)k * NE + e];
+        if (io.a[10]) for (int k = 0; k < 3; ++k) io.a[10][3 * (size_t)i + k] = base[(size_t)(9 + k) * NE + e];
+    }
+}
+
+__global__ void k_state_import(float* slot, int NP, int NE, StateIO io) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n_pts = io.n[0] + io.n[1] + io.n[2];
+    if (t >= n_pts + io.n[3]) return;
+    if (t < n_pts) {
+        const int cls = t < io.n[0] ? 0 : (t < io.n[0] + io.n[1] ? 1 : 2);
+        const int i = t - (cls > 0 ? io.n[0] : 0) - (cls > 1 ? io.n[1] : 0);
+        const int p = io.perm[t];
+        const float* x = io.a[3 * cls];
+        const float* v = io.a[3 * cls + 1];
+        const float* C = io.a[3 * cls + 2];
+        for (int k = 0; k < 3; ++k) {
+            if (x) slot[(size_t)k * NP + p] = x[3 * (size_t)i + k];
+            if (v) slot[(size_t)(3 + k) * NP + p] = v[3 * (size_t)i + k];
+        }
+        if (C) for (int k = 0; k < 9; ++k) slot[(size_t)(6 + k) * NP + p] = C[9 * (size_t)i + k];
+    } else {
+        const int i = t - n_pts, e = io.perm[t];
+        float* base = slot + 15 * (size_t)NP;
+        if (io.a[9]) for (int k = 0; k < 9; ++k) base[(size_t)k * NE + e] = io.a[9][9 * (size_t)i + k];
+        if (io.a[10]) for (int k = 0; k < 3; ++k) base[(size_t)(9 + k) * NE + e] = io.a[10][3 * (size_t)i + k];
+    }
+}

@@ -25,6 +25,32 @@ def _fp(a: Optional[np.ndarray]):
     return a.ctypes.data_as(_abi.c_flt_p)
 
 
+def _is_tensor(x) -> bool:
+    return type(x).__module__.split(".")[0] == "torch"
+
+
+def _tensor_arg(t, dtype: str, shape, name: str):
+    """A torch tensor (host or device memory of the engine's GPU) as a C-ABI
+    array argument: dtype, contiguity and shape are checked here, the memory
+    kind by the library.  A device tensor's pending work on torch's current
+    stream is waited for first: the library works on its own stream, and it
+    has finished with the tensor when the call returns."""
+    import torch
+    want = {"float32": torch.float32, "float64": torch.float64, "int32": torch.int32}[dtype]
+    if t.dtype != want:
+        raise TypeError(f"{name}: {dtype} tensor expected, got {t.dtype}")
+    if not t.is_contiguous():
+        raise ValueError(f"{name}: tensor must be contiguous")
+    if shape is not None and tuple(t.shape) != tuple(shape):
+        raise ValueError(f"{name}: shape {tuple(shape)} expected, got {tuple(t.shape)}")
+    if t.device.type not in ("cpu", "cuda"):
+        raise ValueError(f"{name}: tensor on {t.device} (cpu or the engine's GPU expected)")
+    if t.device.type == "cuda":
+        torch.cuda.current_stream(t.device).synchronize()
+    ptr_type = {"float32": _abi.c_flt_p, "float64": _abi.c_dbl_p, "int32": _abi.c_int_p}[dtype]
+    return ctypes.cast(ctypes.c_void_p(t.data_ptr()), ptr_type)
+
+
 def _snap_option(v) -> int:
     """dcp_options.grid_snapshots from True (on), False (off) or a block bound per substep."""
     if isinstance(v, bool):
@@ -111,28 +137,58 @@ class Engine:
         except Exception:
             pass
 
-    def set_params(self, params: np.ndarray) -> None:
+    def set_params(self, params) -> None:
+        """[B, 6] float64 (E, nu, shearing, contact, damping, friction): numpy,
+        or a torch tensor in host or device memory."""
+        if _is_tensor(params):
+            _abi.check(self.lib.dcp_set_params(self.ctx, _tensor_arg(params, "float64", (self.B, N_PARAMS), "params")))
+            self.params = params.detach().cpu().numpy().copy()
+            return
         p = np.ascontiguousarray(np.asarray(params, dtype=np.float64).reshape(self.B, N_PARAMS))
         _abi.check(self.lib.dcp_set_params(self.ctx, _dp(p)))
         self.params = p.copy()
 
-    def set_element_moduli(self, moduli: np.ndarray) -> None:
+    def set_element_moduli(self, moduli) -> None:
+        if _is_tensor(moduli):
+            n = sum(i.n_elements for i in self.instances)
+            _abi.check(self.lib.dcp_set_element_moduli(self.ctx, _tensor_arg(moduli, "float64", (n, 3), "moduli")))
+            return
         m = np.ascontiguousarray(np.asarray(moduli, dtype=np.float64).reshape(-1, 3))
         _abi.check(self.lib.dcp_set_element_moduli(self.ctx, _dp(m)))
 
-    def set_targets(self, targets: List[np.ndarray]) -> None:
+    def set_targets(self, targets, counts: Optional[Sequence[int]] = None) -> None:
+        """One point cloud per instance (a list of [n_b, 3] arrays), or all of
+        them concatenated in one [sum n_b, 3] float64 torch tensor (host or
+        device memory) with ``counts``."""
+        if _is_tensor(targets):
+            if counts is None or len(counts) != self.B:
+                raise ValueError("a concatenated target tensor needs one count per instance")
+            c = np.ascontiguousarray(np.asarray(counts, dtype=np.int32))
+            _abi.check(self.lib.dcp_set_targets(self.ctx, _tensor_arg(targets, "float64", (int(c.sum()), 3), "targets"),
+                                                c.ctypes.data_as(_abi.c_int_p)))
+            return
         counts = np.array([len(t) for t in targets], dtype=np.int32)
         t = np.ascontiguousarray(np.concatenate([np.asarray(x, dtype=np.float64).reshape(-1, 3)
                                                  for x in targets]))
         _abi.check(self.lib.dcp_set_targets(self.ctx, _dp(t), counts.ctypes.data_as(_abi.c_int_p)))
 
     def reset(self, slot: int = 0, vertex_x=None, grasper_x=None, cube_x=None) -> None:
-        def cat(arrs):
+        """Rest state in ``slot`` (initialize_ti_field) from the creation
+        positions, or from given ones: one array per instance, or all
+        instances' concatenated in one [n, 3] float64 torch tensor (host or
+        device memory)."""
+        def arg(arrs, count):
             if arrs is None:
                 return None
-            return np.ascontiguousarray(np.concatenate([np.asarray(a, np.float64).reshape(-1, 3) for a in arrs]))
-        vx, ox, cx = cat(vertex_x), cat(grasper_x), cat(cube_x)
-        _abi.check(self.lib.dcp_reset(self.ctx, slot, _dp(vx), _dp(ox), _dp(cx)))
+            if _is_tensor(arrs):
+                n = sum(getattr(i, count) for i in self.instances)
+                return _tensor_arg(arrs, "float64", (n, 3), count)
+            a = np.ascontiguousarray(np.concatenate([np.asarray(x, np.float64).reshape(-1, 3) for x in arrs]))
+            keep.append(a)
+            return _dp(a)
+        keep = []
+        _abi.check(self.lib.dcp_reset(self.ctx, slot, arg(vertex_x, "n_vertices"), arg(grasper_x, "n_grasper"),
+                                      arg(cube_x, "n_cube")))
 
     def device_bytes(self) -> int:
         """Device memory held by the context (all buffers, snapshots included)."""
@@ -166,13 +222,22 @@ class Engine:
         sd = None if seed is None else np.ascontiguousarray(np.broadcast_to(np.asarray(seed, np.float64), (self.B,)))
         _abi.check(self.lib.dcp_loss_adjoint(self.ctx, slot, gslot, _dp(sd)))
 
-    def param_grads(self) -> np.ndarray:
+    def param_grads(self, out=None):
+        """[B, 6] float64; ``out``: a torch tensor to fill (host or device memory)."""
+        if out is not None:
+            _abi.check(self.lib.dcp_param_grads(self.ctx, _tensor_arg(out, "float64", (self.B, N_PARAMS), "out")))
+            return out
         out = np.zeros((self.B, N_PARAMS), np.float64)
         _abi.check(self.lib.dcp_param_grads(self.ctx, _dp(out)))
         return out
 
-    def element_moduli_grads(self) -> np.ndarray:
-        out = np.zeros((sum(i.n_elements for i in self.instances), 3), np.float64)
+    def element_moduli_grads(self, out=None):
+        """[sum NE, 3] float64; ``out``: a torch tensor to fill (host or device memory)."""
+        n = sum(i.n_elements for i in self.instances)
+        if out is not None:
+            _abi.check(self.lib.dcp_element_moduli_grads(self.ctx, _tensor_arg(out, "float64", (n, 3), "out")))
+            return out
+        out = np.zeros((n, 3), np.float64)
         _abi.check(self.lib.dcp_element_moduli_grads(self.ctx, _dp(out)))
         return out
 
@@ -217,7 +282,21 @@ class Engine:
                 out[k] = np.zeros((shapes[cls], 3, 3) if f == "C" else (shapes[cls], 3), np.float32)
         return out
 
-    def download_state(self, slot: int) -> Dict[str, np.ndarray]:
+    def state_shapes(self) -> Dict[str, tuple]:
+        return {k: v.shape for k, v in self._alloc_state().items()}
+
+    def _tensor_arrays(self, arrs):
+        shapes = self.state_shapes()
+        return [None if arrs.get(k) is None else _tensor_arg(arrs[k], "float32", shapes[k], k)
+                for k in self.STATE_NAMES]
+
+    def download_state(self, slot: int, out=None) -> Dict[str, np.ndarray]:
+        """The state in ``slot`` in natural particle order.  ``out``: a dict of
+        float32 torch tensors (shapes ``state_shapes()``, any subset, all in
+        host or all in device memory) filled in place."""
+        if out is not None:
+            _abi.check(self.lib.dcp_download_state(self.ctx, slot, *self._tensor_arrays(out)))
+            return out
         st = self._alloc_state()
         _abi.check(self.lib.dcp_download_state(self.ctx, slot, *[_fp(st[k]) for k in self.STATE_NAMES]))
         return st
@@ -232,6 +311,11 @@ class Engine:
                 for k in self.STATE_NAMES if arrs.get(k) is not None}
 
     def upload_state(self, slot: int, **arrs) -> None:
+        """Arrays named as ``STATE_NAMES`` (missing ones are zero): numpy, or
+        float32 torch tensors all in host or all in device memory."""
+        if any(_is_tensor(v) for v in arrs.values() if v is not None):
+            _abi.check(self.lib.dcp_upload_state(self.ctx, slot, *self._tensor_arrays(arrs)))
+            return
         conv = self._conv(arrs)
         _abi.check(self.lib.dcp_upload_state(self.ctx, slot, *[_fp(conv.get(k)) for k in self.STATE_NAMES]))
 
@@ -265,11 +349,22 @@ class Engine:
 
     # ------------------------------------------------------------------ rollouts
     def rollout_grad(self, n_substeps: int, g: int = 0, checkpoint_every: int = 0,
-                     use_graph: bool = True):
+                     use_graph: bool = True, loss_out=None, grads_out=None):
         """Forward + adjoint of a whole window from the reset state in slot 0.
 
-        Returns (loss[B], grads[B, 6], device milliseconds).
+        Returns (loss[B], grads[B, 6], device milliseconds).  ``loss_out`` /
+        ``grads_out``: float64 torch tensors ([B], [B, 6], host or device
+        memory) filled in place and returned instead of numpy arrays.
         """
+        if loss_out is not None or grads_out is not None:
+            if loss_out is None or grads_out is None:
+                raise ValueError("loss_out and grads_out go together")
+            lp = _tensor_arg(loss_out, "float64", (self.B,), "loss_out")
+            gp = _tensor_arg(grads_out, "float64", (self.B, N_PARAMS), "grads_out")
+            _abi.check(self.lib.dcp_rollout_grad(self.ctx, n_substeps, g, checkpoint_every, int(use_graph), lp, gp))
+            ms = ctypes.c_double()
+            _abi.check(self.lib.dcp_last_rollout_ms(self.ctx, ctypes.byref(ms)))
+            return loss_out, grads_out, ms.value
         loss = np.zeros(self.B, np.float64)
         grads = np.zeros((self.B, N_PARAMS), np.float64)
         _abi.check(self.lib.dcp_rollout_grad(self.ctx, n_substeps, g, checkpoint_every,

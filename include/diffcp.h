@@ -14,8 +14,17 @@
  *   - launches are asynchronous except the calls that return host data
  *     (dcp_download_state, dcp_loss, dcp_param_grads, dcp_status,
  *     dcp_rollout_grad), which synchronise the context stream;
- *   - host arrays are in the caller's (natural) particle order; the library
- *     reorders particles internally and restores the order on download.
+ *   - caller arrays are in the caller's (natural) particle order; the library
+ *     reorders particles internally and restores the order on download;
+ *   - every array argument may be host memory or device memory of the
+ *     context's GPU (e.g. a torch tensor's data_ptr), told apart by HIP's
+ *     unified addressing: state-slot arrays in device memory are copied by
+ *     device kernels (all arrays of one call in the same kind of memory),
+ *     inputs the library preprocesses on the host (parameters, targets, rest
+ *     positions, trajectories) are first copied in, and results are written
+ *     to whichever memory the pointer names.  Work the caller queued on
+ *     another stream into a device array must be complete before the call;
+ *     the library has finished with every array when the call returns.
  */
 #ifndef DIFFCP_H
 #define DIFFCP_H
