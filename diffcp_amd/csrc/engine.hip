@@ -321,6 +321,8 @@ static int launch_p2g(dcp_ctx* c, StateView S, int s, int g, const int* skip) {
     }
     hipLaunchKernelGGL(k_p2g_big, dim3(c->grid_big()), dim3(DCP_WG), 0, c->stream, c->sim, S, s, g, skip);
     LAUNCH_OK();
+    hipLaunchKernelGGL(k_active_list, dim3(c->grid_gridop()), dim3(DCP_WG), 0, c->stream, c->sim, skip);
+    LAUNCH_OK();
     return 0;
 }
 

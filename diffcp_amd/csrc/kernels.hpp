@@ -190,72 +190,6 @@ __device__ __forceinline__ void p2g_record(const SimDev& sim, const InstDev& I, 
     r.elem = elem;
 }
 
-__device__ __forceinline__ void p2g_contrib(const InstDev& I, const Stencil& st, const P2GRec& r, int i, int j, int k,
-                                            float& mm, float& m0, float& m1, float& m2) {
-    float w = st.w[i][0] * st.w[j][1] * st.w[k][2];
-    float dp0 = ((float)i - st.fx[0]) * I.dx, dp1 = ((float)j - st.fx[1]) * I.dx, dp2 = ((float)k - st.fx[2]) * I.dx;
-    float a0 = r.A(0, 0) * dp0 + r.A(0, 1) * dp1 + r.A(0, 2) * dp2;
-    float a1 = r.A(1, 0) * dp0 + r.A(1, 1) * dp1 + r.A(1, 2) * dp2;
-    float a2 = r.A(2, 0) * dp0 + r.A(2, 1) * dp1 + r.A(2, 2) * dp2;
-    mm = w * r.m;
-    m0 = w * (r.b.x + a0);
-    m1 = w * (r.b.y + a1);
-    m2 = w * (r.b.z + a2);
-    if (r.elem) {
-        float g0 = st.dw[i][0] * st.w[j][1] * st.w[k][2];
-        float g1 = st.w[i][0] * st.dw[j][1] * st.w[k][2];
-        float g2 = st.w[i][0] * st.w[j][1] * st.dw[k][2];
-        float sdot = (g0 * r.n.x + g1 * r.n.y + g2 * r.n.z) * I.inv_dx;
-        m0 += sdot * r.k.x; m1 += sdot * r.k.y; m2 += sdot * r.k.z;
-    }
-}
-
-// Weight and d/dfx of node i (a runtime value 0..2) of the quadratic B-spline
-// at fx, with the same expressions as simulation.py:142,197, selected from
-// scalars so that no per-lane indexed array (scratch) is needed.
-__device__ __forceinline__ void bsp_node(float fx, int i, float& w, float& dw) {
-    float a = 1.5f - fx, b = fx - 1.0f, c = fx - 0.5f;
-    float w0 = 0.5f * (a * a), w1 = 0.75f - b * b, w2 = 0.5f * (c * c);
-    float d0 = fx - 1.5f, d1 = 2.0f * (1.0f - fx), d2 = c;
-    w = i == 0 ? w0 : (i == 1 ? w1 : w2);
-    dw = i == 0 ? d0 : (i == 1 ? d1 : d2);
-}
-
-// Scatter order of a lane: at step t lane l visits stencil node (t + l) mod 27,
-// so lanes whose particles share a base cell update 27 different LDS words
-// instead of serialising on one.
-__device__ __forceinline__ void rotated_offset(int t, int lane, int& i, int& j, int& k) {
-    int o = t + lane;
-    o -= 27 * (o / 27);
-    i = o / 9;
-    j = (o / 3) % 3;
-    k = o % 3;
-}
-
-__device__ __forceinline__ void p2g_contrib_rt(const InstDev& I, const Stencil& st, const P2GRec& r, int i, int j,
-                                               int k, float& mm, float& m0, float& m1, float& m2) {
-    float W0, W1, W2, D0, D1, D2;
-    bsp_node(st.fx[0], i, W0, D0);
-    bsp_node(st.fx[1], j, W1, D1);
-    bsp_node(st.fx[2], k, W2, D2);
-    float w = W0 * W1 * W2;
-    float dp0 = ((float)i - st.fx[0]) * I.dx, dp1 = ((float)j - st.fx[1]) * I.dx, dp2 = ((float)k - st.fx[2]) * I.dx;
-    float a0 = r.A(0, 0) * dp0 + r.A(0, 1) * dp1 + r.A(0, 2) * dp2;
-    float a1 = r.A(1, 0) * dp0 + r.A(1, 1) * dp1 + r.A(1, 2) * dp2;
-    float a2 = r.A(2, 0) * dp0 + r.A(2, 1) * dp1 + r.A(2, 2) * dp2;
-    mm = w * r.m;
-    m0 = w * (r.b.x + a0);
-    m1 = w * (r.b.y + a1);
-    m2 = w * (r.b.z + a2);
-    if (r.elem) {
-        float g0 = D0 * W1 * W2;
-        float g1 = W0 * D1 * W2;
-        float g2 = W0 * W1 * D2;
-        float sdot = (g0 * r.n.x + g1 * r.n.y + g2 * r.n.z) * I.inv_dx;
-        m0 += sdot * r.k.x; m1 += sdot * r.k.y; m2 += sdot * r.k.z;
-    }
-}
-
 // y + n x for a stencil offset n in {0, 1, 2} known at compile time after
 // unrolling (no multiply by 0, which IEEE arithmetic cannot fold away)
 __device__ __forceinline__ float add_times(float y, int n, float x) {
@@ -271,6 +205,14 @@ __device__ __forceinline__ float add_times(float y, int n, float x) {
 // nn = inv_dx n: a node costs a few adds and multiplies instead of the affine
 // product and the weight gradient.  The weights are rebuilt from fx per layer
 // (a few operations) rather than kept in registers across the layer passes.
+// Points carry nn = kimp = 0, so the element term adds zeros and a wave that
+// holds both vertices and elements does not diverge.
+struct P2GSep {
+    float u[3], ad[3][3];   // ad[c][a] = dx A(a, c)
+    float nn[3], kimp[3];
+    float fx[3], m;
+};
+
 // quadratic B-spline weights and their d/dfx along one axis (simulation.py:142,197)
 __device__ __forceinline__ void bspline_axis(float f, float w[3], float dw[3]) {
     w[0] = 0.5f * (1.5f - f) * (1.5f - f);
@@ -279,6 +221,54 @@ __device__ __forceinline__ void bspline_axis(float f, float w[3], float dw[3]) {
     dw[0] = f - 1.5f;
     dw[1] = 2.0f * (1.0f - f);
     dw[2] = f - 0.5f;
+}
+
+// fx = x inv_dx - base for a (clamped) base cell, as the stencil builds it
+__device__ __forceinline__ float stencil_fx(float x, float inv_dx, int base) {
+    return scaled_position(x, inv_dx) - (float)base;
+}
+
+__device__ __forceinline__ void p2g_separate(const InstDev& I, const P2GRec& r, int b0, int b1, int b2, P2GSep& q) {
+    q.fx[0] = stencil_fx(r.x.x, I.inv_dx, b0);
+    q.fx[1] = stencil_fx(r.x.y, I.inv_dx, b1);
+    q.fx[2] = stencil_fx(r.x.z, I.inv_dx, b2);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) q.ad[c][a] = I.dx * r.A(a, c);
+        q.u[a] = r.b[a] - (q.ad[0][a] * q.fx[0] + q.ad[1][a] * q.fx[1] + q.ad[2][a] * q.fx[2]);
+    }
+    q.nn[0] = I.inv_dx * r.n.x; q.nn[1] = I.inv_dx * r.n.y; q.nn[2] = I.inv_dx * r.n.z;
+    q.kimp[0] = r.k.x; q.kimp[1] = r.k.y; q.kimp[2] = r.k.z;
+    q.m = r.m;
+}
+
+// The nine nodes (i, j, k), j, k = 0..2, of stencil layer i: emit(j, k, mass,
+// momentum x, y, z) with the weights w = W0_i W1_j W2_k (simulation.py:142-152,
+// 172-174, 186-199, 262-265: mass, APIC momentum, element normal impulse).
+template <typename Emit>
+__device__ __forceinline__ void p2g_layer(const P2GSep& q, int i, Emit emit) {
+    float w0[3], d0[3], w1[3], d1[3], w2[3], d2[3];
+    bspline_axis(q.fx[0], w0, d0);
+    bspline_axis(q.fx[1], w1, d1);
+    bspline_axis(q.fx[2], w2, d2);
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        const float w01 = w0[i] * w1[j];
+        const float al = d0[i] * w1[j] * q.nn[0] + w0[i] * d1[j] * q.nn[1];
+        const float be = w01 * q.nn[2];
+        float tij[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) tij[a] = add_times(add_times(q.u[a], i, q.ad[0][a]), j, q.ad[1][a]);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const float w = w01 * w2[k];
+            const float sd = w2[k] * al + d2[k] * be;
+            emit(j, k, w * q.m, w * add_times(tij[0], k, q.ad[2][0]) + sd * q.kimp[0],
+                 w * add_times(tij[1], k, q.ad[2][1]) + sd * q.kimp[1],
+                 w * add_times(tij[2], k, q.ad[2][2]) + sd * q.kimp[2]);
+        }
+    }
 }
 
 __device__ __forceinline__ void block_cell_coords(const InstDev& I, int blk, int lane, int& i, int& j, int& k) {
@@ -343,6 +333,7 @@ __global__ void k_state_export(const float* slot, int NP, int NE, StateIO io);
 __global__ void k_state_import(float* slot, int NP, int NE, StateIO io);
 __global__ void k_p2g(SimDev sim, StateView S, int s, int g, const int* skip);
 __global__ void k_p2g_big(SimDev sim, StateView S, int s, int g, const int* skip);
+__global__ void k_active_list(SimDev sim, const int* skip);
 __global__ void k_gridop(SimDev sim, int keep, GridSnap save, const int* skip);
 __global__ void k_grid_restore(SimDev sim, GridSnap snap);
 __global__ void k_g2p(SimDev sim, StateView S, StateView Sn);
