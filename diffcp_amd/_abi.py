@@ -42,7 +42,7 @@ class Options(ctypes.Structure):
         ("max_states", c_int), ("precision", c_int), ("svd_grad", c_int),
         ("per_element_moduli", c_int), ("patch_size", c_int), ("scatter_wgs", c_int),
         ("grid_wgs", c_int), ("bin_cells", c_int),
-        ("grid_snapshots", c_int), ("return_mapping", c_int),
+        ("grid_snapshots", c_int), ("return_mapping", c_int), ("grid_pool_blocks", c_int),
     ]
 
 
@@ -79,6 +79,7 @@ SIGNATURES = {
     "dcp_last_rollout_ms": (c_int, [c_void_p, c_dbl_p]),
     "dcp_sizes": (c_int, [c_void_p, ctypes.POINTER(ctypes.c_int64)]),
     "dcp_device_bytes": (c_int, [c_void_p, ctypes.POINTER(ctypes.c_int64)]),
+    "dcp_pool_blocks": (c_int, [c_void_p, c_int_p, c_int_p]),
     "dcp_synchronize": (c_int, [c_void_p]),
     "dcp_polar_adjoint": (c_int, [c_flt_p, c_flt_p, c_int, c_int, c_int, c_flt_p, c_flt_p]),
     "dcp_polar_adjoint_noalloc": (c_int, [c_flt_p, c_flt_p, c_int, c_int, c_int, c_flt_p, c_flt_p]),

@@ -273,6 +273,17 @@ struct dcp_ctx {
         return std::max(1, std::min(n_spatches, opt.scatter_wgs > 0 ? opt.scatter_wgs : n_cu * 16));
     }
     int grid_big() const { return n_cu; }
+    // block pool of the grids (grid_pool_blocks > 0)
+    int* btab = nullptr;
+    int pool_cap = 0;
+    bool pooled() const { return btab != nullptr; }
+    // hand the pool out afresh (a new rollout or reset state)
+    int reset_pool() {
+        if (!pooled()) return 0;
+        hipLaunchKernelGGL(k_pool_reset, dim3(std::max(1, std::min((n_flags + 255) / 256, n_cu * 4))), dim3(256), 0,
+                           stream, btab, n_flags, sim.pool_n);
+        return hipGetLastError() == hipSuccess ? 0 : -1;
+    }
     int grid_elems() const { return std::max(1, std::min((NE + DCP_WG - 1) / DCP_WG, n_cu * 16)); }
     // B x per workgroups walking the per-instance active block lists
     int grid_gridop() const {
@@ -311,15 +322,25 @@ static hipError_t time_mark(dcp_ctx* c) {
 // scatters list such patches, and k_p2g_big / k_g2p_adj_big, launched after
 // each, scatter them through the LDS tile over sub-boxes that fit it (with an
 // empty list they exit at once).
+template <bool POOL> static int launch_p2g_form(dcp_ctx* c, StateView S, int s, int g, const int* skip);
 static int launch_p2g(dcp_ctx* c, StateView S, int s, int g, const int* skip) {
+    if (c->pooled()) {
+        // pool slots for the blocks this substep's stencils reach
+        hipLaunchKernelGGL(k_pool_alloc, dim3(c->grid_patches()), dim3(DCP_WG), 0, c->stream, c->sim, S, skip);
+        LAUNCH_OK();
+    }
+    return c->pooled() ? launch_p2g_form<true>(c, S, s, g, skip) : launch_p2g_form<false>(c, S, s, g, skip);
+}
+template <bool POOL>
+static int launch_p2g_form(dcp_ctx* c, StateView S, int s, int g, const int* skip) {
     if (skip) {
-        hipLaunchKernelGGL(k_p2g, dim3(c->grid_spatches()), dim3(DCP_WG), 0, c->stream, c->sim, S, s, g, skip);
+        hipLaunchKernelGGL(k_p2g<POOL>, dim3(c->grid_spatches()), dim3(DCP_WG), 0, c->stream, c->sim, S, s, g, skip);
         LAUNCH_OK();
     } else {
-        TIMED_LAUNCH(DCP_KERNEL_P2G, k_p2g, dim3(c->grid_spatches()), dim3(DCP_WG), 0, c->stream, c->sim, S, s, g,
-                     skip);
+        TIMED_LAUNCH(DCP_KERNEL_P2G, k_p2g<POOL>, dim3(c->grid_spatches()), dim3(DCP_WG), 0, c->stream, c->sim, S, s,
+                     g, skip);
     }
-    hipLaunchKernelGGL(k_p2g_big, dim3(c->grid_big()), dim3(DCP_WG), 0, c->stream, c->sim, S, s, g, skip);
+    hipLaunchKernelGGL(k_p2g_big<POOL>, dim3(c->grid_big()), dim3(DCP_WG), 0, c->stream, c->sim, S, s, g, skip);
     LAUNCH_OK();
     hipLaunchKernelGGL(k_active_list, dim3(c->grid_gridop()), dim3(DCP_WG), 0, c->stream, c->sim, skip);
     LAUNCH_OK();
@@ -364,7 +385,10 @@ static int launch_forward(dcp_ctx* c, int s, int g, StateView S, StateView Sn, i
     if (launch_p2g(c, S, s, g, nullptr)) return -1;
     TIMED_LAUNCH(DCP_KERNEL_GRIDOP, k_gridop, dim3(c->grid_gridop()), dim3(DCP_WG), 0, c->stream, c->sim, 0, c->snap(snap),
                  nullptr);
-    TIMED_LAUNCH(DCP_KERNEL_G2P, k_g2p, dim3(c->grid_patches()), dim3(DCP_WG), 0, c->stream, c->sim, S, Sn);
+    if (c->pooled())
+        TIMED_LAUNCH(DCP_KERNEL_G2P, k_g2p<true>, dim3(c->grid_patches()), dim3(DCP_WG), 0, c->stream, c->sim, S, Sn);
+    else
+        TIMED_LAUNCH(DCP_KERNEL_G2P, k_g2p<false>, dim3(c->grid_patches()), dim3(DCP_WG), 0, c->stream, c->sim, S, Sn);
     return 0;
 }
 
@@ -406,9 +430,17 @@ static int launch_adjoint_grid(dcp_ctx* c, int s, int g, StateView S, int snap) 
 // G2P adjoint and grid-update adjoint (they need the substep's grid).
 static int launch_adjoint_head(dcp_ctx* c, StateView S, StateView G1, StateView Gout, int snap, int pending) {
     PrepSlot ps(c, prep_saved(c, snap) ? snap : -1);
-    TIMED_LAUNCH(DCP_KERNEL_G2P_ADJ, k_g2p_adj, dim3(c->grid_spatches()), dim3(DCP_WG), 0, c->stream, c->sim, S, G1,
-                 Gout, pending);
-    hipLaunchKernelGGL(k_g2p_adj_big, dim3(c->grid_big()), dim3(DCP_WG), 0, c->stream, c->sim, S, G1, Gout, pending);
+    if (c->pooled()) {
+        TIMED_LAUNCH(DCP_KERNEL_G2P_ADJ, k_g2p_adj<true>, dim3(c->grid_spatches()), dim3(DCP_WG), 0, c->stream, c->sim,
+                     S, G1, Gout, pending);
+        hipLaunchKernelGGL(k_g2p_adj_big<true>, dim3(c->grid_big()), dim3(DCP_WG), 0, c->stream, c->sim, S, G1, Gout,
+                           pending);
+    } else {
+        TIMED_LAUNCH(DCP_KERNEL_G2P_ADJ, k_g2p_adj<false>, dim3(c->grid_spatches()), dim3(DCP_WG), 0, c->stream, c->sim,
+                     S, G1, Gout, pending);
+        hipLaunchKernelGGL(k_g2p_adj_big<false>, dim3(c->grid_big()), dim3(DCP_WG), 0, c->stream, c->sim, S, G1, Gout,
+                           pending);
+    }
     LAUNCH_OK();
     TIMED_LAUNCH(DCP_KERNEL_GRIDOP_ADJ, k_gridop_adj, dim3(c->grid_gridop()), dim3(DCP_WG), 0, c->stream, c->sim);
     return 0;
@@ -416,8 +448,12 @@ static int launch_adjoint_head(dcp_ctx* c, StateView S, StateView G1, StateView 
 
 template <typename PT, bool RM>
 static int launch_elem_adj_form(dcp_ctx* c, StateView S, StateView G1, StateView Gout, int fresh, int s, int g) {
-    TIMED_LAUNCH(DCP_KERNEL_PREP_ADJ, (k_elem_adj<PT, RM>), dim3(c->grid_patches()), dim3(DCP_WG), 0, c->stream, c->sim,
-                 S, G1, Gout, fresh, s, g);
+    if (c->pooled())
+        TIMED_LAUNCH(DCP_KERNEL_PREP_ADJ, (k_elem_adj<PT, RM, true>), dim3(c->grid_patches()), dim3(DCP_WG), 0,
+                     c->stream, c->sim, S, G1, Gout, fresh, s, g);
+    else
+        TIMED_LAUNCH(DCP_KERNEL_PREP_ADJ, (k_elem_adj<PT, RM, false>), dim3(c->grid_patches()), dim3(DCP_WG), 0,
+                     c->stream, c->sim, S, G1, Gout, fresh, s, g);
     return 0;
 }
 static int launch_elem_adj(dcp_ctx* c, StateView S, StateView G1, StateView Gout, int fresh, int s, int g) {
@@ -437,7 +473,12 @@ static int launch_adjoint_tail(dcp_ctx* c, int s, int g, StateView S, StateView 
                                bool finalize) {
     PrepSlot ps(c, prep_saved(c, snap) ? snap : -1);
     const int fresh = (s == 0);
-    TIMED_LAUNCH(DCP_KERNEL_P2G_ADJ, k_p2g_adj, dim3(c->grid_patches()), dim3(DCP_WG), 0, c->stream, c->sim, S, Gout, s, g);
+    if (c->pooled())
+        TIMED_LAUNCH(DCP_KERNEL_P2G_ADJ, k_p2g_adj<true>, dim3(c->grid_patches()), dim3(DCP_WG), 0, c->stream, c->sim, S,
+                     Gout, s, g);
+    else
+        TIMED_LAUNCH(DCP_KERNEL_P2G_ADJ, k_p2g_adj<false>, dim3(c->grid_patches()), dim3(DCP_WG), 0, c->stream, c->sim,
+                     S, Gout, s, g);
     if (launch_elem_adj(c, S, G1, Gout, fresh, s, g)) return -1;
     if (finalize)
         TIMED_LAUNCH(DCP_KERNEL_VERTEX_FINAL, k_vertex_final, dim3(c->grid_patches()), dim3(DCP_WG), 0, c->stream, c->sim,
@@ -565,13 +606,27 @@ static int build(dcp_ctx* c, const dcp_instance_desc* d, int B) {
         I.E = 5e3f; I.nu = 0.3f; I.ks = 100.f; I.kc = 100.f;
     }
     c->NP = np; c->NE = ne;
-    // grid pools
+    // grids: dense regions per instance, or one shared block pool (its
+    // capacity plus the trash slot) with per-instance block tables
     long long cells = 0; int flags = 0, traj = 0;
+    const bool pooled = c->opt.grid_pool_blocks > 0;
     for (int b = 0; b < B; ++b) {
         InstDev& I = c->inst_h[b];
-        I.grid_base = cells; I.flag_base = flags; I.traj_off = traj;
+        I.grid_base = pooled ? 0 : cells; I.flag_base = flags; I.traj_off = traj;
+        I.btab = nullptr; I.pool_trash = 0;
         long long nbl = (long long)I.nb * I.nb * I.nb;
         cells += nbl * 64; flags += (int)nbl; traj += I.n_actions * I.n_grasp;
+    }
+    if (pooled) {
+        c->pool_cap = c->opt.grid_pool_blocks;
+        cells = ((long long)c->pool_cap + 1) * 64;
+        if (c->alloc(&c->btab, (size_t)flags) || c->alloc(&c->sim.pool_n, 1)) return -1;
+        HIP_OK(hipMemset(c->btab, 0xff, (size_t)flags * sizeof(int)));   // every entry -1
+        c->sim.pool_cap = c->pool_cap;
+        for (int b = 0; b < B; ++b) {
+            c->inst_h[b].btab = c->btab + c->inst_h[b].flag_base;
+            c->inst_h[b].pool_trash = c->pool_cap;
+        }
     }
     c->grid_cells = cells; c->n_flags = flags;
     // static element data in internal order
@@ -837,6 +892,17 @@ extern "C" int dcp_sizes(dcp_ctx* c, int64_t* o) {
     return 0;
 }
 
+extern "C" int dcp_pool_blocks(dcp_ctx* c, int32_t* used, int32_t* capacity) {
+    if (!c || !used || !capacity) return set_err("bad args");
+    *used = 0;
+    *capacity = c->pool_cap;
+    if (c->pooled()) {
+        HIP_OK(hipMemcpyAsync(used, c->sim.pool_n, sizeof(int32_t), hipMemcpyDefault, c->stream));
+        HIP_OK(hipStreamSynchronize(c->stream));
+    }
+    return 0;
+}
+
 extern "C" int dcp_device_bytes(dcp_ctx* c, int64_t* out) {
     if (!c || !out) return set_err("bad args");
     *out = (int64_t)(c->alloc_bytes + c->snap_bytes);
@@ -1002,6 +1068,7 @@ extern "C" int dcp_reset(dcp_ctx* c, int32_t slot, const double* vx, const doubl
     HIP_OK(hipMemcpyAsync((void*)c->sim.Dinvinv, Dii.data(), Dii.size() * 4, hipMemcpyHostToDevice, c->stream));
     HIP_OK(hipMemcpyAsync(c->states + (size_t)slot * c->slot_floats, st.data(), st.size() * 4, hipMemcpyHostToDevice,
                           c->stream));
+    if (c->reset_pool()) return set_err("dcp_reset: pool reset launch failed");
     HIP_OK(hipStreamSynchronize(c->stream));
     c->rebase_pending = false;
     return 0;
@@ -1559,7 +1626,9 @@ extern "C" int dcp_rollout_grad(dcp_ctx* c, int32_t S, int32_t g, int32_t K, int
     if (plan_rollout(c, S, K, p)) return -1;
     if (ensure_snapshots(c, p.K)) return -1;
     if (c->n_targets_total == 0) return set_err("no targets set (dcp_set_targets)");
-    // the reset state must already be in slot 0 (dcp_reset)
+    // the reset state must already be in slot 0 (dcp_reset); pooled grids
+    // start the rollout with an empty block table
+    if (c->reset_pool()) return set_err("pool reset launch failed");
     // timing events restart with an eager rollout or with a plan's first capture
     if (!use_graph || !c->graphs.count(std::make_tuple(0, 0, g, p.S, p.K))) c->tev_used = 0;
     HIP_OK(hipEventRecord(c->ev0, c->stream));
@@ -1597,6 +1666,7 @@ extern "C" int dcp_rollout(dcp_ctx* c, int32_t S, int32_t g, int32_t use_graph, 
     // use slots 1 and 2 after the reset slot 0
     if (c->max_states < 3) return set_err("dcp_rollout needs max_states >= 3");
     c->tev_used = 0;
+    if (c->reset_pool()) return set_err("pool reset launch failed");
     HIP_OK(hipEventRecord(c->ev0, c->stream));
     int cur = 0;
     for (int f = 0; f < S; ++f) {

@@ -331,12 +331,14 @@ struct StateIO {
 };
 __global__ void k_state_export(const float* slot, int NP, int NE, StateIO io);
 __global__ void k_state_import(float* slot, int NP, int NE, StateIO io);
-__global__ void k_p2g(SimDev sim, StateView S, int s, int g, const int* skip);
-__global__ void k_p2g_big(SimDev sim, StateView S, int s, int g, const int* skip);
+template <bool POOL> __global__ void k_p2g(SimDev sim, StateView S, int s, int g, const int* skip);
+template <bool POOL> __global__ void k_p2g_big(SimDev sim, StateView S, int s, int g, const int* skip);
 __global__ void k_active_list(SimDev sim, const int* skip);
+__global__ void k_pool_alloc(SimDev sim, StateView S, const int* skip);
+__global__ void k_pool_reset(int* btab, int n, int* pool_n);
 __global__ void k_gridop(SimDev sim, int keep, GridSnap save, const int* skip);
 __global__ void k_grid_restore(SimDev sim, GridSnap snap);
-__global__ void k_g2p(SimDev sim, StateView S, StateView Sn);
+template <bool POOL> __global__ void k_g2p(SimDev sim, StateView S, StateView Sn);
 __global__ void k_loss_min(SimDev sim, StateView S, const float* targets, const int4* tchunks, int n_chunks,
                            int n_parts, int n_targets, const int* vnat, float* pmin, int* pnat, int* parg);
 __global__ void k_loss_sum(const float* pmin, const int* pnat, const int* parg, int n_parts, int n_targets,
@@ -345,11 +347,11 @@ __global__ void k_binning_keys(SimDev sim, StateView S, int* keys);
 __global__ void k_patch_order(SimDev sim, StateView S, int* out);
 __global__ void k_rebase(SimDev sim, StateView S, float* f_old, float* Dinv, float* Dinvinv);
 
-__global__ void k_g2p_adj(SimDev sim, StateView S, StateView G1, StateView Gout, int pending);
-__global__ void k_g2p_adj_big(SimDev sim, StateView S, StateView G1, StateView Gout, int pending);
+template <bool POOL> __global__ void k_g2p_adj(SimDev sim, StateView S, StateView G1, StateView Gout, int pending);
+template <bool POOL> __global__ void k_g2p_adj_big(SimDev sim, StateView S, StateView G1, StateView Gout, int pending);
 __global__ void k_gridop_adj(SimDev sim);
-__global__ void k_p2g_adj(SimDev sim, StateView S, StateView Gout, int s, int g);
-template <typename PT, bool RM>
+template <bool POOL> __global__ void k_p2g_adj(SimDev sim, StateView S, StateView Gout, int s, int g);
+template <typename PT, bool RM, bool POOL>
 __global__ void k_elem_adj(SimDev sim, StateView S, StateView G1, StateView Gout, int fresh, int s, int g);
 __global__ void k_vertex_final(SimDev sim, StateView Gout);
 template <typename PT> __global__ void k_polar_adj(const float* Ft, const float* gR, int n, int mode, float* gF, float* R);

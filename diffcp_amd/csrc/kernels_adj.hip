@@ -121,6 +121,7 @@ __device__ __forceinline__ void g2p_adj_store(const SimDev& sim, const InstDev& 
     else store_v3(Gout.px(0), sim.NP, P.start + q, gx1 + gxt);
 }
 
+template <bool POOL>
 __device__ __forceinline__ void g2p_adj_spill(const SimDev& sim, const StateView& S, const StateView& G1,
                                            const StateView& Gout, const Patch& P, const InstDev& I, int pending) {
     for (int q = threadIdx.x; q < P.count; q += DCP_WG) {
@@ -135,9 +136,9 @@ __device__ __forceinline__ void g2p_adj_spill(const SimDev& sim, const StateView
         for (int i = 0; i < 3; ++i)
             g2p_adj_layer(
                 sep, i,
-                [&](int j, int k) { return sim.vout[cell_index(I, st.base[0] + i, st.base[1] + j, st.base[2] + k)]; },
+                [&](int j, int k) { return sim.vout[cell_index<POOL>(I, st.base[0] + i, st.base[1] + j, st.base[2] + k)]; },
                 [&](int j, int k, float g0, float g1, float g2) {
-                    long long gi = cell_index(I, st.base[0] + i, st.base[1] + j, st.base[2] + k);
+                    long long gi = cell_index<POOL>(I, st.base[0] + i, st.base[1] + j, st.base[2] + k);
                     atomicAdd(&sim.gacc[gi].y, g0); atomicAdd(&sim.gacc[gi].z, g1); atomicAdd(&sim.gacc[gi].w, g2);
                 });
         float gfx[3];
@@ -156,6 +157,7 @@ __device__ __forceinline__ void g2p_adj_spill(const SimDev& sim, const StateView
 // G2P reverse, through one LDS tile over box B, of the patch particles whose
 // base cells lie in B (`act` lanes; B is the patch's box, or one sub-box of it
 // in k_g2p_adj_big): node terms, dL/dx stores, and the flush of dL/d grid_v_out.
+template <bool POOL>
 __device__ __forceinline__ void g2p_adj_tile(const SimDev& sim, const InstDev& I, const StateView& Gout,
                                              const PatchBox& B, bool act, const V3<float>& x, const V3<float>& gv,
                                              const V3<float>& gx1, const M3<float>& gC, const Patch& Pl, int ql,
@@ -173,7 +175,7 @@ __device__ __forceinline__ void g2p_adj_tile(const SimDev& sim, const InstDev& I
         const int ck = c % B.e2, t = c / B.e2;
         nci[h] = t / B.e1;
         a[h] = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (has[h]) vt[c] = sim.vout[cell_index(I, nci[h] + B.l0, t - nci[h] * B.e1 + B.l1, ck + B.l2)];
+        if (has[h]) vt[c] = sim.vout[cell_index<POOL>(I, nci[h] + B.l0, t - nci[h] * B.e1 + B.l1, ck + B.l2)];
     }
     __syncthreads();
     // tile index of the particle's base cell; node (i, j, k) is key + (i * e1 + j) * e2 + k
@@ -213,7 +215,7 @@ __device__ __forceinline__ void g2p_adj_tile(const SimDev& sim, const InstDev& I
     for (int h = 0; h < DCP_NODES; ++h) {
         if (has[h] && (a[h].y != 0.f || a[h].z != 0.f || a[h].w != 0.f)) {
             const int c = tid + h * DCP_WG, t = c / B.e2;
-            long long gidx = cell_index(I, nci[h] + B.l0, t - nci[h] * B.e1 + B.l1, c - t * B.e2 + B.l2);
+            long long gidx = cell_index<POOL>(I, nci[h] + B.l0, t - nci[h] * B.e1 + B.l1, c - t * B.e2 + B.l2);
             atomicAdd(&sim.gacc[gidx].y, a[h].y); atomicAdd(&sim.gacc[gidx].z, a[h].z);
             atomicAdd(&sim.gacc[gidx].w, a[h].w);
         }
@@ -237,6 +239,7 @@ __device__ __forceinline__ void g2p_adj_patch_inputs(const SimDev& sim, const In
 // tile reverse is written out here rather than called as g2p_adj_tile
 // (k_g2p_adj_big's): the shared form compiled to a schedule measured 3 %
 // slower on this kernel.
+template <bool POOL>
 __global__ __launch_bounds__(DCP_WG, 3) void k_g2p_adj(SimDev sim, StateView S, StateView G1, StateView Gout, int pending) {
     __shared__ BinLDS L;
     __shared__ float2 cA[DCP_CONTRIB_ZERO + 1];   // contributions, channels 0 and 1
@@ -282,7 +285,7 @@ __global__ __launch_bounds__(DCP_WG, 3) void k_g2p_adj(SimDev sim, StateView S, 
                     const int ck = c % B.e2, t = c / B.e2;
                     nci[h] = t / B.e1;
                     a[h] = make_float4(0.f, 0.f, 0.f, 0.f);
-                    if (has[h]) vt[c] = sim.vout[cell_index(I, nci[h] + B.l0, t - nci[h] * B.e1 + B.l1, ck + B.l2)];
+                    if (has[h]) vt[c] = sim.vout[cell_index<POOL>(I, nci[h] + B.l0, t - nci[h] * B.e1 + B.l1, ck + B.l2)];
                 }
                 __syncthreads();
                 // tile index of the particle's base cell; node (i, j, k) is key + (i * e1 + j) * e2 + k
@@ -322,7 +325,7 @@ __global__ __launch_bounds__(DCP_WG, 3) void k_g2p_adj(SimDev sim, StateView S, 
                 for (int h = 0; h < DCP_NODES; ++h) {
                     if (has[h] && (a[h].y != 0.f || a[h].z != 0.f || a[h].w != 0.f)) {
                         const int c = tid + h * DCP_WG, t = c / B.e2;
-                        long long gidx = cell_index(I, nci[h] + B.l0, t - nci[h] * B.e1 + B.l1, c - t * B.e2 + B.l2);
+                        long long gidx = cell_index<POOL>(I, nci[h] + B.l0, t - nci[h] * B.e1 + B.l1, c - t * B.e2 + B.l2);
                         atomicAdd(&sim.gacc[gidx].y, a[h].y); atomicAdd(&sim.gacc[gidx].z, a[h].z);
                         atomicAdd(&sim.gacc[gidx].w, a[h].w);
                     }
@@ -340,6 +343,7 @@ __global__ __launch_bounds__(DCP_WG, 3) void k_g2p_adj(SimDev sim, StateView S, 
 // The patches k_g2p_adj left: the same tile reverse over the sub-boxes of the
 // patch box (as k_p2g_big).  k_gridop_adj, next on the stream, empties the
 // list.
+template <bool POOL>
 __global__ __launch_bounds__(DCP_WG, 2) void k_g2p_adj_big(SimDev sim, StateView S, StateView G1, StateView Gout,
                                                            int pending) {
     const int n_big = sim.big_n[DCP_BIG_G2P_ADJ];
@@ -381,12 +385,12 @@ __global__ __launch_bounds__(DCP_WG, 2) void k_g2p_adj_big(SimDev sim, StateView
                             __syncthreads();
                         }
                         first = false;
-                        g2p_adj_tile(sim, I, Gout, B, in, x, gv, gx1, gC, Pl, ql, b0, b1, b2, L, cA, cB, vt);
+                        g2p_adj_tile<POOL>(sim, I, Gout, B, in, x, gv, gx1, gC, Pl, ql, b0, b1, b2, L, cA, cB, vt);
                     }
         }
         if (!tiled) {
-            g2p_adj_spill(sim, S, G1, Gout, SP.part(0), I, pending);
-            g2p_adj_spill(sim, S, G1, Gout, SP.part(1), I, pending);
+            g2p_adj_spill<POOL>(sim, S, G1, Gout, SP.part(0), I, pending);
+            g2p_adj_spill<POOL>(sim, S, G1, Gout, SP.part(1), I, pending);
         }
         bin_clear(L);
         if (tid < 3) { L.box[tid] = 1 << 30; L.box[3 + tid] = -(1 << 30); }
@@ -407,7 +411,7 @@ __global__ __launch_bounds__(DCP_WG) void k_gridop_adj(SimDev sim) {
     double gdamp_wave = 0.0;
     for (int it = w.first; it < w.n; it += w.stride) {
         const int blk = sim.active[I.flag_base + it];
-        const long long cell = I.grid_base + (long long)blk * 64 + lane;
+        const long long cell = block_cell0(I, blk) + lane;
         int ci, cj, ck;
         block_cell_coords(I, blk, lane, ci, cj, ck);
         float4 c = sim.acc[cell];
@@ -606,6 +610,7 @@ __device__ __forceinline__ void grasper_traj_grad(const SimDev& sim, const InstD
 // tile was staged; the caller's lane holds particle tid when `act`.
 // `next_box`, when given, is reset here for the next patch (after every lane
 // has read this patch's box at the barrier of box_reduce).
+template <bool POOL>
 __device__ __forceinline__ bool p2g_adj_stage(const SimDev& sim, const InstDev& I, const StateView& S,
                                               const Patch& P, int s, int g, bool act, P2GRec& r, Stencil& st,
                                               int* box, float4* gt, PatchBox& B, int* next_box = nullptr) {
@@ -625,7 +630,7 @@ __device__ __forceinline__ bool p2g_adj_stage(const SimDev& sim, const InstDev& 
             const int c = tid + h * DCP_WG;
             if (c < B.vol) {
                 const int ck = c % B.e2, t = c / B.e2;
-                gt[c] = sim.gmv[cell_index(I, t / B.e1 + B.l0, t % B.e1 + B.l1, ck + B.l2)];
+                gt[c] = sim.gmv[cell_index<POOL>(I, t / B.e1 + B.l0, t % B.e1 + B.l1, ck + B.l2)];
             }
         }
     }
@@ -637,6 +642,7 @@ __device__ __forceinline__ bool p2g_adj_stage(const SimDev& sim, const InstDev& 
 // dL/d(m, m v) of the patch tile is staged in LDS once per patch; a patch
 // whose tile exceeds the cap reads the grid.  Element patches are left to
 // k_elem_adj, which needs the vertex force-impulse gradients stored here.
+template <bool POOL>
 __global__ __launch_bounds__(DCP_WG, 2) void k_p2g_adj(SimDev sim, StateView S, StateView Gout, int s, int g) {
     __shared__ int box[6];
     __shared__ float4 gt[DCP_STILE_CAP];
@@ -651,7 +657,7 @@ __global__ __launch_bounds__(DCP_WG, 2) void k_p2g_adj(SimDev sim, StateView S, 
         P2GRec r;
         Stencil st;
         PatchBox B;
-        const bool tiled = p2g_adj_stage(sim, I, S, P, s, g, act, r, st, box, gt, B);
+        const bool tiled = p2g_adj_stage<POOL>(sim, I, S, P, s, g, act, r, st, box, gt, B);
         P2GAdj o;
         if (act) {
             if (tiled) {
@@ -659,7 +665,7 @@ __global__ __launch_bounds__(DCP_WG, 2) void k_p2g_adj(SimDev sim, StateView S, 
                 p2g_adj_gather<false>(I, r, st, [&](int i, int j, int k) { return gt[key + (i * B.e1 + j) * B.e2 + k]; }, o);
             } else {
                 p2g_adj_gather<false>(I, r, st, [&](int i, int j, int k) {
-                    return sim.gmv[cell_index(I, st.base[0] + i, st.base[1] + j, st.base[2] + k)];
+                    return sim.gmv[cell_index<POOL>(I, st.base[0] + i, st.base[1] + j, st.base[2] + k)];
                 }, o);
             }
             p2g_adj_store_point(sim, I, Gout, P, tid, r, o);
@@ -773,7 +779,7 @@ __device__ __forceinline__ void elem_adjoint(const SimDev& sim, const InstDev& I
 // prep adjoint in registers.  Parameter gradients are reduced per patch in
 // FP64 in a fixed order.  Runs after k_p2g_adj (the vertex force-impulse
 // gradients).
-template <typename PT, bool RM>
+template <typename PT, bool RM, bool POOL>
 __global__ __launch_bounds__(DCP_WG, 2) void k_elem_adj(SimDev sim, StateView S, StateView G1, StateView Gout,
                                                         int fresh, int s, int g) {
     // two boxes, alternating by patch: the one of the next patch is reset
@@ -793,7 +799,7 @@ __global__ __launch_bounds__(DCP_WG, 2) void k_elem_adj(SimDev sim, StateView S,
         P2GRec r;
         Stencil st;
         PatchBox B;
-        const bool tiled = p2g_adj_stage(sim, I, S, P, s, g, act, r, st, box[par], gt, B, box[par ^ 1]);
+        const bool tiled = p2g_adj_stage<POOL>(sim, I, S, P, s, g, act, r, st, box[par], gt, B, box[par ^ 1]);
         par ^= 1;
         double acc[DCP_PATCH_GRADS] = {0.0, 0.0, 0.0, 0.0, 0.0};
         if (act) {
@@ -803,7 +809,7 @@ __global__ __launch_bounds__(DCP_WG, 2) void k_elem_adj(SimDev sim, StateView S,
                 p2g_adj_gather<true>(I, r, st, [&](int i, int j, int k) { return gt[key + (i * B.e1 + j) * B.e2 + k]; }, o);
             } else {
                 p2g_adj_gather<true>(I, r, st, [&](int i, int j, int k) {
-                    return sim.gmv[cell_index(I, st.base[0] + i, st.base[1] + j, st.base[2] + k)];
+                    return sim.gmv[cell_index<POOL>(I, st.base[0] + i, st.base[1] + j, st.base[2] + k)];
                 }, o);
             }
             // the gather's live values end here: keep the prep adjoint's loads below
@@ -898,7 +904,16 @@ __global__ void k_polar_adj(const float* Ft, const float* gR, int n, int mode, f
 template __global__ void k_polar_adj<double>(const float*, const float*, int, int, float*, float*);
 template __global__ void k_polar_adj<float>(const float*, const float*, int, int, float*, float*);
 
-template __global__ void k_elem_adj<double, false>(SimDev, StateView, StateView, StateView, int, int, int);
-template __global__ void k_elem_adj<float, false>(SimDev, StateView, StateView, StateView, int, int, int);
-template __global__ void k_elem_adj<double, true>(SimDev, StateView, StateView, StateView, int, int, int);
-template __global__ void k_elem_adj<float, true>(SimDev, StateView, StateView, StateView, int, int, int);
+#define DCP_ELEM_ADJ(PT, RM, POOL) \
+    template __global__ void k_elem_adj<PT, RM, POOL>(SimDev, StateView, StateView, StateView, int, int, int);
+DCP_ELEM_ADJ(double, false, false) DCP_ELEM_ADJ(float, false, false)
+DCP_ELEM_ADJ(double, true, false) DCP_ELEM_ADJ(float, true, false)
+DCP_ELEM_ADJ(double, false, true) DCP_ELEM_ADJ(float, false, true)
+DCP_ELEM_ADJ(double, true, true) DCP_ELEM_ADJ(float, true, true)
+#undef DCP_ELEM_ADJ
+template __global__ void k_g2p_adj<false>(SimDev, StateView, StateView, StateView, int);
+template __global__ void k_g2p_adj<true>(SimDev, StateView, StateView, StateView, int);
+template __global__ void k_g2p_adj_big<false>(SimDev, StateView, StateView, StateView, int);
+template __global__ void k_g2p_adj_big<true>(SimDev, StateView, StateView, StateView, int);
+template __global__ void k_p2g_adj<false>(SimDev, StateView, StateView, int, int);
+template __global__ void k_p2g_adj<true>(SimDev, StateView, StateView, int, int);

@@ -88,6 +88,7 @@ __device__ __forceinline__ TileBlocks tile_blocks(int l0, int l1, int l2, int e0
 
 // Particles of a patch that does not fit the sorted tile scatter straight into
 // the grid with float atomics (slow path, flagged in the status word).
+template <bool POOL>
 __device__ __forceinline__ void p2g_spill(const SimDev& sim, const StateView& S, const Patch& P, const InstDev& I, int s,
                                        int g) {
     for (int q = threadIdx.x; q < P.count; q += DCP_WG) {
@@ -100,7 +101,7 @@ __device__ __forceinline__ void p2g_spill(const SimDev& sim, const StateView& S,
 #pragma unroll
         for (int i = 0; i < 3; ++i)
             p2g_layer(sq, i, [&](int j, int k, float mm, float m0, float m1, float m2) {
-                long long gi = cell_index(I, st.base[0] + i, st.base[1] + j, st.base[2] + k);
+                long long gi = cell_index<POOL>(I, st.base[0] + i, st.base[1] + j, st.base[2] + k);
                 atomicAdd(&sim.acc[gi].x, mm); atomicAdd(&sim.acc[gi].y, m0);
                 atomicAdd(&sim.acc[gi].z, m1); atomicAdd(&sim.acc[gi].w, m2);
             });
@@ -113,11 +114,12 @@ __device__ __forceinline__ void p2g_spill(const SimDev& sim, const StateView& S,
 }
 
 // Add one tile node's (m, m v) into the instance grid and note its 4^3 block.
+template <bool POOL>
 __device__ __forceinline__ void p2g_flush_node(const SimDev& sim, const InstDev& I, const Patch& P, const PatchBox& B,
                                                const TileBlocks& TB, int* tb_touched, int ci, int cj, int ck, float4 a) {
     if (a.x == 0.f && a.y == 0.f && a.z == 0.f && a.w == 0.f) return;
     ci += B.l0; cj += B.l1; ck += B.l2;
-    long long gi = cell_index(I, ci, cj, ck);
+    long long gi = cell_index<POOL>(I, ci, cj, ck);
     atomicAdd(&sim.acc[gi].x, a.x); atomicAdd(&sim.acc[gi].y, a.y);
     atomicAdd(&sim.acc[gi].z, a.z); atomicAdd(&sim.acc[gi].w, a.w);
     if (TB.n <= DCP_TB_CAP)
@@ -165,6 +167,7 @@ __device__ __forceinline__ void p2g_patch_record(const SimDev& sim, const InstDe
 // Scatter, through one LDS tile over box B, of the patch particles whose base
 // cells lie in B (`act` lanes; B is the patch's box, or one sub-box of it in
 // k_p2g_big).
+template <bool POOL>
 __device__ __forceinline__ void p2g_tile(const SimDev& sim, const InstDev& I, const Patch& P, const PatchBox& B,
                                          bool act, const P2GRec& r, int b0, int b1, int b2, BinLDS& L,
                                          float4* contrib, int* tb_touched) {
@@ -206,7 +209,7 @@ __device__ __forceinline__ void p2g_tile(const SimDev& sim, const InstDev& I, co
     for (int h = 0; h < DCP_NODES; ++h)
         if (has[h]) {
             const int t = (tid + h * DCP_WG) / B.e2;
-            p2g_flush_node(sim, I, P, B, TB, tb_touched, nci[h], t - nci[h] * B.e1,
+            p2g_flush_node<POOL>(sim, I, P, B, TB, tb_touched, nci[h], t - nci[h] * B.e1,
                            tid + h * DCP_WG - t * B.e2, a[h]);
         }
     __syncthreads();
@@ -216,6 +219,7 @@ __device__ __forceinline__ void p2g_tile(const SimDev& sim, const InstDev& I, co
 // A patch whose tile overflows is listed for k_p2g_big.  The tile scatter is
 // written out here rather than called as p2g_tile (k_p2g_big's): the shared
 // form compiled to a schedule measured 2 % slower on this kernel.
+template <bool POOL>
 __global__ __launch_bounds__(DCP_WG, 2) void k_p2g(SimDev sim, StateView S, int s, int g, const int* skip) {
     if (skip && *skip) return;   // adjoint substep whose grid is restored from a snapshot
     __shared__ BinLDS L;
@@ -285,7 +289,7 @@ __global__ __launch_bounds__(DCP_WG, 2) void k_p2g(SimDev sim, StateView S, int 
                 for (int h = 0; h < DCP_NODES; ++h)
                     if (has[h]) {
                         const int t = (tid + h * DCP_WG) / B.e2;
-                        p2g_flush_node(sim, I, P, B, TB, tb_touched, nci[h], t - nci[h] * B.e1,
+                        p2g_flush_node<POOL>(sim, I, P, B, TB, tb_touched, nci[h], t - nci[h] * B.e1,
                                        tid + h * DCP_WG - t * B.e2, a[h]);
                     }
                 __syncthreads();
@@ -306,6 +310,7 @@ __global__ __launch_bounds__(DCP_WG, 2) void k_p2g(SimDev sim, StateView S, int 
 // of more than DCP_WG particles (none by construction) would scatter per
 // particle with global atomics (status bit DCP_STATUS_TILE_SPILL).  k_gridop,
 // after the list scan, empties the list.
+template <bool POOL>
 __global__ __launch_bounds__(DCP_WG, 2) void k_p2g_big(SimDev sim, StateView S, int s, int g, const int* skip) {
     if (skip && *skip) return;
     const int n_big = sim.big_n[DCP_BIG_P2G];
@@ -344,18 +349,64 @@ __global__ __launch_bounds__(DCP_WG, 2) void k_p2g_big(SimDev sim, StateView S, 
                             __syncthreads();
                         }
                         first = false;
-                        p2g_tile(sim, I, P, B, in, r, b0, b1, b2, L, contrib, tb_touched);
+                        p2g_tile<POOL>(sim, I, P, B, in, r, b0, b1, b2, L, contrib, tb_touched);
                     }
         }
         if (!tiled) {
             if (tid == 0) atomicOr(sim.status, (unsigned)DCP_STATUS_TILE_SPILL);
-            p2g_spill(sim, S, SP.part(0), I, s, g);
-            p2g_spill(sim, S, SP.part(1), I, s, g);
+            p2g_spill<POOL>(sim, S, SP.part(0), I, s, g);
+            p2g_spill<POOL>(sim, S, SP.part(1), I, s, g);
         }
         bin_clear(L);
         if (tid < 3) { L.box[tid] = 1 << 30; L.box[3 + tid] = -(1 << 30); }
         __syncthreads();
     }
+}
+
+// ---------------------------------------------------------------------------
+// grid block pool
+// ---------------------------------------------------------------------------
+// Before each scatter (pooled grids only): every 4^3 block a particle's
+// stencil reaches (its base cell b .. b + 2 per axis: one or two blocks per
+// axis) gets a pool slot if it has none.  The first lane to move a table
+// entry from -1 to -2 takes the next slot and publishes it; the others go on
+// (the scatter, the next launch, reads the published slots).  When the pool
+// is full the block is given the trash slot and the status word says so.
+__global__ __launch_bounds__(DCP_WG) void k_pool_alloc(SimDev sim, StateView S, const int* skip) {
+    if (skip && *skip) return;
+    for (int pi = blockIdx.x; pi < sim.n_patches; pi += gridDim.x) {
+        const Patch P = sim.patches[pi];
+        const InstDev I = sim.inst[P.inst];
+        for (int q = threadIdx.x; q < P.count; q += DCP_WG) {
+            const V3<float> x = particle_x(sim, S, P, q);
+            int lo[3], hi[3];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                const int b = stencil_base(x[d], I.inv_dx, I.G);
+                lo[d] = b >> 2;
+                hi[d] = (b + 2) >> 2;
+            }
+            for (int bx = lo[0]; bx <= hi[0]; ++bx)
+                for (int by = lo[1]; by <= hi[1]; ++by)
+                    for (int bz = lo[2]; bz <= hi[2]; ++bz) {
+                        int* t = I.btab + (bx * I.nb + by) * I.nb + bz;
+                        if (*t == -1 && atomicCAS(t, -1, -2) == -1) {
+                            int slot = atomicAdd(sim.pool_n, 1);
+                            if (slot >= sim.pool_cap) {
+                                slot = I.pool_trash;
+                                atomicOr(sim.status, (unsigned)DCP_STATUS_POOL_FULL);
+                            }
+                            atomicExch(t, slot);
+                        }
+                    }
+        }
+    }
+}
+
+// Empty the block table (every entry -1) and hand the pool out afresh.
+__global__ void k_pool_reset(int* btab, int n, int* pool_n) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *pool_n = 0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) btab[i] = -1;
 }
 
 // ---------------------------------------------------------------------------
@@ -442,7 +493,7 @@ __global__ __launch_bounds__(DCP_WG) void k_gridop(SimDev sim, int keep, GridSna
     const bool saving = save.acc && total <= save.cap;
     for (int it = w.first; it < w.n; it += w.stride) {
         const int blk = sim.active[I.flag_base + it];
-        const long long cell = I.grid_base + (long long)blk * 64 + lane;
+        const long long cell = block_cell0(I, blk) + lane;
         int ci, cj, ck;
         block_cell_coords(I, blk, lane, ci, cj, ck);
         float4 c = sim.acc[cell];
@@ -484,7 +535,7 @@ __global__ __launch_bounds__(DCP_WG) void k_grid_restore(SimDev sim, GridSnap sn
     if (first == 0 && lane == 0) sim.n_active[b] = n;
     for (int it = first; it < n; it += stride) {
         const int blk = snap.blk[base + it];
-        const long long cell = I.grid_base + (long long)blk * 64 + lane;
+        const long long cell = block_cell0(I, blk) + lane;
         int ci, cj, ck;
         block_cell_coords(I, blk, lane, ci, cj, ck);
         const float4 c = snap.acc[(size_t)(base + it) * 64 + lane];
@@ -502,6 +553,7 @@ __global__ __launch_bounds__(DCP_WG) void k_grid_restore(SimDev sim, GridSnap sn
 // ---------------------------------------------------------------------------
 // G2P
 // ---------------------------------------------------------------------------
+template <bool POOL>
 __global__ __launch_bounds__(DCP_WG) void k_g2p(SimDev sim, StateView S, StateView Sn) {
     const int tid = threadIdx.x;
     for (int pi = blockIdx.x; pi < sim.n_patches; pi += gridDim.x) {
@@ -528,7 +580,7 @@ __global__ __launch_bounds__(DCP_WG) void k_g2p(SimDev sim, StateView S, StateVi
                     float tW[3] = {0.f, 0.f, 0.f}, tP[3] = {0.f, 0.f, 0.f};
 #pragma unroll
                     for (int k = 0; k < 3; ++k) {
-                        const float4 gv = sim.vout[cell_index(I, st.base[0] + i, st.base[1] + j, st.base[2] + k)];
+                        const float4 gv = sim.vout[cell_index<POOL>(I, st.base[0] + i, st.base[1] + j, st.base[2] + k)];
                         const float w2 = st.w[k][2], wp2 = w2 * ((float)k - st.fx[2]);
                         tW[0] += w2 * gv.x; tW[1] += w2 * gv.y; tW[2] += w2 * gv.z;
                         tP[0] += wp2 * gv.x; tP[1] += wp2 * gv.y; tP[2] += wp2 * gv.z;
@@ -762,6 +814,12 @@ __global__ void k_binning_keys(SimDev sim, StateView S, int* keys) {
     }
 }
 
+template __global__ void k_p2g<false>(SimDev, StateView, int, int, const int*);
+template __global__ void k_p2g<true>(SimDev, StateView, int, int, const int*);
+template __global__ void k_p2g_big<false>(SimDev, StateView, int, int, const int*);
+template __global__ void k_p2g_big<true>(SimDev, StateView, int, int, const int*);
+template __global__ void k_g2p<false>(SimDev, StateView, StateView);
+template __global__ void k_g2p<true>(SimDev, StateView, StateView);
 template __global__ void k_prep<double, false>(SimDev, StateView, StateView, int);
 template __global__ void k_prep<float, false>(SimDev, StateView, StateView, int);
 template __global__ void k_prep<double, true>(SimDev, StateView, StateView, int);

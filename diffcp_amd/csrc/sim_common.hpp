@@ -6,8 +6,11 @@
 //   a state slot = x[3][NP] v[3][NP] C[9][NP] | Ce[9][NE] n[3][NE]   (floats)
 //     n = third column of d at the previous substep (the rest-shape normal
 //     at a fresh state); F and d are rebuilt from the vertex positions.
-//   grid     = one dense G^3 region per instance, cells in 4x4x4 blocks
-//     (64 cells = 1 KB of float4 contiguous), only active blocks touched.
+//   grid     = cells in 4x4x4 blocks (64 cells = 1 KB of float4 contiguous):
+//     either one dense G^3 region per instance (only active blocks touched),
+//     or one shared pool of blocks with a per-instance block table (a block
+//     gets a pool slot the first substep a particle stencil reaches it and
+//     keeps it until the table is reset at the start of a rollout).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -21,7 +24,9 @@ enum : int { CLS_VERTEX = 0, CLS_GRASPER = 1, CLS_CUBE = 2, CLS_ELEMENT = 3 };
 
 struct InstDev {
     int G, nb;                     // grid side, blocks per side (ceil(G/4))
-    long long grid_base;           // first cell of this instance in the grid pool
+    long long grid_base;           // first cell of this instance's dense region (0 with a block pool)
+    int* btab;                     // block pool: this instance's block -> pool slot (-1: none); null: dense
+    int pool_trash;                // block pool: slot that stands in for a block without one
     int flag_base;                 // first block flag of this instance
     int steps_per_frame, frames_per_window, group_size, n_actions, n_grasp, traj_off;
     int v_start, v_count, o_start, o_count, c_start, c_count, e_start, e_count;
@@ -99,6 +104,8 @@ struct SimDev {
     int* active;                   // active block ids, per instance at flag_base
     int* n_active;                 // [B] active blocks per instance
     unsigned* status;
+    int* pool_n;                   // block pool: slots handed out (null: dense grids)
+    int pool_cap;                  // block pool: slots available (the trash slot follows them)
     // adjoint temporaries
     float* g_xe;                   // [3][NE] dL/dx_e of the G2P reverse
     float* g_fv;                   // [3][NP] d/d (vertex force impulse)
@@ -257,8 +264,29 @@ __device__ __forceinline__ void make_stencil(V3<float> x, float inv_dx, int G, S
     if (out && status) atomicOr(status, 2u);
 }
 
+// First cell of 4^3 block blk of an instance grid: its place in the dense
+// region, or the pool slot its table entry names.  A block without a slot
+// (never reached by a stencil: a tile corner staged but never read) reads
+// and writes the trash slot, so every access stays inside the pool.  Per-node
+// kernels take the form as a template parameter (POOL), so that the dense
+// kernels carry no table code; per-block kernels test I.btab.
+__device__ __forceinline__ long long block_cell0(const InstDev& I, int blk) {
+    if (I.btab) {
+        const int t = I.btab[blk];
+        return (long long)(t >= 0 ? t : I.pool_trash) * 64;
+    }
+    return I.grid_base + (long long)blk * 64;
+}
 // blocked cell index inside an instance grid
+template <bool POOL>
 __device__ __forceinline__ long long cell_index(const InstDev& I, int i, int j, int k) {
-    int blk = ((i >> 2) * I.nb + (j >> 2)) * I.nb + (k >> 2);
-    return I.grid_base + (long long)blk * 64 + ((i & 3) << 4) + ((j & 3) << 2) + (k & 3);
+    const int blk = ((i >> 2) * I.nb + (j >> 2)) * I.nb + (k >> 2);
+    long long c0;
+    if (POOL) {
+        const int t = I.btab[blk];
+        c0 = (long long)(t >= 0 ? t : I.pool_trash) * 64;
+    } else {
+        c0 = I.grid_base + (long long)blk * 64;
+    }
+    return c0 + ((i & 3) << 4) + ((j & 3) << 2) + (k & 3);
 }

@@ -71,7 +71,7 @@ class Engine:
                  precision: str = "mixed", svd_grad: str = "reference",
                  per_element_moduli: bool = False, patch_size: int = 0, scatter_wgs: int = 0,
                  grid_wgs: int = 0, bin_cells: int = 0, grid_snapshots: bool = True,
-                 return_mapping: bool = False, stream: Optional[int] = None):
+                 return_mapping: bool = False, grid_pool_blocks: int = 0, stream: Optional[int] = None):
         self.lib = _abi.load()
         self.instances = list(instances)
         self.B = len(self.instances)
@@ -110,7 +110,7 @@ class Engine:
                            svd_grad=SVD_GRAD[svd_grad], per_element_moduli=int(per_element_moduli),
                            patch_size=patch_size, scatter_wgs=scatter_wgs, grid_wgs=grid_wgs,
                            bin_cells=bin_cells, grid_snapshots=_snap_option(grid_snapshots),
-                           return_mapping=int(return_mapping))
+                           return_mapping=int(return_mapping), grid_pool_blocks=int(grid_pool_blocks))
         ctx = ctypes.c_void_p()
         _abi.check(self.lib.dcp_create(descs, self.B, ctypes.byref(opt), stream, ctypes.byref(ctx)))
         self.ctx = ctx
@@ -195,6 +195,13 @@ class Engine:
         out = ctypes.c_int64()
         _abi.check(self.lib.dcp_device_bytes(self.ctx, ctypes.byref(out)))
         return int(out.value)
+
+    def pool_blocks(self) -> tuple:
+        """(slots handed out since the last reset, capacity) of the grid block
+        pool; (0, 0) for dense grids."""
+        used, cap = ctypes.c_int(), ctypes.c_int()
+        _abi.check(self.lib.dcp_pool_blocks(self.ctx, ctypes.byref(used), ctypes.byref(cap)))
+        return int(used.value), int(cap.value)
 
     def rebase_elements(self, slot: int) -> None:
         """Element rest shapes from the vertex positions in ``slot`` (reset_quadrature_particles)."""

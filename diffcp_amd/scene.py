@@ -268,6 +268,7 @@ class Workload:
     per_element_moduli: bool = False
     n_targets: int = 1024
     gt_params: Optional[tuple] = None     # ground-truth (E, nu) for target generation
+    grid_pool_blocks: int = 0             # > 0: grids share a pool of this many 4^3 blocks
 
     @property
     def particles(self) -> int:
@@ -319,6 +320,11 @@ def config_c4(seed: int = 3) -> Workload:
     return Workload("c4_precision", [inst], 500, 50, ("E", "nu"), seed, gt_params=(8e3, 0.25))
 
 
+# Grid block pool of the large cloth (only active blocks allocated): 4^3
+# blocks, against 64^3 = 262,144 for a dense 256^3 grid.
+C5_POOL_BLOCKS = 4096
+
+
 def config_c5(seed: int = 4, substeps: int = 1000) -> Workload:
     """256x256-vertex cloth on 256^3, per-triangle lognormal moduli, sqrt(S) checkpoints."""
     inst = make_instance(256, 256, substeps, seed=seed, E=5e3, nu=0.3, density=HEAVY_DENSITY, name="c5")
@@ -327,7 +333,7 @@ def config_c5(seed: int = 4, substeps: int = 1000) -> Workload:
     inst.element_moduli = base[None, :] * np.exp(rng.normal(0.0, 0.3, size=(inst.n_elements, 3)))
     k = max(1, int(round(math.sqrt(substeps))))
     return Workload("c5_large", [inst], substeps, k, ("element_moduli",), seed,
-                    per_element_moduli=True, gt_params=(8e3, 0.25))
+                    per_element_moduli=True, gt_params=(8e3, 0.25), grid_pool_blocks=C5_POOL_BLOCKS)
 
 
 def precision_stress_instance(n: int = 24, n_grid: int = 64, substeps: int = 120, seed: int = 3) -> Instance:

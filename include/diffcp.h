@@ -45,6 +45,7 @@ typedef struct dcp_ctx dcp_ctx;
 #define DCP_STATUS_NONFINITE   1u   /* NaN/Inf in a particle velocity or grid value */
 #define DCP_STATUS_OUT_OF_GRID 2u   /* a particle stencil left [0, n_grid) (clamped) */
 #define DCP_STATUS_TILE_SPILL  4u   /* a patch of more than 256 particles used global atomics */
+#define DCP_STATUS_POOL_FULL   8u   /* the grid block pool ran out of slots (results invalid) */
 
 /* One cloth scene.  Replaces the module globals of
  * src/initialization.py:21-193 and the fields of
@@ -90,6 +91,10 @@ typedef struct dcp_options {
     int32_t return_mapping;      /* 0: as the reference, whose element return mapping (simulation.py:408-428)
                                     is computed and discarded; 1: live: F = Q return_map(R) with cf = the
                                     friction parameter, and its adjoint (gradients w.r.t. friction) */
+    int32_t grid_pool_blocks;    /* 0: each instance has a dense G^3 grid; > 0: the grids share a pool of this
+                                    many 4^3 blocks, handed out to the blocks particle stencils reach (a block
+                                    keeps its slot until the next dcp_reset / rollout start); a full pool sets
+                                    DCP_STATUS_POOL_FULL */
 } dcp_options;
 
 /* Build a context: upload B instances, build the internal layout (spatial
@@ -219,6 +224,12 @@ int dcp_binning_keys(dcp_ctx* ctx, int32_t slot, int32_t* keys_out);
  * Replaces nothing in the reference, which never sorts: its base cells are
  * the keys of simulation.py:140,155,177,194. */
 int dcp_patch_order(dcp_ctx* ctx, int32_t slot, int32_t* out);
+
+/* Block pool of the grids (dcp_options.grid_pool_blocks): slots handed out
+ * since the last reset (used) and available (capacity); 0 and 0 for dense
+ * grids.  No counterpart in the reference, whose grids are dense fields
+ * (taichi_variable_definition.py:33-36).  Synchronises the context stream. */
+int dcp_pool_blocks(dcp_ctx* ctx, int32_t* used, int32_t* capacity);
 
 /* Sticky status bits (DCP_STATUS_*); `clear` resets them. */
 int dcp_status(dcp_ctx* ctx, uint32_t* out, int32_t clear);

@@ -14,8 +14,9 @@
     fold): positions at substeps 250 and 500 and the gradient against the
     oracle (fixture c4_positions.npz), both precision modes;
   * the large-cloth setting on a reduced cloth (256^3 grid, per-triangle
-    moduli, sqrt(S) = 10-substep checkpoints): per-triangle moduli gradients
-    and a directional derivative against the oracle (fixture c5_reduced.npz);
+    moduli, sqrt(S) = 10-substep checkpoints), with a dense grid and with a
+    pool of active blocks: per-triangle moduli gradients and a directional
+    derivative against the oracle (fixture c5_reduced.npz);
   * a reference-shaped scene: a 15^3 grid (inv_dx not a power of two),
     scene_scaling 0.5 and 300-substep frames: bit-exact binning keys,
     positions after 320 and 620 substeps (past the frame boundaries; 2e-5:
@@ -203,15 +204,19 @@ def test_precision_stress_positions_and_gradient(precision):
         assert abs(grads[0, j] - fd[j]) < 5e-3 * abs(fd[j]), (j, grads[0, j], fd[j])
 
 
-def test_large_cloth_setting_per_triangle_moduli():
+@pytest.mark.parametrize("pool", [0, 2048])
+def test_large_cloth_setting_per_triangle_moduli(pool):
     ref = np.load(os.path.join(GOLDEN, "c5_reduced.npz"))
     inst = scene.large_cloth_reduced_instance()
     S, K = int(ref["substeps"]), int(ref["checkpoint_every"])
-    eng = _engine([inst], max_states=_slots(S, K), per_element_moduli=True)
+    eng = _engine([inst], max_states=_slots(S, K), per_element_moduli=True, grid_pool_blocks=pool)
     eng.set_targets([ref["targets"]])
     eng.reset(0)
     loss, _, _ = eng.rollout_grad(S, 0, K, use_graph=True)
     assert eng.status() & BAD_STATUS == 0
+    used, cap = eng.pool_blocks()
+    print(f"pool {pool}: {used} of {cap} blocks handed out")
+    assert cap == pool and (0 < used <= cap if pool else used == 0)
     ge = eng.element_moduli_grads()
     assert abs(loss[0] - float(ref["loss"])) < 1e-4 * float(ref["loss"])
     # per-triangle gradients as loss sensitivities theta g, within 2e-3 of the
