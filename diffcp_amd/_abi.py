@@ -43,6 +43,7 @@ class Options(ctypes.Structure):
         ("per_element_moduli", c_int), ("patch_size", c_int), ("scatter_wgs", c_int),
         ("grid_wgs", c_int), ("bin_cells", c_int),
         ("grid_snapshots", c_int), ("return_mapping", c_int), ("grid_pool_blocks", c_int),
+        ("deterministic", c_int),
     ]
 
 

@@ -277,6 +277,8 @@ struct dcp_ctx {
     int* btab = nullptr;
     int pool_cap = 0;
     bool pooled() const { return btab != nullptr; }
+    // template form of the scatter kernels: DCP_GF_POOL | DCP_GF_DET bits
+    int grid_form() const { return (pooled() ? DCP_GF_POOL : 0) | (sim.det ? DCP_GF_DET : 0); }
     // hand the pool out afresh (a new rollout or reset state)
     int reset_pool() {
         if (!pooled()) return 0;
@@ -322,25 +324,30 @@ static hipError_t time_mark(dcp_ctx* c) {
 // scatters list such patches, and k_p2g_big / k_g2p_adj_big, launched after
 // each, scatter them through the LDS tile over sub-boxes that fit it (with an
 // empty list they exit at once).
-template <bool POOL> static int launch_p2g_form(dcp_ctx* c, StateView S, int s, int g, const int* skip);
+template <int GF> static int launch_p2g_form(dcp_ctx* c, StateView S, int s, int g, const int* skip);
 static int launch_p2g(dcp_ctx* c, StateView S, int s, int g, const int* skip) {
     if (c->pooled()) {
         // pool slots for the blocks this substep's stencils reach
         hipLaunchKernelGGL(k_pool_alloc, dim3(c->grid_patches()), dim3(DCP_WG), 0, c->stream, c->sim, S, skip);
         LAUNCH_OK();
     }
-    return c->pooled() ? launch_p2g_form<true>(c, S, s, g, skip) : launch_p2g_form<false>(c, S, s, g, skip);
+    switch (c->grid_form()) {
+        case 0: return launch_p2g_form<0>(c, S, s, g, skip);
+        case 1: return launch_p2g_form<1>(c, S, s, g, skip);
+        case 2: return launch_p2g_form<2>(c, S, s, g, skip);
+        default: return launch_p2g_form<3>(c, S, s, g, skip);
+    }
 }
-template <bool POOL>
+template <int GF>
 static int launch_p2g_form(dcp_ctx* c, StateView S, int s, int g, const int* skip) {
     if (skip) {
-        hipLaunchKernelGGL(k_p2g<POOL>, dim3(c->grid_spatches()), dim3(DCP_WG), 0, c->stream, c->sim, S, s, g, skip);
+        hipLaunchKernelGGL(k_p2g<GF>, dim3(c->grid_spatches()), dim3(DCP_WG), 0, c->stream, c->sim, S, s, g, skip);
         LAUNCH_OK();
     } else {
-        TIMED_LAUNCH(DCP_KERNEL_P2G, k_p2g<POOL>, dim3(c->grid_spatches()), dim3(DCP_WG), 0, c->stream, c->sim, S, s,
-                     g, skip);
+        TIMED_LAUNCH(DCP_KERNEL_P2G, k_p2g<GF>, dim3(c->grid_spatches()), dim3(DCP_WG), 0, c->stream, c->sim, S, s, g,
+                     skip);
     }
-    hipLaunchKernelGGL(k_p2g_big<POOL>, dim3(c->grid_big()), dim3(DCP_WG), 0, c->stream, c->sim, S, s, g, skip);
+    hipLaunchKernelGGL(k_p2g_big<GF>, dim3(c->grid_big()), dim3(DCP_WG), 0, c->stream, c->sim, S, s, g, skip);
     LAUNCH_OK();
     hipLaunchKernelGGL(k_active_list, dim3(c->grid_gridop()), dim3(DCP_WG), 0, c->stream, c->sim, skip);
     LAUNCH_OK();
@@ -427,21 +434,27 @@ static int launch_adjoint_grid(dcp_ctx* c, int s, int g, StateView S, int snap) 
     return 0;
 }
 
+template <int GF>
+static int launch_g2p_adj_form(dcp_ctx* c, StateView S, StateView G1, StateView Gout, int pending) {
+    TIMED_LAUNCH(DCP_KERNEL_G2P_ADJ, k_g2p_adj<GF>, dim3(c->grid_spatches()), dim3(DCP_WG), 0, c->stream, c->sim, S,
+                 G1, Gout, pending);
+    hipLaunchKernelGGL(k_g2p_adj_big<GF>, dim3(c->grid_big()), dim3(DCP_WG), 0, c->stream, c->sim, S, G1, Gout,
+                       pending);
+    LAUNCH_OK();
+    return 0;
+}
+
 // G2P adjoint and grid-update adjoint (they need the substep's grid).
 static int launch_adjoint_head(dcp_ctx* c, StateView S, StateView G1, StateView Gout, int snap, int pending) {
     PrepSlot ps(c, prep_saved(c, snap) ? snap : -1);
-    if (c->pooled()) {
-        TIMED_LAUNCH(DCP_KERNEL_G2P_ADJ, k_g2p_adj<true>, dim3(c->grid_spatches()), dim3(DCP_WG), 0, c->stream, c->sim,
-                     S, G1, Gout, pending);
-        hipLaunchKernelGGL(k_g2p_adj_big<true>, dim3(c->grid_big()), dim3(DCP_WG), 0, c->stream, c->sim, S, G1, Gout,
-                           pending);
-    } else {
-        TIMED_LAUNCH(DCP_KERNEL_G2P_ADJ, k_g2p_adj<false>, dim3(c->grid_spatches()), dim3(DCP_WG), 0, c->stream, c->sim,
-                     S, G1, Gout, pending);
-        hipLaunchKernelGGL(k_g2p_adj_big<false>, dim3(c->grid_big()), dim3(DCP_WG), 0, c->stream, c->sim, S, G1, Gout,
-                           pending);
+    int r;
+    switch (c->grid_form()) {
+        case 0: r = launch_g2p_adj_form<0>(c, S, G1, Gout, pending); break;
+        case 1: r = launch_g2p_adj_form<1>(c, S, G1, Gout, pending); break;
+        case 2: r = launch_g2p_adj_form<2>(c, S, G1, Gout, pending); break;
+        default: r = launch_g2p_adj_form<3>(c, S, G1, Gout, pending); break;
     }
-    LAUNCH_OK();
+    if (r) return -1;
     TIMED_LAUNCH(DCP_KERNEL_GRIDOP_ADJ, k_gridop_adj, dim3(c->grid_gridop()), dim3(DCP_WG), 0, c->stream, c->sim);
     return 0;
 }
@@ -778,14 +791,24 @@ static int build(dcp_ctx* c, const dcp_instance_desc* d, int B) {
     }
     if (c->alloc(&c->prep_work, DCP_PREP_FLOATS * (size_t)ne)) return -1;
     c->use_prep(-1);
+    // deterministic: scatter sums in two-word fixed point (8 words per cell),
+    // so that a substep's grid does not depend on the order of the atomic
+    // adds; otherwise float atomics into acc / gacc
+    S.det = c->opt.deterministic != 0;
+    if (S.det ? (c->alloc(&S.accq, 8 * (size_t)cells) || c->alloc(&S.gaccq, 8 * (size_t)cells))
+              : c->alloc(&S.gacc, (size_t)cells))
+        return -1;
+    if (c->alloc(&S.poison, 2 * (size_t)B) || c->alloc(&S.damp_fx, 2 * (size_t)B) ||
+        c->alloc(&S.traj_fx, 6 * (size_t)std::max(1, traj)) || c->alloc(&S.traj_grad, 3 * (size_t)std::max(1, traj)))
+        return -1;
     if (c->alloc(&S.acc, (size_t)cells) ||
-        c->alloc(&S.vout, (size_t)cells) || c->alloc(&S.gacc, (size_t)cells) || c->alloc(&S.gmv, (size_t)cells) ||
+        c->alloc(&S.vout, (size_t)cells) || c->alloc(&S.gmv, (size_t)cells) ||
         c->alloc(&S.flags, (size_t)flags) || c->alloc(&S.active, (size_t)flags) || c->alloc(&S.n_active, (size_t)B) ||
         c->alloc(&S.status, 1) || c->alloc(&S.g_xe, 3 * (size_t)ne) ||
         c->alloc(&S.g_fv, 3 * (size_t)np) ||
         c->alloc(&S.ec_gx, 9 * (size_t)ne) || c->alloc(&S.ec_gv, 3 * (size_t)ne) ||
         c->alloc(&S.pgrad_patch, (size_t)DCP_PATCH_GRADS * (DCP_WG / 64) * c->n_patches) || c->alloc(&S.pgrad_inst, (size_t)DCP_N_PARAMS * B) ||
-        c->alloc(&S.emod_grad, 3 * (size_t)ne) || c->alloc(&S.traj_grad, 3 * (size_t)std::max(1, traj)))
+        c->alloc(&S.emod_grad, 3 * (size_t)ne))
         return -1;
     if (c->alloc(&c->states, c->slot_floats * c->max_states) || c->alloc(&c->adj, c->slot_floats * c->n_adj) ||
         c->alloc(&c->scratch, c->slot_floats) || c->alloc(&c->loss_d, B) || c->alloc(&c->seed_d, B) ||
@@ -1106,10 +1129,18 @@ extern "C" int dcp_zero_param_grads(dcp_ctx* c) {
                           c->stream));
     HIP_OK(hipMemsetAsync(c->sim.pgrad_inst, 0, DCP_N_PARAMS * sizeof(double) * c->B, c->stream));
     HIP_OK(hipMemsetAsync(c->sim.emod_grad, 0, 3 * sizeof(double) * c->NE, c->stream));
+    HIP_OK(hipMemsetAsync(c->sim.damp_fx, 0, 2 * sizeof(unsigned long long) * c->B, c->stream));
     size_t nt = 0;
     for (auto& I : c->inst_h) nt += (size_t)I.n_actions * I.n_grasp;
+    HIP_OK(hipMemsetAsync(c->sim.traj_fx, 0, 6 * sizeof(unsigned long long) * std::max<size_t>(1, nt), c->stream));
     HIP_OK(hipMemsetAsync(c->sim.traj_grad, 0, 3 * sizeof(double) * std::max<size_t>(1, nt), c->stream));
     return 0;
+}
+
+// value of a two-word fixed-point sum (hi 2^-24 + lo 2^-62), as the kernels
+// add them
+static double fx_value(unsigned long long hi, unsigned long long lo) {
+    return std::ldexp((double)(long long)hi, -24) + std::ldexp((double)(long long)lo, -62);
 }
 
 extern "C" int dcp_loss(dcp_ctx* c, int32_t slot, double* out) {
@@ -1138,9 +1169,12 @@ static int reduce_param_grads(dcp_ctx* c, double* out_caller) {
     std::vector<double> pp((size_t)DCP_PATCH_GRADS * nw * c->n_patches), pi((size_t)DCP_N_PARAMS * c->B);
     HIP_OK(hipMemcpyAsync(pp.data(), c->sim.pgrad_patch, pp.size() * 8, hipMemcpyDeviceToHost, c->stream));
     HIP_OK(hipMemcpyAsync(pi.data(), c->sim.pgrad_inst, pi.size() * 8, hipMemcpyDeviceToHost, c->stream));
+    std::vector<unsigned long long> dfx(2 * (size_t)c->B);
+    HIP_OK(hipMemcpyAsync(dfx.data(), c->sim.damp_fx, dfx.size() * 8, hipMemcpyDeviceToHost, c->stream));
     HIP_OK(hipStreamSynchronize(c->stream));
     for (int b = 0; b < c->B; ++b)
         for (int k = 0; k < DCP_N_PARAMS; ++k) out[b * DCP_N_PARAMS + k] = pi[b * DCP_N_PARAMS + k];
+    for (int b = 0; b < c->B; ++b) out[b * DCP_N_PARAMS + 4] += fx_value(dfx[2 * b], dfx[2 * b + 1]);
     for (int p = 0; p < c->n_patches; ++p) {
         const Patch& P = c->patches_h[p];
         if (P.cls != CLS_ELEMENT) continue;
@@ -1194,8 +1228,14 @@ extern "C" int dcp_trajectory_grads(dcp_ctx* c, double* out) {
     if (!c || !out) return set_err("bad args");
     size_t nt = 0;
     for (auto& I : c->inst_h) nt += (size_t)I.n_actions * I.n_grasp;
-    HIP_OK(hipMemcpyAsync(out, c->sim.traj_grad, 3 * nt * 8, hipMemcpyDefault, c->stream));
+    std::vector<unsigned long long> q(6 * std::max<size_t>(1, nt));
+    std::vector<double> g(3 * std::max<size_t>(1, nt));
+    HIP_OK(hipMemcpyAsync(q.data(), c->sim.traj_fx, q.size() * 8, hipMemcpyDeviceToHost, c->stream));
+    HIP_OK(hipMemcpyAsync(g.data(), c->sim.traj_grad, g.size() * 8, hipMemcpyDeviceToHost, c->stream));
     HIP_OK(hipStreamSynchronize(c->stream));
+    g.resize(3 * nt);   // one of the two forms holds zeros
+    for (size_t i = 0; i < 3 * nt; ++i) g[i] += fx_value(q[2 * i], q[2 * i + 1]);
+    HIP_OK(hipMemcpy(out, g.data(), g.size() * 8, hipMemcpyDefault));
     return 0;
 }
 

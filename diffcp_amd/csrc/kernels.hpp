@@ -20,6 +20,31 @@ __device__ __forceinline__ int wave_max(int v) {
     for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
     return v;
 }
+// add channels c0 .. c0+n-1 of a tile node into the fixed-point scatter sums
+// q of a grid cell (words 2c, 2c+1 of channel c); a term out of range marks
+// the instance's poison flag instead
+template <int N>
+__device__ __forceinline__ void fx_add_node(unsigned long long* q, long long cell, const float* v, int* poison) {
+    bool ok = true;
+#pragma unroll
+    for (int c = 0; c < N; ++c)
+        ok &= fx_atomic_add(fx_cell_word(q, cell, 2 * c), fx_cell_word(q, cell, 2 * c + 1), (double)v[c]);
+    if (!ok) atomicOr(poison, 1);
+}
+// the (m, m v) or d/d grid_v_out channels of a cell from its fixed-point
+// sums; NaN when the instance's poison flag is set; the words are cleared
+template <int N>
+__device__ __forceinline__ void fx_take_cell(unsigned long long* q, long long cell, bool poisoned, float* v) {
+#pragma unroll
+    for (int c = 0; c < N; ++c) {
+        unsigned long long* w0 = fx_cell_word(q, cell, 2 * c);
+        unsigned long long* w1 = fx_cell_word(q, cell, 2 * c + 1);
+        v[c] = poisoned ? __int_as_float(0x7fc00000) : (float)fx_join(*w0, *w1);
+        *w0 = 0ull;
+        *w1 = 0ull;
+    }
+}
+
 __device__ __forceinline__ double wave_sum_d(double v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
@@ -331,8 +356,8 @@ struct StateIO {
 };
 __global__ void k_state_export(const float* slot, int NP, int NE, StateIO io);
 __global__ void k_state_import(float* slot, int NP, int NE, StateIO io);
-template <bool POOL> __global__ void k_p2g(SimDev sim, StateView S, int s, int g, const int* skip);
-template <bool POOL> __global__ void k_p2g_big(SimDev sim, StateView S, int s, int g, const int* skip);
+template <int GF> __global__ void k_p2g(SimDev sim, StateView S, int s, int g, const int* skip);
+template <int GF> __global__ void k_p2g_big(SimDev sim, StateView S, int s, int g, const int* skip);
 __global__ void k_active_list(SimDev sim, const int* skip);
 __global__ void k_pool_alloc(SimDev sim, StateView S, const int* skip);
 __global__ void k_pool_reset(int* btab, int n, int* pool_n);
@@ -347,8 +372,8 @@ __global__ void k_binning_keys(SimDev sim, StateView S, int* keys);
 __global__ void k_patch_order(SimDev sim, StateView S, int* out);
 __global__ void k_rebase(SimDev sim, StateView S, float* f_old, float* Dinv, float* Dinvinv);
 
-template <bool POOL> __global__ void k_g2p_adj(SimDev sim, StateView S, StateView G1, StateView Gout, int pending);
-template <bool POOL> __global__ void k_g2p_adj_big(SimDev sim, StateView S, StateView G1, StateView Gout, int pending);
+template <int GF> __global__ void k_g2p_adj(SimDev sim, StateView S, StateView G1, StateView Gout, int pending);
+template <int GF> __global__ void k_g2p_adj_big(SimDev sim, StateView S, StateView G1, StateView Gout, int pending);
 __global__ void k_gridop_adj(SimDev sim);
 template <bool POOL> __global__ void k_p2g_adj(SimDev sim, StateView S, StateView Gout, int s, int g);
 template <typename PT, bool RM, bool POOL>

@@ -121,9 +121,10 @@ __device__ __forceinline__ void g2p_adj_store(const SimDev& sim, const InstDev& 
     else store_v3(Gout.px(0), sim.NP, P.start + q, gx1 + gxt);
 }
 
-template <bool POOL>
+template <int GF>
 __device__ __forceinline__ void g2p_adj_spill(const SimDev& sim, const StateView& S, const StateView& G1,
                                            const StateView& Gout, const Patch& P, const InstDev& I, int pending) {
+    [[maybe_unused]] constexpr bool POOL = (GF & DCP_GF_POOL) != 0, DET = (GF & DCP_GF_DET) != 0;
     for (int q = threadIdx.x; q < P.count; q += DCP_WG) {
         Stencil st;
         make_stencil(particle_x(sim, S, P, q), I.inv_dx, I.G, st, nullptr);
@@ -139,7 +140,12 @@ __device__ __forceinline__ void g2p_adj_spill(const SimDev& sim, const StateView
                 [&](int j, int k) { return sim.vout[cell_index<POOL>(I, st.base[0] + i, st.base[1] + j, st.base[2] + k)]; },
                 [&](int j, int k, float g0, float g1, float g2) {
                     long long gi = cell_index<POOL>(I, st.base[0] + i, st.base[1] + j, st.base[2] + k);
-                    atomicAdd(&sim.gacc[gi].y, g0); atomicAdd(&sim.gacc[gi].z, g1); atomicAdd(&sim.gacc[gi].w, g2);
+                    if (DET) {
+                        const float v[3] = {g0, g1, g2};
+                        fx_add_node<3>(sim.gaccq, gi, v, sim.poison + sim.B + P.inst);
+                    } else {
+                        atomicAdd(&sim.gacc[gi].y, g0); atomicAdd(&sim.gacc[gi].z, g1); atomicAdd(&sim.gacc[gi].w, g2);
+                    }
                 });
         float gfx[3];
         g2p_adj_finish(sep, gfx);
@@ -157,11 +163,12 @@ __device__ __forceinline__ void g2p_adj_spill(const SimDev& sim, const StateView
 // G2P reverse, through one LDS tile over box B, of the patch particles whose
 // base cells lie in B (`act` lanes; B is the patch's box, or one sub-box of it
 // in k_g2p_adj_big): node terms, dL/dx stores, and the flush of dL/d grid_v_out.
-template <bool POOL>
+template <int GF>
 __device__ __forceinline__ void g2p_adj_tile(const SimDev& sim, const InstDev& I, const StateView& Gout,
                                              const PatchBox& B, bool act, const V3<float>& x, const V3<float>& gv,
                                              const V3<float>& gx1, const M3<float>& gC, const Patch& Pl, int ql,
                                              int b0, int b1, int b2, BinLDS& L, float2* cA, float* cB, float4* vt) {
+    [[maybe_unused]] constexpr bool POOL = (GF & DCP_GF_POOL) != 0, DET = (GF & DCP_GF_DET) != 0;
     const int tid = threadIdx.x;
     const int pos = patch_bin(act ? tile_cell(B, b0, b1, b2) : -1, B, L);
     // this lane's tile nodes: tid + h * DCP_WG (vol <= DCP_NODES * DCP_WG), and their planes
@@ -216,8 +223,13 @@ __device__ __forceinline__ void g2p_adj_tile(const SimDev& sim, const InstDev& I
         if (has[h] && (a[h].y != 0.f || a[h].z != 0.f || a[h].w != 0.f)) {
             const int c = tid + h * DCP_WG, t = c / B.e2;
             long long gidx = cell_index<POOL>(I, nci[h] + B.l0, t - nci[h] * B.e1 + B.l1, c - t * B.e2 + B.l2);
-            atomicAdd(&sim.gacc[gidx].y, a[h].y); atomicAdd(&sim.gacc[gidx].z, a[h].z);
-            atomicAdd(&sim.gacc[gidx].w, a[h].w);
+            if (DET) {
+                const float v[3] = {a[h].y, a[h].z, a[h].w};
+                fx_add_node<3>(sim.gaccq, gidx, v, sim.poison + sim.B + Pl.inst);
+            } else {
+                atomicAdd(&sim.gacc[gidx].y, a[h].y); atomicAdd(&sim.gacc[gidx].z, a[h].z);
+                atomicAdd(&sim.gacc[gidx].w, a[h].w);
+            }
         }
     }
 }
@@ -239,8 +251,13 @@ __device__ __forceinline__ void g2p_adj_patch_inputs(const SimDev& sim, const In
 // tile reverse is written out here rather than called as g2p_adj_tile
 // (k_g2p_adj_big's): the shared form compiled to a schedule measured 3 %
 // slower on this kernel.
-template <bool POOL>
+template <int GF>
 __global__ __launch_bounds__(DCP_WG, 3) void k_g2p_adj(SimDev sim, StateView S, StateView G1, StateView Gout, int pending) {
+    [[maybe_unused]] constexpr bool POOL = (GF & DCP_GF_POOL) != 0, DET = (GF & DCP_GF_DET) != 0;
+    // a recomputed grid update, just before on the stream, has read the
+    // scatter's poison flags
+    if (blockIdx.x == 0)
+        for (int b = threadIdx.x; b < sim.B; b += DCP_WG) sim.poison[b] = 0;
     __shared__ BinLDS L;
     __shared__ float2 cA[DCP_CONTRIB_ZERO + 1];   // contributions, channels 0 and 1
     __shared__ float cB[DCP_CONTRIB_ZERO + 1];    // channel 2
@@ -326,8 +343,13 @@ __global__ __launch_bounds__(DCP_WG, 3) void k_g2p_adj(SimDev sim, StateView S, 
                     if (has[h] && (a[h].y != 0.f || a[h].z != 0.f || a[h].w != 0.f)) {
                         const int c = tid + h * DCP_WG, t = c / B.e2;
                         long long gidx = cell_index<POOL>(I, nci[h] + B.l0, t - nci[h] * B.e1 + B.l1, c - t * B.e2 + B.l2);
-                        atomicAdd(&sim.gacc[gidx].y, a[h].y); atomicAdd(&sim.gacc[gidx].z, a[h].z);
-                        atomicAdd(&sim.gacc[gidx].w, a[h].w);
+                        if (DET) {
+                            const float v[3] = {a[h].y, a[h].z, a[h].w};
+                            fx_add_node<3>(sim.gaccq, gidx, v, sim.poison + sim.B + SP.inst);
+                        } else {
+                            atomicAdd(&sim.gacc[gidx].y, a[h].y); atomicAdd(&sim.gacc[gidx].z, a[h].z);
+                            atomicAdd(&sim.gacc[gidx].w, a[h].w);
+                        }
                     }
                 }
             }
@@ -343,9 +365,10 @@ __global__ __launch_bounds__(DCP_WG, 3) void k_g2p_adj(SimDev sim, StateView S, 
 // The patches k_g2p_adj left: the same tile reverse over the sub-boxes of the
 // patch box (as k_p2g_big).  k_gridop_adj, next on the stream, empties the
 // list.
-template <bool POOL>
+template <int GF>
 __global__ __launch_bounds__(DCP_WG, 2) void k_g2p_adj_big(SimDev sim, StateView S, StateView G1, StateView Gout,
                                                            int pending) {
+    [[maybe_unused]] constexpr bool POOL = (GF & DCP_GF_POOL) != 0, DET = (GF & DCP_GF_DET) != 0;
     const int n_big = sim.big_n[DCP_BIG_G2P_ADJ];
     if ((int)blockIdx.x >= n_big) return;   // the usual case: an empty list
     __shared__ BinLDS L;
@@ -385,12 +408,12 @@ __global__ __launch_bounds__(DCP_WG, 2) void k_g2p_adj_big(SimDev sim, StateView
                             __syncthreads();
                         }
                         first = false;
-                        g2p_adj_tile<POOL>(sim, I, Gout, B, in, x, gv, gx1, gC, Pl, ql, b0, b1, b2, L, cA, cB, vt);
+                        g2p_adj_tile<GF>(sim, I, Gout, B, in, x, gv, gx1, gC, Pl, ql, b0, b1, b2, L, cA, cB, vt);
                     }
         }
         if (!tiled) {
-            g2p_adj_spill<POOL>(sim, S, G1, Gout, SP.part(0), I, pending);
-            g2p_adj_spill<POOL>(sim, S, G1, Gout, SP.part(1), I, pending);
+            g2p_adj_spill<GF>(sim, S, G1, Gout, SP.part(0), I, pending);
+            g2p_adj_spill<GF>(sim, S, G1, Gout, SP.part(1), I, pending);
         }
         bin_clear(L);
         if (tid < 3) { L.box[tid] = 1 << 30; L.box[3 + tid] = -(1 << 30); }
@@ -408,14 +431,25 @@ __global__ __launch_bounds__(DCP_WG) void k_gridop_adj(SimDev sim) {
     ActiveWalk w;
     if (!active_walk(sim, sim.inst, w)) return;
     const InstDev I = sim.inst[w.b];
-    double gdamp_wave = 0.0;
+    const bool poisoned = sim.poison[sim.B + w.b] != 0;
+    long long dhi = 0, dlo = 0;   // deterministic sums: this wave's dL/d damping, fixed point
+    bool dok = true;
+    double gdamp_wave = 0.0;      // float-atomic sums: this lane's part of it
     for (int it = w.first; it < w.n; it += w.stride) {
         const int blk = sim.active[I.flag_base + it];
         const long long cell = block_cell0(I, blk) + lane;
         int ci, cj, ck;
         block_cell_coords(I, blk, lane, ci, cj, ck);
         float4 c = sim.acc[cell];
-        float4 ga = sim.gacc[cell];
+        float4 ga;
+        if (sim.det) {
+            float gv[3];
+            fx_take_cell<3>(sim.gaccq, cell, poisoned, gv);
+            ga = make_float4(0.f, gv[0], gv[1], gv[2]);
+        } else {
+            ga = sim.gacc[cell];
+            sim.gacc[cell] = make_float4(0.f, 0.f, 0.f, 0.f);
+        }
         float v[3], mask[3], vr[3];
         grid_velocity(I, c, ci, cj, ck, v, mask, vr);
         float gd0 = mask[0] * ga.y, gd1 = mask[1] * ga.z, gd2 = mask[2] * ga.w;
@@ -426,12 +460,27 @@ __global__ __launch_bounds__(DCP_WG) void k_gridop_adj(SimDev sim) {
         float gm = -inv_m * (g0 * vr[0] + g1 * vr[1] + g2 * vr[2]);
         sim.gmv[cell] = make_float4(gm, inv_m * g0, inv_m * g1, inv_m * g2);
         sim.acc[cell] = make_float4(0.f, 0.f, 0.f, 0.f);
-        sim.gacc[cell] = make_float4(0.f, 0.f, 0.f, 0.f);
-        gdamp_wave += gdamp;
+        if (sim.det) {
+            // the block's sum in lane order, then fixed point: the same bits
+            // whichever wave takes the block and in whatever order
+            const double sb = wave_sum_d(gdamp);
+            long long h, l;
+            if (fx_split(sb, h, l)) { dhi += h; dlo += l; } else { dok = false; }
+        } else {
+            gdamp_wave += gdamp;
+        }
         if (lane == 0) sim.flags[I.flag_base + blk] = 0;
     }
-    double s = wave_sum_d(gdamp_wave);
-    if (lane == 0 && s != 0.0) atomicAdd(&sim.pgrad_inst[w.b * DCP_N_PARAMS + 4], s);
+    if (sim.det) {
+        if (lane == 0) {
+            if (dhi) atomicAdd(sim.damp_fx + 2 * w.b, (unsigned long long)dhi);
+            if (dlo) atomicAdd(sim.damp_fx + 2 * w.b + 1, (unsigned long long)dlo);
+            if (!dok) atomicOr(sim.status, (unsigned)DCP_STATUS_NONFINITE);
+        }
+    } else {
+        const double s = wave_sum_d(gdamp_wave);
+        if (lane == 0 && s != 0.0) atomicAdd(&sim.pgrad_inst[w.b * DCP_N_PARAMS + 4], s);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -593,14 +642,22 @@ __device__ __forceinline__ void grasper_traj_grad(const SimDev& sim, const InstD
     const int frame = grasper_frame(I, s, g);
     if (!sim.traj_grad || frame + 1 >= I.n_actions) return;
     const int gid = (P.start - I.o_start) / I.group_size;
-    double* t1 = sim.traj_grad + 3 * ((size_t)I.traj_off + (size_t)(frame + 1) * I.n_grasp + gid);
-    double* t0 = sim.traj_grad + 3 * ((size_t)I.traj_off + (size_t)frame * I.n_grasp + gid);
+    const size_t f1 = (size_t)I.traj_off + (size_t)(frame + 1) * I.n_grasp + gid;
+    const size_t f0 = (size_t)I.traj_off + (size_t)frame * I.n_grasp + gid;
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
         const double gvp = wave_sum_d(act ? (double)r.m * (double)o.gb[a] / (double)I.interval : 0.0);
         if ((threadIdx.x & 63) == 0 && gvp != 0.0) {
-            atomicAdd(t1 + a, gvp);
-            atomicAdd(t0 + a, -gvp);
+            if (sim.det) {
+                // fixed point: the wave sums of a frame add up to the same bits in any order
+                unsigned long long* t1 = sim.traj_fx + 6 * f1 + 2 * a;
+                unsigned long long* t0 = sim.traj_fx + 6 * f0 + 2 * a;
+                const bool ok = fx_atomic_add(t1, t1 + 1, gvp) && fx_atomic_add(t0, t0 + 1, -gvp);
+                if (!ok) atomicOr(sim.status, (unsigned)DCP_STATUS_NONFINITE);
+            } else {
+                atomicAdd(sim.traj_grad + 3 * f1 + a, gvp);
+                atomicAdd(sim.traj_grad + 3 * f0 + a, -gvp);
+            }
         }
     }
 }
@@ -644,6 +701,10 @@ __device__ __forceinline__ bool p2g_adj_stage(const SimDev& sim, const InstDev& 
 // k_elem_adj, which needs the vertex force-impulse gradients stored here.
 template <bool POOL>
 __global__ __launch_bounds__(DCP_WG, 2) void k_p2g_adj(SimDev sim, StateView S, StateView Gout, int s, int g) {
+    // the grid-update adjoint, just before on the stream, has read the
+    // adjoint scatter's poison flags
+    if (blockIdx.x == 0)
+        for (int b = threadIdx.x; b < sim.B; b += DCP_WG) sim.poison[sim.B + b] = 0;
     __shared__ int box[6];
     __shared__ float4 gt[DCP_STILE_CAP];
     const int tid = threadIdx.x;
@@ -911,9 +972,10 @@ DCP_ELEM_ADJ(double, true, false) DCP_ELEM_ADJ(float, true, false)
 DCP_ELEM_ADJ(double, false, true) DCP_ELEM_ADJ(float, false, true)
 DCP_ELEM_ADJ(double, true, true) DCP_ELEM_ADJ(float, true, true)
 #undef DCP_ELEM_ADJ
-template __global__ void k_g2p_adj<false>(SimDev, StateView, StateView, StateView, int);
-template __global__ void k_g2p_adj<true>(SimDev, StateView, StateView, StateView, int);
-template __global__ void k_g2p_adj_big<false>(SimDev, StateView, StateView, StateView, int);
-template __global__ void k_g2p_adj_big<true>(SimDev, StateView, StateView, StateView, int);
+#define DCP_G2P_ADJ_FORMS(GF)                                                           \
+    template __global__ void k_g2p_adj<GF>(SimDev, StateView, StateView, StateView, int); \
+    template __global__ void k_g2p_adj_big<GF>(SimDev, StateView, StateView, StateView, int);
+DCP_G2P_ADJ_FORMS(0) DCP_G2P_ADJ_FORMS(1) DCP_G2P_ADJ_FORMS(2) DCP_G2P_ADJ_FORMS(3)
+#undef DCP_G2P_ADJ_FORMS
 template __global__ void k_p2g_adj<false>(SimDev, StateView, StateView, int, int);
 template __global__ void k_p2g_adj<true>(SimDev, StateView, StateView, int, int);

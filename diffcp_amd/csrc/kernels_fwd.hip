@@ -88,9 +88,10 @@ __device__ __forceinline__ TileBlocks tile_blocks(int l0, int l1, int l2, int e0
 
 // Particles of a patch that does not fit the sorted tile scatter straight into
 // the grid with float atomics (slow path, flagged in the status word).
-template <bool POOL>
+template <int GF>
 __device__ __forceinline__ void p2g_spill(const SimDev& sim, const StateView& S, const Patch& P, const InstDev& I, int s,
                                        int g) {
+    [[maybe_unused]] constexpr bool POOL = (GF & DCP_GF_POOL) != 0, DET = (GF & DCP_GF_DET) != 0;
     for (int q = threadIdx.x; q < P.count; q += DCP_WG) {
         P2GRec r;
         p2g_record(sim, I, S, P, q, s, g, r);
@@ -102,8 +103,13 @@ __device__ __forceinline__ void p2g_spill(const SimDev& sim, const StateView& S,
         for (int i = 0; i < 3; ++i)
             p2g_layer(sq, i, [&](int j, int k, float mm, float m0, float m1, float m2) {
                 long long gi = cell_index<POOL>(I, st.base[0] + i, st.base[1] + j, st.base[2] + k);
-                atomicAdd(&sim.acc[gi].x, mm); atomicAdd(&sim.acc[gi].y, m0);
-                atomicAdd(&sim.acc[gi].z, m1); atomicAdd(&sim.acc[gi].w, m2);
+                if (DET) {
+                    const float v[4] = {mm, m0, m1, m2};
+                    fx_add_node<4>(sim.accq, gi, v, sim.poison + P.inst);
+                } else {
+                    atomicAdd(&sim.acc[gi].x, mm); atomicAdd(&sim.acc[gi].y, m0);
+                    atomicAdd(&sim.acc[gi].z, m1); atomicAdd(&sim.acc[gi].w, m2);
+                }
             });
         // the 4^3 blocks of the 27 nodes: one or two per axis
         for (int bx = st.base[0] >> 2; bx <= (st.base[0] + 2) >> 2; ++bx)
@@ -114,14 +120,20 @@ __device__ __forceinline__ void p2g_spill(const SimDev& sim, const StateView& S,
 }
 
 // Add one tile node's (m, m v) into the instance grid and note its 4^3 block.
-template <bool POOL>
+template <int GF>
 __device__ __forceinline__ void p2g_flush_node(const SimDev& sim, const InstDev& I, const Patch& P, const PatchBox& B,
                                                const TileBlocks& TB, int* tb_touched, int ci, int cj, int ck, float4 a) {
+    [[maybe_unused]] constexpr bool POOL = (GF & DCP_GF_POOL) != 0, DET = (GF & DCP_GF_DET) != 0;
     if (a.x == 0.f && a.y == 0.f && a.z == 0.f && a.w == 0.f) return;
     ci += B.l0; cj += B.l1; ck += B.l2;
     long long gi = cell_index<POOL>(I, ci, cj, ck);
-    atomicAdd(&sim.acc[gi].x, a.x); atomicAdd(&sim.acc[gi].y, a.y);
-    atomicAdd(&sim.acc[gi].z, a.z); atomicAdd(&sim.acc[gi].w, a.w);
+    if (DET) {
+        const float v[4] = {a.x, a.y, a.z, a.w};
+        fx_add_node<4>(sim.accq, gi, v, sim.poison + P.inst);
+    } else {
+        atomicAdd(&sim.acc[gi].x, a.x); atomicAdd(&sim.acc[gi].y, a.y);
+        atomicAdd(&sim.acc[gi].z, a.z); atomicAdd(&sim.acc[gi].w, a.w);
+    }
     if (TB.n <= DCP_TB_CAP)
         tb_touched[((ci >> 2) - TB.bx0) * TB.nby * TB.nbz + ((cj >> 2) - TB.by0) * TB.nbz + ((ck >> 2) - TB.bz0)] = 1;
     else
@@ -167,10 +179,11 @@ __device__ __forceinline__ void p2g_patch_record(const SimDev& sim, const InstDe
 // Scatter, through one LDS tile over box B, of the patch particles whose base
 // cells lie in B (`act` lanes; B is the patch's box, or one sub-box of it in
 // k_p2g_big).
-template <bool POOL>
+template <int GF>
 __device__ __forceinline__ void p2g_tile(const SimDev& sim, const InstDev& I, const Patch& P, const PatchBox& B,
                                          bool act, const P2GRec& r, int b0, int b1, int b2, BinLDS& L,
                                          float4* contrib, int* tb_touched) {
+    [[maybe_unused]] constexpr bool POOL = (GF & DCP_GF_POOL) != 0, DET = (GF & DCP_GF_DET) != 0;
     const int tid = threadIdx.x;
     const TileBlocks TB = tile_blocks(B.l0, B.l1, B.l2, B.e0, B.e1, B.e2);
     for (int t = tid; t < TB.n && t < DCP_TB_CAP; t += DCP_WG) tb_touched[t] = 0;
@@ -209,7 +222,7 @@ __device__ __forceinline__ void p2g_tile(const SimDev& sim, const InstDev& I, co
     for (int h = 0; h < DCP_NODES; ++h)
         if (has[h]) {
             const int t = (tid + h * DCP_WG) / B.e2;
-            p2g_flush_node<POOL>(sim, I, P, B, TB, tb_touched, nci[h], t - nci[h] * B.e1,
+            p2g_flush_node<GF>(sim, I, P, B, TB, tb_touched, nci[h], t - nci[h] * B.e1,
                            tid + h * DCP_WG - t * B.e2, a[h]);
         }
     __syncthreads();
@@ -219,8 +232,9 @@ __device__ __forceinline__ void p2g_tile(const SimDev& sim, const InstDev& I, co
 // A patch whose tile overflows is listed for k_p2g_big.  The tile scatter is
 // written out here rather than called as p2g_tile (k_p2g_big's): the shared
 // form compiled to a schedule measured 2 % slower on this kernel.
-template <bool POOL>
-__global__ __launch_bounds__(DCP_WG, 2) void k_p2g(SimDev sim, StateView S, int s, int g, const int* skip) {
+template <int GF>
+__global__ __launch_bounds__(DCP_WG, 3) void k_p2g(SimDev sim, StateView S, int s, int g, const int* skip) {
+    [[maybe_unused]] constexpr bool POOL = (GF & DCP_GF_POOL) != 0, DET = (GF & DCP_GF_DET) != 0;
     if (skip && *skip) return;   // adjoint substep whose grid is restored from a snapshot
     __shared__ BinLDS L;
     __shared__ float4 contrib[DCP_CONTRIB_ZERO + 1];
@@ -289,7 +303,7 @@ __global__ __launch_bounds__(DCP_WG, 2) void k_p2g(SimDev sim, StateView S, int 
                 for (int h = 0; h < DCP_NODES; ++h)
                     if (has[h]) {
                         const int t = (tid + h * DCP_WG) / B.e2;
-                        p2g_flush_node<POOL>(sim, I, P, B, TB, tb_touched, nci[h], t - nci[h] * B.e1,
+                        p2g_flush_node<GF>(sim, I, P, B, TB, tb_touched, nci[h], t - nci[h] * B.e1,
                                        tid + h * DCP_WG - t * B.e2, a[h]);
                     }
                 __syncthreads();
@@ -310,8 +324,9 @@ __global__ __launch_bounds__(DCP_WG, 2) void k_p2g(SimDev sim, StateView S, int 
 // of more than DCP_WG particles (none by construction) would scatter per
 // particle with global atomics (status bit DCP_STATUS_TILE_SPILL).  k_gridop,
 // after the list scan, empties the list.
-template <bool POOL>
+template <int GF>
 __global__ __launch_bounds__(DCP_WG, 2) void k_p2g_big(SimDev sim, StateView S, int s, int g, const int* skip) {
+    [[maybe_unused]] constexpr bool POOL = (GF & DCP_GF_POOL) != 0, DET = (GF & DCP_GF_DET) != 0;
     if (skip && *skip) return;
     const int n_big = sim.big_n[DCP_BIG_P2G];
     if ((int)blockIdx.x >= n_big) return;   // the usual case: an empty list
@@ -349,13 +364,13 @@ __global__ __launch_bounds__(DCP_WG, 2) void k_p2g_big(SimDev sim, StateView S, 
                             __syncthreads();
                         }
                         first = false;
-                        p2g_tile<POOL>(sim, I, P, B, in, r, b0, b1, b2, L, contrib, tb_touched);
+                        p2g_tile<GF>(sim, I, P, B, in, r, b0, b1, b2, L, contrib, tb_touched);
                     }
         }
         if (!tiled) {
             if (tid == 0) atomicOr(sim.status, (unsigned)DCP_STATUS_TILE_SPILL);
-            p2g_spill<POOL>(sim, S, SP.part(0), I, s, g);
-            p2g_spill<POOL>(sim, S, SP.part(1), I, s, g);
+            p2g_spill<GF>(sim, S, SP.part(0), I, s, g);
+            p2g_spill<GF>(sim, S, SP.part(1), I, s, g);
         }
         bin_clear(L);
         if (tid < 3) { L.box[tid] = 1 << 30; L.box[3 + tid] = -(1 << 30); }
@@ -491,12 +506,20 @@ __global__ __launch_bounds__(DCP_WG) void k_gridop(SimDev sim, int keep, GridSna
         if (blockIdx.x == 0 && threadIdx.x == 0) *save.ok = total <= save.cap;
     }
     const bool saving = save.acc && total <= save.cap;
+    const bool poisoned = sim.poison[w.b] != 0;
     for (int it = w.first; it < w.n; it += w.stride) {
         const int blk = sim.active[I.flag_base + it];
         const long long cell = block_cell0(I, blk) + lane;
         int ci, cj, ck;
         block_cell_coords(I, blk, lane, ci, cj, ck);
-        float4 c = sim.acc[cell];
+        float4 c;
+        if (sim.det) {
+            float cv[4];
+            fx_take_cell<4>(sim.accq, cell, poisoned, cv);
+            c = make_float4(cv[0], cv[1], cv[2], cv[3]);
+        } else {
+            c = sim.acc[cell];
+        }
         if (saving) {
             save.acc[(size_t)(base + it) * 64 + lane] = c;
             if (lane == 0) save.blk[base + it] = blk;
@@ -505,8 +528,10 @@ __global__ __launch_bounds__(DCP_WG) void k_gridop(SimDev sim, int keep, GridSna
         grid_velocity(I, c, ci, cj, ck, v, nullptr);
         sim.vout[cell] = make_float4(v[0], v[1], v[2], 0.f);
         if (!(isfinite(v[0]) && isfinite(v[1]) && isfinite(v[2]))) atomicOr(sim.status, (unsigned)DCP_STATUS_NONFINITE);
-        if (!keep) {
-            sim.acc[cell] = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (keep) {
+            if (sim.det) sim.acc[cell] = c;    // for the grid-update adjoint, which clears it
+        } else {
+            if (!sim.det) sim.acc[cell] = make_float4(0.f, 0.f, 0.f, 0.f);
             if (lane == 0) sim.flags[I.flag_base + blk] = 0;
         }
     }
@@ -556,6 +581,9 @@ __global__ __launch_bounds__(DCP_WG) void k_grid_restore(SimDev sim, GridSnap sn
 template <bool POOL>
 __global__ __launch_bounds__(DCP_WG) void k_g2p(SimDev sim, StateView S, StateView Sn) {
     const int tid = threadIdx.x;
+    // the grid update, just before on the stream, has read the poison flags
+    if (blockIdx.x == 0)
+        for (int b = tid; b < sim.B; b += DCP_WG) sim.poison[b] = 0;
     for (int pi = blockIdx.x; pi < sim.n_patches; pi += gridDim.x) {
         const Patch P = sim.patches[pi];
         const InstDev I = sim.inst[P.inst];
@@ -814,10 +842,11 @@ __global__ void k_binning_keys(SimDev sim, StateView S, int* keys) {
     }
 }
 
-template __global__ void k_p2g<false>(SimDev, StateView, int, int, const int*);
-template __global__ void k_p2g<true>(SimDev, StateView, int, int, const int*);
-template __global__ void k_p2g_big<false>(SimDev, StateView, int, int, const int*);
-template __global__ void k_p2g_big<true>(SimDev, StateView, int, int, const int*);
+#define DCP_P2G_FORMS(GF)                                                          \
+    template __global__ void k_p2g<GF>(SimDev, StateView, int, int, const int*); \
+    template __global__ void k_p2g_big<GF>(SimDev, StateView, int, int, const int*);
+DCP_P2G_FORMS(0) DCP_P2G_FORMS(1) DCP_P2G_FORMS(2) DCP_P2G_FORMS(3)
+#undef DCP_P2G_FORMS
 template __global__ void k_g2p<false>(SimDev, StateView, StateView);
 template __global__ void k_g2p<true>(SimDev, StateView, StateView);
 template __global__ void k_prep<double, false>(SimDev, StateView, StateView, int);

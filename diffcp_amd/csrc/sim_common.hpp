@@ -22,6 +22,10 @@
 
 enum : int { CLS_VERTEX = 0, CLS_GRASPER = 1, CLS_CUBE = 2, CLS_ELEMENT = 3 };
 
+// template forms of the scatter kernels (bit flags)
+#define DCP_GF_POOL 1              // grid from the block pool
+#define DCP_GF_DET 2               // fixed-point scatter sums
+
 struct InstDev {
     int G, nb;                     // grid side, blocks per side (ceil(G/4))
     long long grid_base;           // first cell of this instance's dense region (0 with a block pool)
@@ -96,10 +100,14 @@ struct SimDev {
     float* xe; float* ve; float* ke; float* fc;          // [3][NE] [3][NE] [3][NE] [9][NE]
     float* nd;                                            // [3][NE] n of the current d
     // grid
-    float4* acc;                   // (m, mv) accumulators
+    int det;                       // 1: scatter sums in fixed point (bitwise reproducible), 0: float atomics
+    unsigned long long* accq;      // det: (m, mv) scatter sums, two-word fixed point, per block [8 words][64 cells]
+    float4* acc;                   // (m, mv): float-atomic scatter sums, or the grid update's (kept for the adjoint)
     float4* vout;                  // grid_v_out
-    float4* gacc;                  // d/d grid_v_out accumulators
+    unsigned long long* gaccq;     // det: d/d grid_v_out scatter sums, as accq (words 0..5)
+    float4* gacc;                  // not det: d/d grid_v_out float-atomic scatter sums (y, z, w)
     float4* gmv;                   // (d/dm, d/dmv)
+    int* poison;                   // [2][B]: a non-finite or out-of-range term reached the (adjoint) scatter sums
     int* flags;                    // active block flags (per instance at flag_base)
     int* active;                   // active block ids, per instance at flag_base
     int* n_active;                 // [B] active blocks per instance
@@ -111,9 +119,11 @@ struct SimDev {
     float* g_fv;                   // [3][NP] d/d (vertex force impulse)
     float* ec_gx; float* ec_gv;    // [9][NE] per-corner d/dx; [3][NE] d/dv (one for the three corners)
     double* pgrad_patch;           // [n_patches][waves][DCP_PATCH_GRADS]  E, nu, shear, contact, friction (element patches)
-    double* pgrad_inst;            // [B][DCP_N_PARAMS] (damping, friction via atomics)
+    double* pgrad_inst;            // [B][DCP_N_PARAMS]
+    unsigned long long* damp_fx;   // [B][2] dL/d damping, two-word fixed point
     double* emod_grad;             // [3][NE]
-    double* traj_grad;             // [frames*grasp][3]
+    double* traj_grad;             // [frames*grasp][3] dL/d trajectory (float-atomic form)
+    unsigned long long* traj_fx;   // [frames*grasp][3][2] the same, two-word fixed point (det)
 };
 
 // ---------------------------------------------------------------------------
@@ -262,6 +272,34 @@ __device__ __forceinline__ void make_stencil(V3<float> x, float inv_dx, int G, S
         s.dw[2][d] = f - 0.5f;
     }
     if (out && status) atomicOr(status, 2u);
+}
+
+// Two-word fixed point for sums whose value must not depend on the order of
+// their terms (atomic scatters, reductions over waves): x = hi 2^-24 +
+// lo 2^-62 with hi, lo 64-bit integers, which add exactly in any order.  A
+// term must satisfy |x| < 2^38; below 2^-62 it rounds to a multiple of 2^-62.
+__device__ __forceinline__ bool fx_split(double d, long long& hi, long long& lo) {
+    if (!(fabs(d) < 0x1p38)) { hi = lo = 0; return false; }
+    const double h = rint(d * 0x1p24);
+    hi = (long long)h;
+    lo = (long long)rint((d - h * 0x1p-24) * 0x1p62);
+    return true;
+}
+__device__ __forceinline__ double fx_join(unsigned long long hi, unsigned long long lo) {
+    return (double)(long long)hi * 0x1p-24 + (double)(long long)lo * 0x1p-62;
+}
+// atomically add x to the fixed-point pair (w0: hi, w1: lo); false (nothing
+// added) when x is not finite or out of range
+__device__ __forceinline__ bool fx_atomic_add(unsigned long long* w0, unsigned long long* w1, double x) {
+    long long hi, lo;
+    if (!fx_split(x, hi, lo)) return false;
+    if (hi) atomicAdd(w0, (unsigned long long)hi);
+    if (lo) atomicAdd(w1, (unsigned long long)lo);
+    return true;
+}
+// words of cell `cell` in a per-block [8 words][64 cells] fixed-point array
+__device__ __forceinline__ unsigned long long* fx_cell_word(unsigned long long* q, long long cell, int word) {
+    return q + ((cell >> 6) << 9) + (word << 6) + (cell & 63);
 }
 
 // First cell of 4^3 block blk of an instance grid: its place in the dense

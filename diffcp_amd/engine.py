@@ -71,7 +71,8 @@ class Engine:
                  precision: str = "mixed", svd_grad: str = "reference",
                  per_element_moduli: bool = False, patch_size: int = 0, scatter_wgs: int = 0,
                  grid_wgs: int = 0, bin_cells: int = 0, grid_snapshots: bool = True,
-                 return_mapping: bool = False, grid_pool_blocks: int = 0, stream: Optional[int] = None):
+                 return_mapping: bool = False, grid_pool_blocks: int = 0, deterministic: bool = False,
+                 stream: Optional[int] = None):
         self.lib = _abi.load()
         self.instances = list(instances)
         self.B = len(self.instances)
@@ -110,7 +111,8 @@ class Engine:
                            svd_grad=SVD_GRAD[svd_grad], per_element_moduli=int(per_element_moduli),
                            patch_size=patch_size, scatter_wgs=scatter_wgs, grid_wgs=grid_wgs,
                            bin_cells=bin_cells, grid_snapshots=_snap_option(grid_snapshots),
-                           return_mapping=int(return_mapping), grid_pool_blocks=int(grid_pool_blocks))
+                           return_mapping=int(return_mapping), grid_pool_blocks=int(grid_pool_blocks),
+                           deterministic=int(deterministic))
         ctx = ctypes.c_void_p()
         _abi.check(self.lib.dcp_create(descs, self.B, ctypes.byref(opt), stream, ctypes.byref(ctx)))
         self.ctx = ctx

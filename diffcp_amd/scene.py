@@ -380,3 +380,23 @@ def tiny_instance(n: int = 4, n_grid: int = 16, substeps: int = 8, seed: int = 0
     kw.setdefault("cube_side", 0.1)
     kw.setdefault("steps_per_frame", 2)
     return make_instance(n, n_grid, substeps, seed=seed, **kw)
+
+
+def wide_grasp_instance(n_grid, spread_xz, y_lo, y_hi):
+    """Tiny scene whose two grasp points each have two particles moved to far
+    corners (x, z -+spread_xz, y y_lo / y_hi): each grasp point's patch tile
+    exceeds the 1024-cell LDS tile, while its other six particles keep holding
+    the cloth corner."""
+    inst = tiny_instance(n=5, n_grid=n_grid, substeps=40)
+    ng = inst.trajectory.shape[1]
+    gs = len(inst.grasper_x) // ng
+    gx = inst.grasper_x.copy()
+    for g in range(ng):
+        c = gx[g * gs:(g + 1) * gs].mean(0)
+        gx[g * gs] = c + np.array([-spread_xz, y_lo, -spread_xz])
+        gx[g * gs + 1] = c + np.array([spread_xz, y_hi, spread_xz])
+    gx[:, [0, 2]] = np.clip(gx[:, [0, 2]], 0.15, 0.85)
+    inst.grasper_x = gx
+    rng = np.random.default_rng(7)
+    inst.targets = inst.vertex_x[rng.choice(inst.n_vertices, 12, replace=False)] + rng.normal(0, 0.01, (12, 3))
+    return inst

@@ -95,6 +95,9 @@ typedef struct dcp_options {
                                     many 4^3 blocks, handed out to the blocks particle stencils reach (a block
                                     keeps its slot until the next dcp_reset / rollout start); a full pool sets
                                     DCP_STATUS_POOL_FULL */
+    int32_t deterministic;       /* 0: the scatters add into the grid with float atomics, whose order varies from
+                                    run to run; 1: they add two-word fixed-point sums (and the damping and
+                                    trajectory gradients too), so that a window gives the same bits every run */
 } dcp_options;
 
 /* Build a context: upload B instances, build the internal layout (spatial

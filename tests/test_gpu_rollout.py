@@ -178,28 +178,6 @@ def test_trajectory_gradient_matches_oracle():
     assert np.abs(gt - ref).max() < 2e-3 * np.abs(ref).max(), (np.abs(gt - ref).max(), np.abs(ref).max())
 
 
-def _wide_grasp_instance(n_grid, spread_xz, y_lo, y_hi):
-    """Tiny scene whose two grasp points each have two particles moved to far
-    corners (x, z -+spread_xz, y y_lo / y_hi): each grasp point's patch tile
-    exceeds the 1024-cell LDS tile, while its other six particles keep holding
-    the cloth corner."""
-    inst = scene.tiny_instance(n=5, n_grid=n_grid, substeps=40)
-    ng = inst.trajectory.shape[1]
-    gs = len(inst.grasper_x) // ng
-    gx = inst.grasper_x.copy()
-    for g in range(ng):
-        c = gx[g * gs:(g + 1) * gs].mean(0)
-        gx[g * gs] = c + np.array([-spread_xz, y_lo, -spread_xz])
-        gx[g * gs + 1] = c + np.array([spread_xz, y_hi, spread_xz])
-    gx[:, [0, 2]] = np.clip(gx[:, [0, 2]], 0.15, 0.85)
-    inst.grasper_x = gx
-    base = np.trunc(gx[:gs] * inst.n_grid - 0.5)
-    assert np.prod(base.max(0) - base.min(0) + 3) > 1024
-    rng = np.random.default_rng(7)
-    inst.targets = inst.vertex_x[rng.choice(inst.n_vertices, 12, replace=False)] + rng.normal(0, 0.01, (12, 3))
-    return inst
-
-
 # 16^3: tiles of 13 x 9 x 13 cells, cut in x only; 48^3: 33 x 22 x 33 cells,
 # one x-plane alone above the cap, cut in x and y
 @pytest.mark.parametrize("n_grid,spread_xz,y_lo,y_hi", [(16, 0.35, -0.1, 0.25), (48, 0.4, -0.15, 0.25)])
@@ -210,7 +188,10 @@ def test_overflowing_patch_tiles_through_sub_boxes(n_grid, spread_xz, y_lo, y_hi
     match the oracle's FP64 autograd (loss 1e-4 relative; E and nu each 2e-3
     relative, the trajectory 2e-3 of its largest component) and each other (5e-5: float-atomic order and the
     first-argmin flips of a tiny scene)."""
-    inst = _wide_grasp_instance(n_grid, spread_xz, y_lo, y_hi)
+    inst = scene.wide_grasp_instance(n_grid, spread_xz, y_lo, y_hi)
+    gs = len(inst.grasper_x) // inst.trajectory.shape[1]
+    base = np.trunc(inst.grasper_x[:gs] * inst.n_grid - 0.5)
+    assert np.prod(base.max(0) - base.min(0) + 3) > 1024
     S = 30
     eng = _engine([inst], max_states=_slots(S, S), svd_grad="stable")
     eng.reset(0)
