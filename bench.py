@@ -24,6 +24,8 @@ Also reported on the same JSON line:
   * ``large``: the large single cloth (256x256 vertices, 256^3 grid, 1000
     substeps, sqrt(S) checkpoints, per-triangle moduli gradients): window
     time, throughput, G2P-adjoint HBM rate and the device memory held;
+  * ``deterministic``: the batched window again with fixed-point scatter
+    sums (bitwise reproducible windows), 2 timed steps;
   * ``cpu_baseline``: the FP32 C++/OpenMP restatement in ``oracle/`` timed
     on the host on a bounded sample of the same workload.
 
@@ -105,13 +107,13 @@ def load_traffic():
     return d.get("bytes_per_dispatch")
 
 
-def bench_batched(steps: int, warmup: int, use_graph: bool = True):
+def bench_batched(steps: int, warmup: int, use_graph: bool = True, deterministic: bool = False, w=None):
     from diffcp_amd import scene
     from diffcp_amd.engine import Engine
 
-    w = scene.config_c3()
+    w = w or scene.config_c3()
     S, K = w.substeps, w.checkpoint_every
-    eng = Engine(w.instances, max_states=Engine.slots_for(S, K))
+    eng = Engine(w.instances, max_states=Engine.slots_for(S, K), deterministic=deterministic)
     set_ground_truth_targets(eng, w, S - 2)
     # active cells at the initial state and at the loss state (mean)
     slot, _ = eng.rollout(S - 2)
@@ -302,6 +304,7 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-large", action="store_true", help="skip the large single-cloth leg")
+    ap.add_argument("--no-det", action="store_true", help="skip the deterministic-sums batched leg")
     ap.add_argument("--eager", action="store_true",
                     help="launch kernels one by one instead of replaying graphs (for counter profiling)")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -354,6 +357,13 @@ def main() -> int:
         "roofline": roofline,
         "status": b["status"], "finite": b["finite"],
     }
+    if not args.no_det:
+        log("batched workload with deterministic (fixed-point) scatter sums")
+        d = bench_batched(2, 1, use_graph=not args.eager, deterministic=True, w=w)
+        out["deterministic"] = {"value": w.particles * S * 2 / d["wall"], "ms_per_step": d["wall"] / 2 * 1e3,
+                                "g2p_adj_avg_launch_us": d["k_ms"] / max(d["k_n"], 1) * 1e3,
+                                "status": d["status"], "finite": d["finite"],
+                                "note": "dcp_options.deterministic = 1: bitwise reproducible windows"}
     if not args.no_plain:
         log("plain workload (single instance, per-kernel launches)")
         p = bench_plain()
