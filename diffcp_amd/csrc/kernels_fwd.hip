@@ -233,7 +233,7 @@ __device__ __forceinline__ void p2g_tile(const SimDev& sim, const InstDev& I, co
 // written out here rather than called as p2g_tile (k_p2g_big's): the shared
 // form compiled to a schedule measured 2 % slower on this kernel.
 template <int GF>
-__global__ __launch_bounds__(DCP_WG, 3) void k_p2g(SimDev sim, StateView S, int s, int g, const int* skip) {
+__global__ __launch_bounds__(DCP_WG, (GF & DCP_GF_DET) ? 3 : 2) void k_p2g(SimDev sim, StateView S, int s, int g, const int* skip) {
     [[maybe_unused]] constexpr bool POOL = (GF & DCP_GF_POOL) != 0, DET = (GF & DCP_GF_DET) != 0;
     if (skip && *skip) return;   // adjoint substep whose grid is restored from a snapshot
     __shared__ BinLDS L;
