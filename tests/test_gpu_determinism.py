@@ -14,6 +14,8 @@ the same window gives the same bits:
     are equal bit for bit;
   * a scene whose patch tiles overflow (the sub-box scatter kernels) and a
     pooled grid give equal bits across two windows;
+  * pooled and dense grids give the same bits (the same integers reach the
+    same nodes), and so does the live return mapping across two windows;
   * the eager launch sequence and the graph replay of the same window agree
     to the last bit as well (the same kernels on the same inputs);
   * the default float-atomic form gives the same window within its
@@ -99,4 +101,39 @@ def test_overflowing_tiles_same_bits(pool):
     a = _window(eng, S)
     _same(a, _window(eng, S), "overflow tiles")
     assert eng.status() & 7 == 0
+    eng.close()
+
+
+def test_pooled_and_dense_grids_give_the_same_bits():
+    """With fixed-point sums the pooled grid (four instances sharing one
+    block pool) and the dense grids add the same integers into their nodes:
+    the window gives the same bits either way."""
+    from diffcp_amd.engine import Engine
+    S = 30
+    insts = _batch(S)
+    outs = []
+    for pool in (0, 2048):
+        eng = Engine(insts, max_states=Engine.slots_for(S, S), grid_pool_blocks=pool, deterministic=True)
+        eng.set_targets([i.targets for i in insts])
+        outs.append(_window(eng, S))
+        assert eng.status() == 0
+        if pool:
+            used, cap = eng.pool_blocks()
+            assert 0 < used <= cap
+        eng.close()
+    _same(outs[0], outs[1], "pooled vs dense")
+
+
+def test_return_mapping_same_bits():
+    """The live return mapping (friction gradients) in the deterministic form."""
+    from diffcp_amd.engine import Engine
+    S = 30
+    insts = _batch(S)[:2]
+    for inst in insts:
+        inst.friction = 0.4
+    eng = Engine(insts, max_states=Engine.slots_for(S, S), return_mapping=True, deterministic=True)
+    eng.set_targets([i.targets for i in insts])
+    a = _window(eng, S)
+    assert np.abs(a[1][:, 5]).max() > 0          # dL/dfriction is live
+    _same(a, _window(eng, S), "return mapping")
     eng.close()
