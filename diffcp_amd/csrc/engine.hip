@@ -272,7 +272,7 @@ struct dcp_ctx {
     int grid_spatches() const {
         return std::max(1, std::min(n_spatches, opt.scatter_wgs > 0 ? opt.scatter_wgs : n_cu * 16));
     }
-    int grid_big() const { return n_cu; }
+    int grid_big() const { return 2 * n_cu; }
     // block pool of the grids (grid_pool_blocks > 0)
     int* btab = nullptr;
     int pool_cap = 0;

@@ -369,8 +369,8 @@ template <int GF>
 __global__ __launch_bounds__(DCP_WG, 2) void k_g2p_adj_big(SimDev sim, StateView S, StateView G1, StateView Gout,
                                                            int pending) {
     [[maybe_unused]] constexpr bool POOL = (GF & DCP_GF_POOL) != 0, DET = (GF & DCP_GF_DET) != 0;
-    const int n_big = sim.big_n[DCP_BIG_G2P_ADJ];
-    if ((int)blockIdx.x >= n_big) return;   // the usual case: an empty list
+    const int n_items = sim.big_n[DCP_BIG_G2P_ADJ] * DCP_BIG_SPLIT;
+    if ((int)blockIdx.x >= n_items) return;   // the usual case: an empty list
     __shared__ BinLDS L;
     __shared__ float2 cA[DCP_CONTRIB_ZERO + 1];
     __shared__ float cB[DCP_CONTRIB_ZERO + 1];
@@ -380,7 +380,8 @@ __global__ __launch_bounds__(DCP_WG, 2) void k_g2p_adj_big(SimDev sim, StateView
     if (tid < 3) { L.box[tid] = 1 << 30; L.box[3 + tid] = -(1 << 30); }
     if (tid == 0) { cA[DCP_CONTRIB_ZERO] = make_float2(0.f, 0.f); cB[DCP_CONTRIB_ZERO] = 0.f; }
     __syncthreads();
-    for (int bi = blockIdx.x; bi < n_big; bi += gridDim.x) {
+    for (int it = blockIdx.x; it < n_items; it += gridDim.x) {
+        const int bi = it / DCP_BIG_SPLIT, part = it - bi * DCP_BIG_SPLIT;
         const int pi = sim.big_list[sim.n_spatches + bi];
         const SPatch SP = sim.spatches[pi];
         const InstDev I = sim.inst[SP.inst];
@@ -397,9 +398,11 @@ __global__ __launch_bounds__(DCP_WG, 2) void k_g2p_adj_big(SimDev sim, StateView
             box_reduce(b0, b1, b2, act, L.box, PB);
             const SubPlan SPL = sub_plan(PB);
             bool first = true;
+            int k = 0;
             for (int x0 = 0; x0 < SPL.n[0]; x0 += SPL.w[0])
                 for (int y0 = 0; y0 < SPL.n[1]; y0 += SPL.w[1])
                     for (int z0 = 0; z0 < SPL.n[2]; z0 += SPL.w[2]) {
+                        if (k++ % DCP_BIG_SPLIT != part) continue;   // another item's sub-box
                         const PatchBox B = sub_box(PB, SPL, x0, y0, z0);
                         const bool in = act && in_sub_box(B, b0, b1, b2);
                         if (!__syncthreads_or(in)) continue;   // no particle in this sub-box
@@ -411,7 +414,7 @@ __global__ __launch_bounds__(DCP_WG, 2) void k_g2p_adj_big(SimDev sim, StateView
                         g2p_adj_tile<GF>(sim, I, Gout, B, in, x, gv, gx1, gC, Pl, ql, b0, b1, b2, L, cA, cB, vt);
                     }
         }
-        if (!tiled) {
+        if (!tiled && part == 0) {
             g2p_adj_spill<GF>(sim, S, G1, Gout, SP.part(0), I, pending);
             g2p_adj_spill<GF>(sim, S, G1, Gout, SP.part(1), I, pending);
         }

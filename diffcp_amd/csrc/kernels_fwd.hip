@@ -328,8 +328,8 @@ template <int GF>
 __global__ __launch_bounds__(DCP_WG, 2) void k_p2g_big(SimDev sim, StateView S, int s, int g, const int* skip) {
     [[maybe_unused]] constexpr bool POOL = (GF & DCP_GF_POOL) != 0, DET = (GF & DCP_GF_DET) != 0;
     if (skip && *skip) return;
-    const int n_big = sim.big_n[DCP_BIG_P2G];
-    if ((int)blockIdx.x >= n_big) return;   // the usual case: an empty list
+    const int n_items = sim.big_n[DCP_BIG_P2G] * DCP_BIG_SPLIT;
+    if ((int)blockIdx.x >= n_items) return;   // the usual case: an empty list
     __shared__ BinLDS L;
     __shared__ float4 contrib[DCP_CONTRIB_ZERO + 1];
     __shared__ int tb_touched[DCP_TB_CAP];
@@ -338,7 +338,8 @@ __global__ __launch_bounds__(DCP_WG, 2) void k_p2g_big(SimDev sim, StateView S, 
     if (tid < 3) { L.box[tid] = 1 << 30; L.box[3 + tid] = -(1 << 30); }
     if (tid == 0) contrib[DCP_CONTRIB_ZERO] = make_float4(0.f, 0.f, 0.f, 0.f);
     __syncthreads();
-    for (int bi = blockIdx.x; bi < n_big; bi += gridDim.x) {
+    for (int it = blockIdx.x; it < n_items; it += gridDim.x) {
+        const int bi = it / DCP_BIG_SPLIT, part = it - bi * DCP_BIG_SPLIT;
         const int pi = sim.big_list[bi];
         const SPatch SP = sim.spatches[pi];
         const Patch P = SP.part(0);
@@ -353,9 +354,11 @@ __global__ __launch_bounds__(DCP_WG, 2) void k_p2g_big(SimDev sim, StateView S, 
             box_reduce(b0, b1, b2, act, L.box, PB);
             const SubPlan SPL = sub_plan(PB);
             bool first = true;
+            int k = 0;
             for (int x0 = 0; x0 < SPL.n[0]; x0 += SPL.w[0])
                 for (int y0 = 0; y0 < SPL.n[1]; y0 += SPL.w[1])
                     for (int z0 = 0; z0 < SPL.n[2]; z0 += SPL.w[2]) {
+                        if (k++ % DCP_BIG_SPLIT != part) continue;   // another item's sub-box
                         const PatchBox B = sub_box(PB, SPL, x0, y0, z0);
                         const bool in = act && in_sub_box(B, b0, b1, b2);
                         if (!__syncthreads_or(in)) continue;   // no particle in this sub-box
@@ -367,7 +370,7 @@ __global__ __launch_bounds__(DCP_WG, 2) void k_p2g_big(SimDev sim, StateView S, 
                         p2g_tile<GF>(sim, I, P, B, in, r, b0, b1, b2, L, contrib, tb_touched);
                     }
         }
-        if (!tiled) {
+        if (!tiled && part == 0) {
             if (tid == 0) atomicOr(sim.status, (unsigned)DCP_STATUS_TILE_SPILL);
             p2g_spill<GF>(sim, S, SP.part(0), I, s, g);
             p2g_spill<GF>(sim, S, SP.part(1), I, s, g);

@@ -317,6 +317,12 @@ __device__ __forceinline__ bool in_sub_box(const PatchBox& S, int b0, int b1, in
            b2 < S.l2 + S.e2 - 2;
 }
 
+// The sub-boxes of a listed patch are shared out over DCP_BIG_SPLIT work
+// items (sub-box k goes to item k mod DCP_BIG_SPLIT), so that the few patches
+// of a folded cloth that overflow do not each run all their sub-boxes in one
+// workgroup; every item of a patch loads its records and box again.
+#define DCP_BIG_SPLIT 8
+
 // Lists of the patches whose tile exceeds the cap, filled by k_p2g /
 // k_g2p_adj, worked off by k_p2g_big / k_g2p_adj_big and emptied by the grid
 // update that follows each (k_gridop / k_gridop_adj): lengths in big_n[0..1],
