@@ -8,6 +8,8 @@ Tolerances are relative to each field's scale (max |oracle|):
     undeformed cloth the E/nu terms carry J-1 and Ft-R at the ~1e-5 strain
     level, below FP32 position resolution, and agree to 5e-2;
   * integer binning keys: bit-exact.
+The tiny forward and single-substep adjoint run in all four forms of the
+scatter kernels (dense or pooled grid, float or fixed-point scatter sums).
 """
 import numpy as np
 import pytest
@@ -44,11 +46,18 @@ def _gpu_rollout(inst, N, **kw):
     return eng
 
 
+# the four template forms of the scatter kernels: dense or pooled grid, float
+# or fixed-point scatter sums
+GRID_FORMS = [{}, {"deterministic": True}, {"grid_pool_blocks": 256}, {"grid_pool_blocks": 256, "deterministic": True}]
+FORM_IDS = ["dense", "dense-det", "pool", "pool-det"]
+
+
+@pytest.mark.parametrize("form", GRID_FORMS, ids=FORM_IDS)
 @pytest.mark.parametrize("precision", ["mixed", "fp32"])
-def test_forward_tiny_matches_oracle(precision):
+def test_forward_tiny_matches_oracle(precision, form):
     inst = scene.tiny_instance(n=5, n_grid=16, substeps=10)
     N = 8
-    eng = _gpu_rollout(inst, N, precision=precision)
+    eng = _gpu_rollout(inst, N, precision=precision, **form)
     assert eng.status() & BAD_STATUS == 0
     states, _ = O.rollout(inst, n_substeps=N, keep_states=True)
     for s in (1, N):
@@ -75,12 +84,13 @@ def test_forward_c1_full_rollout_positions():
     assert eng.status() & BAD_STATUS == 0
 
 
+@pytest.mark.parametrize("form", GRID_FORMS, ids=FORM_IDS)
 @pytest.mark.parametrize("svd_grad", ["stable", "reference"])
-def test_adjoint_substep_tiny(svd_grad):
-    inst = scene.tiny_instance(n=5, n_grid=16, substeps=10)
+def test_adjoint_substep_tiny(svd_grad, form):
+    inst = scene.tiny_instance(n=5, n_grid=16, substeps=10, damping=1.5)
     K = O.Consts.from_instance(inst)
     N = 6
-    eng = _gpu_rollout(inst, N, svd_grad=svd_grad)
+    eng = _gpu_rollout(inst, N, svd_grad=svd_grad, **form)
     _, static = O.initialize_state(inst)
     rng = np.random.default_rng(11)
     for f in (0, 4):
