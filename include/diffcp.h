@@ -75,6 +75,9 @@ typedef struct dcp_instance_desc {
     const double*  trajectory;   /* [n_actions][n_grasp_points][3]                 */
 } dcp_instance_desc;
 
+/* Context options.  Zero-initialise the whole struct (memset, `= {0}`) and
+ * set the fields you need: every field's 0 is its default, including fields
+ * a later version appends. */
 typedef struct dcp_options {
     int32_t max_states;          /* state slots kept on device (>= 2)              */
     int32_t precision;           /* 0: FP32 bulk + FP64 2x2 polar/SVD-grad; 1: all FP32 */
