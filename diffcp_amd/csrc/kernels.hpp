@@ -141,8 +141,15 @@ struct P2GRec {
     bool elem;
 };
 
-// Sums, in CSR order, of the per-corner vectors a[corner][.][element] (and of
-// the per-element vectors b[.][element], when TWO) over the (element, corner)
+// Per-element corner vectors are stored element-major, [element][corner][3]
+// (and per-element vectors [element][3]): a vertex's gather reads one 12-byte
+// run per incident (element, corner) entry, whose CSR code ec = 3 element +
+// corner is also its vector's index.
+__device__ __forceinline__ V3<float> load3(const float* p) { return {p[0], p[1], p[2]}; }
+__device__ __forceinline__ void store3(float* p, V3<float> v) { p[0] = v.x; p[1] = v.y; p[2] = v.z; }
+
+// Sums, in CSR order, of the per-corner vectors a[element][corner][.] (and of
+// the per-element vectors b[element][.], when TWO) over the (element, corner)
 // entries incident to vertex p.  Entries go eight at a time with their list and
 // vector loads issued together: one entry after the other would chain two load
 // latencies per entry.
@@ -158,9 +165,8 @@ __device__ __forceinline__ void incident_sum(const SimDev& sim, int p, const flo
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
             if (ec[u] >= 0) {
-                const int e = ec[u] / 3, c = ec[u] - 3 * e;
-                va[u] = load_v3(a + (size_t)3 * c * sim.NE, sim.NE, e);
-                if (TWO) vb[u] = load_v3(b, sim.NE, e);
+                va[u] = load3(a + (size_t)3 * ec[u]);
+                if (TWO) vb[u] = load3(b + (size_t)3 * (ec[u] / 3));
             }
         }
 #pragma unroll

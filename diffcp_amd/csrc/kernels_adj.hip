@@ -823,10 +823,10 @@ __device__ __forceinline__ void elem_adjoint(const SimDev& sim, const InstDev& I
     }
     store_m9(Gout.eC(0), NE, e, gC);
     store_v3(Gout.en(0), NE, e, gnp);
-    store_v3(sim.ec_gx, NE, e, gx1);
-    store_v3(sim.ec_gx + (size_t)3 * NE, NE, e, gx2);
-    store_v3(sim.ec_gx + (size_t)6 * NE, NE, e, gx3);
-    store_v3(sim.ec_gv, NE, e, gvc);          // the same for the three corners
+    store3(sim.ec_gx + (size_t)9 * e, gx1);        // [element][corner][3]
+    store3(sim.ec_gx + (size_t)9 * e + 3, gx2);
+    store3(sim.ec_gx + (size_t)9 * e + 6, gx3);
+    store3(sim.ec_gv + (size_t)3 * e, gvc);        // the same for the three corners
     if (sim.per_elem) {
         sim.emod_grad[e] += (double)gp[0];
         sim.emod_grad[(size_t)NE + e] += (double)gp[2];

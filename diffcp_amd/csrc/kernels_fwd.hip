@@ -53,9 +53,9 @@ __global__ __launch_bounds__(DCP_WG) void k_prep(SimDev sim, StateView S, StateV
         store_v3(sim.ve, NE, e, ve);
         store_v3(sim.ke, NE, e, k);
         store_v3(sim.nd, NE, e, n);
-        store_v3(sim.fc, NE, e, f1);
-        store_v3(sim.fc + (size_t)3 * NE, NE, e, f2);
-        store_v3(sim.fc + (size_t)6 * NE, NE, e, f3);
+        store3(sim.fc + (size_t)9 * e, f1);          // [element][corner][3]
+        store3(sim.fc + (size_t)9 * e + 3, f2);
+        store3(sim.fc + (size_t)9 * e + 6, f3);
         if (Snext.p) store_v3(Snext.en(0), NE, e, n);
     }
 }

@@ -97,7 +97,7 @@ struct SimDev {
     const float* traj;             // trajectory pool [frames*grasp][3]
     const int* elem_inst;          // [NE] instance of each element
     // forward temporaries (element prep outputs)
-    float* xe; float* ve; float* ke; float* fc;          // [3][NE] [3][NE] [3][NE] [9][NE]
+    float* xe; float* ve; float* ke; float* fc;          // [3][NE] [3][NE] [3][NE] [NE][3][3]
     float* nd;                                            // [3][NE] n of the current d
     // grid
     int det;                       // 1: scatter sums in fixed point (bitwise reproducible), 0: float atomics
@@ -117,7 +117,7 @@ struct SimDev {
     // adjoint temporaries
     float* g_xe;                   // [3][NE] dL/dx_e of the G2P reverse
     float* g_fv;                   // [3][NP] d/d (vertex force impulse)
-    float* ec_gx; float* ec_gv;    // [9][NE] per-corner d/dx; [3][NE] d/dv (one for the three corners)
+    float* ec_gx; float* ec_gv;    // [NE][3][3] per-corner d/dx; [NE][3] d/dv (one for the three corners)
     double* pgrad_patch;           // [n_patches][waves][DCP_PATCH_GRADS]  E, nu, shear, contact, friction (element patches)
     double* pgrad_inst;            // [B][DCP_N_PARAMS]
     unsigned long long* damp_fx;   // [B][2] dL/d damping, two-word fixed point
