@@ -328,6 +328,32 @@ __device__ __forceinline__ bool active_walk(const SimDev& sim, const InstDev* in
     return true;
 }
 
+// DCP_WALK_U consecutive entries of a wave's walk (it0, it0 + stride, ...)
+// with their first cells, loaded together: a wave's few blocks then cost one
+// round of dependent loads (list entry, pool slot, cell data) instead of one
+// round per block.  blk < 0 past the end of the list.
+#define DCP_WALK_U 4
+struct WalkBatch {
+    int blk[DCP_WALK_U];
+    long long c0[DCP_WALK_U];
+};
+__device__ __forceinline__ void walk_batch(const int* list, const InstDev& I, int it0, int stride, int n,
+                                           WalkBatch& wb) {
+#pragma unroll
+    for (int u = 0; u < DCP_WALK_U; ++u) {
+        const int it = it0 + u * stride;
+        wb.blk[u] = it < n ? list[it] : -1;
+    }
+#pragma unroll
+    for (int u = 0; u < DCP_WALK_U; ++u) wb.c0[u] = wb.blk[u] >= 0 ? block_cell0(I, wb.blk[u]) : 0;
+}
+
+__device__ __forceinline__ int wave_sum_i(int v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
 // v_out of one cell (simulation.py:277-302) with the damping extension;
 // mask[c] = 1 where the component passed the wall conditions
 __device__ __forceinline__ void grid_velocity(const InstDev& I, float4 c, int i, int j, int k, float v[3], float* mask,

@@ -438,41 +438,53 @@ __global__ __launch_bounds__(DCP_WG) void k_gridop_adj(SimDev sim) {
     long long dhi = 0, dlo = 0;   // deterministic sums: this wave's dL/d damping, fixed point
     bool dok = true;
     double gdamp_wave = 0.0;      // float-atomic sums: this lane's part of it
-    for (int it = w.first; it < w.n; it += w.stride) {
-        const int blk = sim.active[I.flag_base + it];
-        const long long cell = block_cell0(I, blk) + lane;
-        int ci, cj, ck;
-        block_cell_coords(I, blk, lane, ci, cj, ck);
-        float4 c = sim.acc[cell];
-        float4 ga;
-        if (sim.det) {
-            float gv[3];
-            fx_take_cell<3>(sim.gaccq, cell, poisoned, gv);
-            ga = make_float4(0.f, gv[0], gv[1], gv[2]);
-        } else {
-            ga = sim.gacc[cell];
-            sim.gacc[cell] = make_float4(0.f, 0.f, 0.f, 0.f);
+    for (int it0 = w.first; it0 < w.n; it0 += DCP_WALK_U * w.stride) {
+        WalkBatch wb;
+        walk_batch(sim.active + I.flag_base, I, it0, w.stride, w.n, wb);
+        float4 cs[DCP_WALK_U], gs[DCP_WALK_U];
+#pragma unroll
+        for (int u = 0; u < DCP_WALK_U; ++u) {
+            const bool in = wb.blk[u] >= 0;
+            cs[u] = in ? sim.acc[wb.c0[u] + lane] : make_float4(0.f, 0.f, 0.f, 0.f);
+            gs[u] = (in && !sim.det) ? sim.gacc[wb.c0[u] + lane] : make_float4(0.f, 0.f, 0.f, 0.f);
         }
-        float v[3], mask[3], vr[3];
-        grid_velocity(I, c, ci, cj, ck, v, mask, vr);
-        float gd0 = mask[0] * ga.y, gd1 = mask[1] * ga.z, gd2 = mask[2] * ga.w;
-        double gdamp = -(double)I.dt * ((double)gd0 * vr[0] + (double)gd1 * vr[1] + (double)gd2 * vr[2]);
-        float damp = 1.0f - I.damping * I.dt;
-        float g0 = damp * gd0, g1 = damp * gd1, g2 = damp * gd2;
-        float inv_m = 1.0f / (c.x + 1e-9f);
-        float gm = -inv_m * (g0 * vr[0] + g1 * vr[1] + g2 * vr[2]);
-        sim.gmv[cell] = make_float4(gm, inv_m * g0, inv_m * g1, inv_m * g2);
-        sim.acc[cell] = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (sim.det) {
-            // the block's sum in lane order, then fixed point: the same bits
-            // whichever wave takes the block and in whatever order
-            const double sb = wave_sum_d(gdamp);
-            long long h, l;
-            if (fx_split(sb, h, l)) { dhi += h; dlo += l; } else { dok = false; }
-        } else {
-            gdamp_wave += gdamp;
+#pragma unroll
+        for (int u = 0; u < DCP_WALK_U; ++u) {
+            const int blk = wb.blk[u];
+            if (blk < 0) break;
+            const long long cell = wb.c0[u] + lane;
+            int ci, cj, ck;
+            block_cell_coords(I, blk, lane, ci, cj, ck);
+            const float4 c = cs[u];
+            float4 ga = gs[u];
+            if (sim.det) {
+                float gv[3];
+                fx_take_cell<3>(sim.gaccq, cell, poisoned, gv);
+                ga = make_float4(0.f, gv[0], gv[1], gv[2]);
+            } else {
+                sim.gacc[cell] = make_float4(0.f, 0.f, 0.f, 0.f);
+            }
+            float v[3], mask[3], vr[3];
+            grid_velocity(I, c, ci, cj, ck, v, mask, vr);
+            float gd0 = mask[0] * ga.y, gd1 = mask[1] * ga.z, gd2 = mask[2] * ga.w;
+            double gdamp = -(double)I.dt * ((double)gd0 * vr[0] + (double)gd1 * vr[1] + (double)gd2 * vr[2]);
+            float damp = 1.0f - I.damping * I.dt;
+            float g0 = damp * gd0, g1 = damp * gd1, g2 = damp * gd2;
+            float inv_m = 1.0f / (c.x + 1e-9f);
+            float gm = -inv_m * (g0 * vr[0] + g1 * vr[1] + g2 * vr[2]);
+            sim.gmv[cell] = make_float4(gm, inv_m * g0, inv_m * g1, inv_m * g2);
+            sim.acc[cell] = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (sim.det) {
+                // the block's sum in lane order, then fixed point: the same bits
+                // whichever wave takes the block and in whatever order
+                const double sb = wave_sum_d(gdamp);
+                long long h, l;
+                if (fx_split(sb, h, l)) { dhi += h; dlo += l; } else { dok = false; }
+            } else {
+                gdamp_wave += gdamp;
+            }
+            if (lane == 0) sim.flags[I.flag_base + blk] = 0;
         }
-        if (lane == 0) sim.flags[I.flag_base + blk] = 0;
     }
     if (sim.det) {
         if (lane == 0) {

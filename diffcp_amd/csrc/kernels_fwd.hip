@@ -480,15 +480,18 @@ __global__ __launch_bounds__(DCP_WG) void k_active_list(SimDev sim, const int* s
 }
 
 // Offset of instance b's blocks in a snapshot and the total, from per-instance
-// block counts (B is at most a few hundred: a loop per wave is cheap).
+// block counts: lane l reads instances l, l + 64, ... (one round of loads for
+// up to 64 instances), then two wave sums.  Called by every lane of the wave.
 __device__ __forceinline__ void snap_offsets(const SimDev& sim, const int* counts, int b, int& base, int& total) {
-    base = 0; total = 0;
-    for (int q = 0; q < sim.B; ++q) {
+    int pb = 0, pt = 0;
+    for (int q = threadIdx.x & 63; q < sim.B; q += 64) {
         const int nb = sim.inst[q].nb;
         const int n = min(counts[q], nb * nb * nb);
-        if (q < b) base += n;
-        total += n;
+        if (q < b) pb += n;
+        pt += n;
     }
+    base = wave_sum_i(pb);
+    total = wave_sum_i(pt);
 }
 
 // Grid update; with `save`, the (m, m v) of the active blocks are copied into
@@ -510,32 +513,40 @@ __global__ __launch_bounds__(DCP_WG) void k_gridop(SimDev sim, int keep, GridSna
     }
     const bool saving = save.acc && total <= save.cap;
     const bool poisoned = sim.poison[w.b] != 0;
-    for (int it = w.first; it < w.n; it += w.stride) {
-        const int blk = sim.active[I.flag_base + it];
-        const long long cell = block_cell0(I, blk) + lane;
-        int ci, cj, ck;
-        block_cell_coords(I, blk, lane, ci, cj, ck);
-        float4 c;
-        if (sim.det) {
-            float cv[4];
-            fx_take_cell<4>(sim.accq, cell, poisoned, cv);
-            c = make_float4(cv[0], cv[1], cv[2], cv[3]);
-        } else {
-            c = sim.acc[cell];
-        }
-        if (saving) {
-            save.acc[(size_t)(base + it) * 64 + lane] = c;
-            if (lane == 0) save.blk[base + it] = blk;
-        }
-        float v[3];
-        grid_velocity(I, c, ci, cj, ck, v, nullptr);
-        sim.vout[cell] = make_float4(v[0], v[1], v[2], 0.f);
-        if (!(isfinite(v[0]) && isfinite(v[1]) && isfinite(v[2]))) atomicOr(sim.status, (unsigned)DCP_STATUS_NONFINITE);
-        if (keep) {
-            if (sim.det) sim.acc[cell] = c;    // for the grid-update adjoint, which clears it
-        } else {
-            if (!sim.det) sim.acc[cell] = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (lane == 0) sim.flags[I.flag_base + blk] = 0;
+    for (int it0 = w.first; it0 < w.n; it0 += DCP_WALK_U * w.stride) {
+        WalkBatch wb;
+        walk_batch(sim.active + I.flag_base, I, it0, w.stride, w.n, wb);
+        float4 cs[DCP_WALK_U];
+#pragma unroll
+        for (int u = 0; u < DCP_WALK_U; ++u)
+            cs[u] = (wb.blk[u] >= 0 && !sim.det) ? sim.acc[wb.c0[u] + lane] : make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+        for (int u = 0; u < DCP_WALK_U; ++u) {
+            const int blk = wb.blk[u], it = it0 + u * w.stride;
+            if (blk < 0) break;
+            const long long cell = wb.c0[u] + lane;
+            int ci, cj, ck;
+            block_cell_coords(I, blk, lane, ci, cj, ck);
+            float4 c = cs[u];
+            if (sim.det) {
+                float cv[4];
+                fx_take_cell<4>(sim.accq, cell, poisoned, cv);
+                c = make_float4(cv[0], cv[1], cv[2], cv[3]);
+            }
+            if (saving) {
+                save.acc[(size_t)(base + it) * 64 + lane] = c;
+                if (lane == 0) save.blk[base + it] = blk;
+            }
+            float v[3];
+            grid_velocity(I, c, ci, cj, ck, v, nullptr);
+            sim.vout[cell] = make_float4(v[0], v[1], v[2], 0.f);
+            if (!(isfinite(v[0]) && isfinite(v[1]) && isfinite(v[2]))) atomicOr(sim.status, (unsigned)DCP_STATUS_NONFINITE);
+            if (keep) {
+                if (sim.det) sim.acc[cell] = c;    // for the grid-update adjoint, which clears it
+            } else {
+                if (!sim.det) sim.acc[cell] = make_float4(0.f, 0.f, 0.f, 0.f);
+                if (lane == 0) sim.flags[I.flag_base + blk] = 0;
+            }
         }
     }
 }
@@ -561,19 +572,29 @@ __global__ __launch_bounds__(DCP_WG) void k_grid_restore(SimDev sim, GridSnap sn
     snap_offsets(sim, snap.cnt, b, base, total);
     const int n = min(snap.cnt[b], I.nb * I.nb * I.nb);
     if (first == 0 && lane == 0) sim.n_active[b] = n;
-    for (int it = first; it < n; it += stride) {
-        const int blk = snap.blk[base + it];
-        const long long cell = block_cell0(I, blk) + lane;
-        int ci, cj, ck;
-        block_cell_coords(I, blk, lane, ci, cj, ck);
-        const float4 c = snap.acc[(size_t)(base + it) * 64 + lane];
-        sim.acc[cell] = c;
-        float v[3];
-        grid_velocity(I, c, ci, cj, ck, v, nullptr);
-        sim.vout[cell] = make_float4(v[0], v[1], v[2], 0.f);
-        if (lane == 0) {
-            sim.flags[I.flag_base + blk] = 1;
-            sim.active[I.flag_base + it] = blk;
+    for (int it0 = first; it0 < n; it0 += DCP_WALK_U * stride) {
+        WalkBatch wb;
+        walk_batch(snap.blk + base, I, it0, stride, n, wb);
+        float4 cs[DCP_WALK_U];
+#pragma unroll
+        for (int u = 0; u < DCP_WALK_U; ++u)
+            cs[u] = wb.blk[u] >= 0 ? snap.acc[(size_t)(base + it0 + u * stride) * 64 + lane] : make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+        for (int u = 0; u < DCP_WALK_U; ++u) {
+            const int blk = wb.blk[u], it = it0 + u * stride;
+            if (blk < 0) break;
+            const long long cell = wb.c0[u] + lane;
+            int ci, cj, ck;
+            block_cell_coords(I, blk, lane, ci, cj, ck);
+            const float4 c = cs[u];
+            sim.acc[cell] = c;
+            float v[3];
+            grid_velocity(I, c, ci, cj, ck, v, nullptr);
+            sim.vout[cell] = make_float4(v[0], v[1], v[2], 0.f);
+            if (lane == 0) {
+                sim.flags[I.flag_base + blk] = 1;
+                sim.active[I.flag_base + it] = blk;
+            }
         }
     }
 }
