@@ -546,6 +546,8 @@ static int build(dcp_ctx* c, const dcp_instance_desc* d, int B) {
         const dcp_instance_desc& D = d[b];
         if (D.n_grid < 4 || D.n_vertices <= 0 || D.n_elements < 0 || D.n_cube < 0 || D.n_grasper < 0)
             return set_err("instance " + std::to_string(b) + ": invalid sizes");
+        if (D.n_grid > DCP_MAX_GRID)
+            return set_err("instance " + std::to_string(b) + ": n_grid above " + std::to_string(DCP_MAX_GRID));
         if (D.n_grasper > 0 && (D.n_grasp_points <= 0 || D.n_grasper % D.n_grasp_points != 0))
             return set_err("instance " + std::to_string(b) + ": n_grasper must be a multiple of n_grasp_points");
         if (D.steps_per_frame <= 0) return set_err("steps_per_frame must be > 0");

@@ -10,6 +10,12 @@
 #define DCP_C_SYM_KEEP ((float)(1.0 - 0.95))
 #define DCP_BOUND 3        // initialization.py:22
 
+// An index that is the same for every lane of the workgroup (a patch of a
+// workgroup-strided loop, its instance), said so to the compiler: the records
+// it selects are then scalar loads held in scalar registers, not per-lane
+// copies in vector registers.
+__device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
 __device__ __forceinline__ int wave_min(int v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));

@@ -609,8 +609,8 @@ __global__ __launch_bounds__(DCP_WG) void k_g2p(SimDev sim, StateView S, StateVi
     if (blockIdx.x == 0)
         for (int b = tid; b < sim.B; b += DCP_WG) sim.poison[b] = 0;
     for (int pi = blockIdx.x; pi < sim.n_patches; pi += gridDim.x) {
-        const Patch P = sim.patches[pi];
-        const InstDev I = sim.inst[P.inst];
+        const Patch P = sim.patches[uniform(pi)];
+        const InstDev I = sim.inst[uniform(P.inst)];
         for (int q = tid; q < P.count; q += DCP_WG) {
             V3<float> x = particle_x(sim, S, P, q);
             Stencil st;
@@ -624,6 +624,9 @@ __global__ __launch_bounds__(DCP_WG) void k_g2p(SimDev sim, StateView S, StateVi
             // u*(f1, f2) = sum_{j,k} f1(j) f2(k) g with f in {W, W dp}
             V3<float> nv = {0.f, 0.f, 0.f};
             M3<float> nC = mzero<float>();
+            StencilOffsets so;
+            if (!POOL) stencil_offsets(I, st.base, so);
+            const float4* vg = sim.vout + I.grid_base;
 #pragma unroll
             for (int i = 0; i < 3; ++i) {
                 float uWW[3] = {0.f, 0.f, 0.f}, uPW[3] = {0.f, 0.f, 0.f}, uWP[3] = {0.f, 0.f, 0.f};
@@ -632,7 +635,8 @@ __global__ __launch_bounds__(DCP_WG) void k_g2p(SimDev sim, StateView S, StateVi
                     float tW[3] = {0.f, 0.f, 0.f}, tP[3] = {0.f, 0.f, 0.f};
 #pragma unroll
                     for (int k = 0; k < 3; ++k) {
-                        const float4 gv = sim.vout[cell_index<POOL>(I, st.base[0] + i, st.base[1] + j, st.base[2] + k)];
+                        const float4 gv = POOL ? sim.vout[cell_index<true>(I, st.base[0] + i, st.base[1] + j, st.base[2] + k)]
+                                               : cell_at(vg, so.x[i] + so.y[j] + so.z[k]);
                         const float w2 = st.w[k][2], wp2 = w2 * ((float)k - st.fx[2]);
                         tW[0] += w2 * gv.x; tW[1] += w2 * gv.y; tW[2] += w2 * gv.z;
                         tP[0] += wp2 * gv.x; tP[1] += wp2 * gv.y; tP[2] += wp2 * gv.z;

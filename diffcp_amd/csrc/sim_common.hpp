@@ -328,3 +328,25 @@ __device__ __forceinline__ long long cell_index(const InstDev& I, int i, int j, 
     }
     return c0 + ((i & 3) << 4) + ((j & 3) << 2) + (k & 3);
 }
+// Dense grids: the blocked index of cell (i, j, k) past the instance's first
+// cell is a sum of per-axis terms, so the 27 nodes of a stencil with base b
+// are at byte offsets x[i] + y[j] + z[k] of a float4 cell array from the
+// instance's first cell: a uniform base pointer plus one 32-bit offset per
+// node, which the loads take as they are (a grid of at most DCP_MAX_GRID^3
+// cells spans less than 2^31 bytes of float4).
+#define DCP_MAX_GRID 512
+struct StencilOffsets {
+    unsigned x[3], y[3], z[3];
+};
+__device__ __forceinline__ void stencil_offsets(const InstDev& I, const int base[3], StencilOffsets& o) {
+#pragma unroll
+    for (int t = 0; t < 3; ++t) {
+        const unsigned ci = base[0] + t, cj = base[1] + t, ck = base[2] + t, nb = (unsigned)I.nb;
+        o.x[t] = ((((ci >> 2) * nb * nb) << 6) + ((ci & 3) << 4)) << 4;
+        o.y[t] = ((((cj >> 2) * nb) << 6) + ((cj & 3) << 2)) << 4;
+        o.z[t] = (((ck >> 2) << 6) + (ck & 3)) << 4;
+    }
+}
+__device__ __forceinline__ float4 cell_at(const float4* base, unsigned byte_off) {
+    return *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(base) + byte_off);
+}

@@ -51,7 +51,7 @@ typedef struct dcp_ctx dcp_ctx;
  * src/initialization.py:21-193 and the fields of
  * src/taichi_variable_definition.py:5-52 (sizes, masses, trajectory). */
 typedef struct dcp_instance_desc {
-    int32_t n_grid;              /* grid side G (dx = 1/G)                         */
+    int32_t n_grid;              /* grid side G (dx = 1/G), 4 .. 1024              */
     int32_t n_vertices;          /* cloth vertex particles                         */
     int32_t n_elements;          /* triangles = element (quadrature) particles     */
     int32_t n_cube;              /* rigid cube particles                           */
