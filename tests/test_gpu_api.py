@@ -135,3 +135,13 @@ def test_set_targets_between_graph_replays():
         for loss, grads in out[1:]:
             assert np.allclose(loss, out[0][0], rtol=1e-6, atol=0), (len(tg), loss, out[0][0])
             assert np.allclose(grads[:, :4], out[0][1][:, :4], rtol=1e-4, atol=1e-4 * np.abs(out[0][1][:, :4]).max())
+
+
+def test_grid_side_above_the_limit_is_rejected():
+    """Grid sides are 4 .. 512 (per-node offsets of a dense grid are 32-bit)."""
+    from diffcp_amd import DiffcpError
+    from diffcp_amd.engine import Engine
+    inst = scene.tiny_instance(n=4, n_grid=16, substeps=4)
+    inst.n_grid = 513
+    with pytest.raises(DiffcpError, match="n_grid above 512"):
+        Engine([inst], max_states=2)
