@@ -128,7 +128,7 @@ template <typename T> struct DevBuf {
 #define DCP_PREP_FLOATS 21   // x_e, v_e, k, n (3 each) and three corner forces (9) per element
 
 struct dcp_ctx {
-    int B = 0, NP = 0, NE = 0, n_patches = 0, n_spatches = 0, n_cu = 256;
+    int B = 0, NP = 0, NE = 0, n_patches = 0, n_spatches = 0, n_gpatches = 0, n_gpoint = 0, n_cu = 256;
     dcp_options opt{};
     hipStream_t stream = nullptr;
     bool own_stream = false;
@@ -136,6 +136,7 @@ struct dcp_ctx {
     std::vector<InstDev> inst_h;
     std::vector<Patch> patches_h;
     std::vector<SPatch> spatches_h;
+    std::vector<Patch> gpatches_h;
     std::vector<int> tri_h;            // internal [3][NE]
     std::vector<int> elem_inst_h;
     std::vector<int> vnat_h;           // per point: natural vertex index (vertices) or index within class
@@ -269,6 +270,10 @@ struct dcp_ctx {
     int grid_patches() const {
         return std::max(1, std::min(n_patches, opt.scatter_wgs > 0 ? opt.scatter_wgs : n_cu * 16));
     }
+    // gather patches: all of them (G2P), or the point ones (P2G adjoint)
+    int grid_gpatches(bool points) const {
+        return std::max(1, std::min(points ? n_gpoint : n_gpatches, opt.scatter_wgs > 0 ? opt.scatter_wgs : n_cu * 16));
+    }
     int grid_spatches() const {
         return std::max(1, std::min(n_spatches, opt.scatter_wgs > 0 ? opt.scatter_wgs : n_cu * 16));
     }
@@ -393,9 +398,9 @@ static int launch_forward(dcp_ctx* c, int s, int g, StateView S, StateView Sn, i
     TIMED_LAUNCH(DCP_KERNEL_GRIDOP, k_gridop, dim3(c->grid_gridop()), dim3(DCP_WG), 0, c->stream, c->sim, 0, c->snap(snap),
                  nullptr);
     if (c->pooled())
-        TIMED_LAUNCH(DCP_KERNEL_G2P, k_g2p<true>, dim3(c->grid_patches()), dim3(DCP_WG), 0, c->stream, c->sim, S, Sn);
+        TIMED_LAUNCH(DCP_KERNEL_G2P, k_g2p<true>, dim3(c->grid_gpatches(false)), dim3(DCP_WG), 0, c->stream, c->sim, S, Sn);
     else
-        TIMED_LAUNCH(DCP_KERNEL_G2P, k_g2p<false>, dim3(c->grid_patches()), dim3(DCP_WG), 0, c->stream, c->sim, S, Sn);
+        TIMED_LAUNCH(DCP_KERNEL_G2P, k_g2p<false>, dim3(c->grid_gpatches(false)), dim3(DCP_WG), 0, c->stream, c->sim, S, Sn);
     return 0;
 }
 
@@ -487,10 +492,10 @@ static int launch_adjoint_tail(dcp_ctx* c, int s, int g, StateView S, StateView 
     PrepSlot ps(c, prep_saved(c, snap) ? snap : -1);
     const int fresh = (s == 0);
     if (c->pooled())
-        TIMED_LAUNCH(DCP_KERNEL_P2G_ADJ, k_p2g_adj<true>, dim3(c->grid_patches()), dim3(DCP_WG), 0, c->stream, c->sim, S,
+        TIMED_LAUNCH(DCP_KERNEL_P2G_ADJ, k_p2g_adj<true>, dim3(c->grid_gpatches(true)), dim3(DCP_WG), 0, c->stream, c->sim, S,
                      Gout, s, g);
     else
-        TIMED_LAUNCH(DCP_KERNEL_P2G_ADJ, k_p2g_adj<false>, dim3(c->grid_patches()), dim3(DCP_WG), 0, c->stream, c->sim,
+        TIMED_LAUNCH(DCP_KERNEL_P2G_ADJ, k_p2g_adj<false>, dim3(c->grid_gpatches(true)), dim3(DCP_WG), 0, c->stream, c->sim,
                      S, Gout, s, g);
     if (launch_elem_adj(c, S, G1, Gout, fresh, s, g)) return -1;
     if (finalize)
@@ -736,6 +741,24 @@ static int build(dcp_ctx* c, const dcp_instance_desc* d, int B) {
         if (P.cls == CLS_GRASPER || P.cls == CLS_CUBE)
             c->spatches_h.push_back(SPatch{P.inst, P.cls, P.start, P.count, CLS_ELEMENT, 0, 0});
     c->n_spatches = (int)c->spatches_h.size();
+    // gather patches (G2P, P2G adjoint: per-particle stencil reads, no tile):
+    // each class of an instance cut into full patches of ps particles in
+    // their bin order, graspers one grasp point per patch; point patches first
+    for (int b = 0; b < B; ++b) {
+        const InstDev& I = c->inst_h[b];
+        auto gchunk = [&](int cls, int start, int count) {
+            for (int o = 0; o < count; o += ps) c->gpatches_h.push_back(Patch{b, cls, start + o, std::min(ps, count - o)});
+        };
+        gchunk(CLS_VERTEX, I.v_start, I.v_count);
+        for (const Patch& P : c->patches_h)
+            if (P.inst == b && P.cls == CLS_GRASPER) c->gpatches_h.push_back(P);
+        gchunk(CLS_CUBE, I.c_start, I.c_count);
+    }
+    c->n_gpoint = (int)c->gpatches_h.size();
+    for (int b = 0; b < B; ++b)
+        for (int o = 0; o < c->inst_h[b].e_count; o += ps)
+            c->gpatches_h.push_back(Patch{b, CLS_ELEMENT, c->inst_h[b].e_start + o, std::min(ps, c->inst_h[b].e_count - o)});
+    c->n_gpatches = (int)c->gpatches_h.size();
     // 4^3 blocks the initial stencils touch (sizes the grid snapshots)
     c->init_blocks = 0;
     for (int b = 0; b < B; ++b) {
@@ -773,13 +796,15 @@ static int build(dcp_ctx* c, const dcp_instance_desc* d, int B) {
     S.max_active = flags;
     c->slot_floats = 15 * (size_t)np + 12 * (size_t)ne;
     c->max_states = std::max(2, c->opt.max_states);
-    Patch* pd; SPatch* spd; int *trid, *eid, *iod, *ild; float* trd;
-    if (c->upload(&pd, c->patches_h) || c->upload(&spd, c->spatches_h) || c->upload(&trid, c->tri_h) ||
+    Patch *pd, *gpd; SPatch* spd; int *trid, *eid, *iod, *ild; float* trd;
+    if (c->upload(&pd, c->patches_h) || c->upload(&spd, c->spatches_h) || c->upload(&gpd, c->gpatches_h) ||
+        c->upload(&trid, c->tri_h) ||
         c->upload(&eid, c->elem_inst_h) ||
         c->upload(&iod, inc_off) || c->upload(&ild, inc_list) || c->upload(&trd, trajh) ||
         c->upload(&c->inst_d, c->inst_h) || c->upload(&c->vnat, c->vnat_h))
         return -1;
-    S.inst = c->inst_d; S.patches = pd; S.spatches = spd; S.n_spatches = c->n_spatches; S.tri = trid; S.elem_inst = eid; S.inc_off = iod; S.inc_list = ild; S.traj = trd;
+    S.inst = c->inst_d; S.patches = pd; S.spatches = spd; S.n_spatches = c->n_spatches;
+    S.gpatches = gpd; S.n_gpatches = c->n_gpatches; S.n_gpoint = c->n_gpoint; S.tri = trid; S.elem_inst = eid; S.inc_off = iod; S.inc_list = ild; S.traj = trd;
     // overflow lists of the two scatters; the counts start at zero and the
     // *_big kernels leave them at zero
     if (c->alloc(&S.big_list, 2 * (size_t)std::max(1, c->n_spatches)) || c->upload(&S.big_n, std::vector<int>(4, 0)))

@@ -5,8 +5,8 @@
 //   k_g2p_adj       per patch: dL/d grid_v_out scattered through an LDS tile
 //                   (the same tiling as P2G), dL/dx through the weights
 //   k_gridop_adj    per active block: wall masks, damping, normalisation
-//   k_p2g_adj       per vertex / grasper / cube patch: 27-cell gather of
-//                   dL/d(m, mv) over an LDS tile
+//   k_p2g_adj       per vertex / grasper / cube particle: 27-cell gather of
+//                   dL/d(m, mv) straight from the grid
 //   k_elem_adj      per element patch: the same gather for the element
 //                   particles feeding, in registers, the element prep adjoint
 //                   (stress, QR, polar (FP64), kinematics); parameter
@@ -710,45 +710,41 @@ __device__ __forceinline__ bool p2g_adj_stage(const SimDev& sim, const InstDev& 
     return tiled;
 }
 
-// P2G adjoint of the vertex, grasper and cube patches, one particle per lane:
-// dL/d(m, m v) of the patch tile is staged in LDS once per patch; a patch
-// whose tile exceeds the cap reads the grid.  Element patches are left to
-// k_elem_adj, which needs the vertex force-impulse gradients stored here.
+// P2G adjoint of the vertex, grasper and cube particles, one particle per lane
+// over the point gather patches: each lane reads its stencil's 27 nodes of
+// dL/d(m, m v) from the grid (no tile: no box, no barrier; the reads hit the
+// L2).  Element particles are left to k_elem_adj, which needs the vertex
+// force-impulse gradients stored here.
 template <bool POOL>
-__global__ __launch_bounds__(DCP_WG, 2) void k_p2g_adj(SimDev sim, StateView S, StateView Gout, int s, int g) {
+__global__ __launch_bounds__(DCP_WG) void k_p2g_adj(SimDev sim, StateView S, StateView Gout, int s, int g) {
     // the grid-update adjoint, just before on the stream, has read the
     // adjoint scatter's poison flags
     if (blockIdx.x == 0)
         for (int b = threadIdx.x; b < sim.B; b += DCP_WG) sim.poison[sim.B + b] = 0;
-    __shared__ int box[6];
-    __shared__ float4 gt[DCP_STILE_CAP];
     const int tid = threadIdx.x;
-    if (tid < 3) { box[tid] = 1 << 30; box[3 + tid] = -(1 << 30); }
-    __syncthreads();
-    for (int pi = blockIdx.x; pi < sim.n_patches; pi += gridDim.x) {
-        const Patch P = sim.patches[pi];
-        if (P.cls == CLS_ELEMENT) continue;
-        const InstDev I = sim.inst[P.inst];
+    for (int pi = blockIdx.x; pi < sim.n_gpoint; pi += gridDim.x) {
+        const Patch P = sim.gpatches[uniform(pi)];
+        const InstDev I = sim.inst[uniform(P.inst)];
         const bool act = tid < P.count;   // patches hold at most DCP_WG particles
         P2GRec r;
-        Stencil st;
-        PatchBox B;
-        const bool tiled = p2g_adj_stage<POOL>(sim, I, S, P, s, g, act, r, st, box, gt, B);
         P2GAdj o;
         if (act) {
-            if (tiled) {
-                const int key = tile_cell(B, st.base[0], st.base[1], st.base[2]);
-                p2g_adj_gather<false>(I, r, st, [&](int i, int j, int k) { return gt[key + (i * B.e1 + j) * B.e2 + k]; }, o);
-            } else {
+            p2g_record(sim, I, S, P, tid, s, g, r);
+            Stencil st;
+            make_stencil(r.x, I.inv_dx, I.G, st, nullptr);
+            if (POOL) {
                 p2g_adj_gather<false>(I, r, st, [&](int i, int j, int k) {
-                    return sim.gmv[cell_index<POOL>(I, st.base[0] + i, st.base[1] + j, st.base[2] + k)];
+                    return sim.gmv[cell_index<true>(I, st.base[0] + i, st.base[1] + j, st.base[2] + k)];
                 }, o);
+            } else {
+                StencilOffsets so;
+                stencil_offsets(I, st.base, so);
+                const float4* gg = sim.gmv + I.grid_base;
+                p2g_adj_gather<false>(I, r, st, [&](int i, int j, int k) { return cell_at(gg, so.x[i] + so.y[j] + so.z[k]); }, o);
             }
             p2g_adj_store_point(sim, I, Gout, P, tid, r, o);
         }
         if (P.cls == CLS_GRASPER) grasper_traj_grad(sim, I, P, s, g, act, r, o);
-        if (tid < 3) { box[tid] = 1 << 30; box[3 + tid] = -(1 << 30); }
-        __syncthreads();
     }
 }
 

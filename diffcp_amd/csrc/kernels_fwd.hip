@@ -13,8 +13,8 @@
 //              weights, simulation.py:266-271)
 //   k_gridop   per active 4^3 block: normalise, damping, wall conditions
 //              (simulation.py:274-302)
-//   k_g2p      per patch: APIC gather, x/v/C update, element C damping
-//              (simulation.py:304-383)
+//   k_g2p      per particle of full-size gather patches: APIC gather, x/v/C
+//              update, element C damping (simulation.py:304-383)
 #include "kernels.hpp"
 #include "patch_sort.hpp"
 
@@ -608,8 +608,8 @@ __global__ __launch_bounds__(DCP_WG) void k_g2p(SimDev sim, StateView S, StateVi
     // the grid update, just before on the stream, has read the poison flags
     if (blockIdx.x == 0)
         for (int b = tid; b < sim.B; b += DCP_WG) sim.poison[b] = 0;
-    for (int pi = blockIdx.x; pi < sim.n_patches; pi += gridDim.x) {
-        const Patch P = sim.patches[uniform(pi)];
+    for (int pi = blockIdx.x; pi < sim.n_gpatches; pi += gridDim.x) {
+        const Patch P = sim.gpatches[uniform(pi)];
         const InstDev I = sim.inst[uniform(P.inst)];
         for (int q = tid; q < P.count; q += DCP_WG) {
             V3<float> x = particle_x(sim, S, P, q);

@@ -85,6 +85,8 @@ struct SimDev {
     const Patch* patches;          // point patches then element patches
     const SPatch* spatches;        // scatter patches (P2G and the G2P adjoint)
     int n_spatches;
+    const Patch* gpatches;         // gather patches (G2P, P2G adjoint): full chunks, point patches first
+    int n_gpatches, n_gpoint;
     int* big_list;                 // [2][n_spatches]: scatter patches whose tile exceeds the cap
     int* big_n;                    // [4]: list lengths, finished-workgroup counters
     const int* tri;                // [3][NE] point indices
