@@ -270,10 +270,13 @@ struct dcp_ctx {
     int grid_patches() const {
         return std::max(1, std::min(n_patches, opt.scatter_wgs > 0 ? opt.scatter_wgs : n_cu * 16));
     }
-    // gather patches: all of them (G2P), or the point ones (P2G adjoint)
+    // gather patches: all of them (G2P), the point ones (P2G adjoint) or the
+    // element ones (element adjoint)
+    int n_gelem() const { return n_gpatches - n_gpoint; }
     int grid_gpatches(bool points) const {
         return std::max(1, std::min(points ? n_gpoint : n_gpatches, opt.scatter_wgs > 0 ? opt.scatter_wgs : n_cu * 16));
     }
+    int grid_gelem() const { return std::max(1, std::min(n_gelem(), opt.scatter_wgs > 0 ? opt.scatter_wgs : n_cu * 16)); }
     int grid_spatches() const {
         return std::max(1, std::min(n_spatches, opt.scatter_wgs > 0 ? opt.scatter_wgs : n_cu * 16));
     }
@@ -467,10 +470,10 @@ static int launch_adjoint_head(dcp_ctx* c, StateView S, StateView G1, StateView 
 template <typename PT, bool RM>
 static int launch_elem_adj_form(dcp_ctx* c, StateView S, StateView G1, StateView Gout, int fresh, int s, int g) {
     if (c->pooled())
-        TIMED_LAUNCH(DCP_KERNEL_PREP_ADJ, (k_elem_adj<PT, RM, true>), dim3(c->grid_patches()), dim3(DCP_WG), 0,
+        TIMED_LAUNCH(DCP_KERNEL_PREP_ADJ, (k_elem_adj<PT, RM, true>), dim3(c->grid_gelem()), dim3(DCP_WG), 0,
                      c->stream, c->sim, S, G1, Gout, fresh, s, g);
     else
-        TIMED_LAUNCH(DCP_KERNEL_PREP_ADJ, (k_elem_adj<PT, RM, false>), dim3(c->grid_patches()), dim3(DCP_WG), 0,
+        TIMED_LAUNCH(DCP_KERNEL_PREP_ADJ, (k_elem_adj<PT, RM, false>), dim3(c->grid_gelem()), dim3(DCP_WG), 0,
                      c->stream, c->sim, S, G1, Gout, fresh, s, g);
     return 0;
 }
@@ -834,7 +837,7 @@ static int build(dcp_ctx* c, const dcp_instance_desc* d, int B) {
         c->alloc(&S.status, 1) || c->alloc(&S.g_xe, 3 * (size_t)ne) ||
         c->alloc(&S.g_fv, 3 * (size_t)np) ||
         c->alloc(&S.ec_gx, 9 * (size_t)ne) || c->alloc(&S.ec_gv, 3 * (size_t)ne) ||
-        c->alloc(&S.pgrad_patch, (size_t)DCP_PATCH_GRADS * (DCP_WG / 64) * c->n_patches) || c->alloc(&S.pgrad_inst, (size_t)DCP_N_PARAMS * B) ||
+        c->alloc(&S.pgrad_patch, (size_t)DCP_PATCH_GRADS * (DCP_WG / 64) * std::max(1, c->n_gelem())) || c->alloc(&S.pgrad_inst, (size_t)DCP_N_PARAMS * B) ||
         c->alloc(&S.emod_grad, 3 * (size_t)ne))
         return -1;
     if (c->alloc(&c->states, c->slot_floats * c->max_states) || c->alloc(&c->adj, c->slot_floats * c->n_adj) ||
@@ -1152,7 +1155,7 @@ extern "C" int dcp_zero_adjoint(dcp_ctx* c, int32_t gs) {
 
 extern "C" int dcp_zero_param_grads(dcp_ctx* c) {
     if (!c) return set_err("bad args");
-    HIP_OK(hipMemsetAsync(c->sim.pgrad_patch, 0, DCP_PATCH_GRADS * (DCP_WG / 64) * sizeof(double) * c->n_patches,
+    HIP_OK(hipMemsetAsync(c->sim.pgrad_patch, 0, DCP_PATCH_GRADS * (DCP_WG / 64) * sizeof(double) * c->n_gelem(),
                           c->stream));
     HIP_OK(hipMemsetAsync(c->sim.pgrad_inst, 0, DCP_N_PARAMS * sizeof(double) * c->B, c->stream));
     HIP_OK(hipMemsetAsync(c->sim.emod_grad, 0, 3 * sizeof(double) * c->NE, c->stream));
@@ -1193,7 +1196,7 @@ static int reduce_param_grads(dcp_ctx* c, double* out_caller) {
     std::vector<double> outv((size_t)DCP_N_PARAMS * c->B);
     double* out = outv.data();
     const int nw = DCP_WG / 64;
-    std::vector<double> pp((size_t)DCP_PATCH_GRADS * nw * c->n_patches), pi((size_t)DCP_N_PARAMS * c->B);
+    std::vector<double> pp((size_t)DCP_PATCH_GRADS * nw * c->n_gelem()), pi((size_t)DCP_N_PARAMS * c->B);
     HIP_OK(hipMemcpyAsync(pp.data(), c->sim.pgrad_patch, pp.size() * 8, hipMemcpyDeviceToHost, c->stream));
     HIP_OK(hipMemcpyAsync(pi.data(), c->sim.pgrad_inst, pi.size() * 8, hipMemcpyDeviceToHost, c->stream));
     std::vector<unsigned long long> dfx(2 * (size_t)c->B);
@@ -1202,9 +1205,8 @@ static int reduce_param_grads(dcp_ctx* c, double* out_caller) {
     for (int b = 0; b < c->B; ++b)
         for (int k = 0; k < DCP_N_PARAMS; ++k) out[b * DCP_N_PARAMS + k] = pi[b * DCP_N_PARAMS + k];
     for (int b = 0; b < c->B; ++b) out[b * DCP_N_PARAMS + 4] += fx_value(dfx[2 * b], dfx[2 * b + 1]);
-    for (int p = 0; p < c->n_patches; ++p) {
-        const Patch& P = c->patches_h[p];
-        if (P.cls != CLS_ELEMENT) continue;
+    for (int p = 0; p < c->n_gelem(); ++p) {
+        const Patch& P = c->gpatches_h[c->n_gpoint + p];
         // E, nu, shearing, contact, friction (parameter slots 0-3 and 5), waves in order
         for (int w = 0; w < nw; ++w) {
             const double* q = &pp[DCP_PATCH_GRADS * ((size_t)p * nw + w)];

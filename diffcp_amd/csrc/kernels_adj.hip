@@ -846,11 +846,12 @@ __device__ __forceinline__ void elem_adjoint(const SimDev& sim, const InstDev& I
     }
 }
 
-// Element patches, one element per lane: the P2G reverse of the element
-// particle (gather of dL/d(m, m v) over the patch tile) feeds the element's
-// prep adjoint in registers.  Parameter gradients are reduced per patch in
-// FP64 in a fixed order.  Runs after k_p2g_adj (the vertex force-impulse
-// gradients).
+// Element gather patches (full 256-element chunks of an instance's elements in
+// their bin order), one element per lane: the P2G reverse of the element
+// particle (gather of dL/d(m, m v) over the patch tile, or from the grid when
+// the chunk's box exceeds the tile cap) feeds the element's prep adjoint in
+// registers.  Parameter gradients are reduced per patch in FP64 in a fixed
+// order.  Runs after k_p2g_adj (the vertex force-impulse gradients).
 template <typename PT, bool RM, bool POOL>
 __global__ __launch_bounds__(DCP_WG, 2) void k_elem_adj(SimDev sim, StateView S, StateView G1, StateView Gout,
                                                         int fresh, int s, int g) {
@@ -863,9 +864,8 @@ __global__ __launch_bounds__(DCP_WG, 2) void k_elem_adj(SimDev sim, StateView S,
     if (tid < 6) { box[tid / 3][tid % 3] = 1 << 30; box[tid / 3][3 + tid % 3] = -(1 << 30); }
     __syncthreads();
     int par = 0;
-    for (int pi = blockIdx.x; pi < sim.n_patches; pi += gridDim.x) {
-        const Patch P = sim.patches[pi];
-        if (P.cls != CLS_ELEMENT) continue;
+    for (int pi = blockIdx.x; pi < sim.n_gpatches - sim.n_gpoint; pi += gridDim.x) {
+        const Patch P = sim.gpatches[sim.n_gpoint + pi];
         const InstDev I = sim.inst[P.inst];
         const bool act = tid < P.count;   // patches hold at most DCP_WG particles
         P2GRec r;

@@ -120,7 +120,7 @@ struct SimDev {
     float* g_xe;                   // [3][NE] dL/dx_e of the G2P reverse
     float* g_fv;                   // [3][NP] d/d (vertex force impulse)
     float* ec_gx; float* ec_gv;    // [NE][3][3] per-corner d/dx; [NE][3] d/dv (one for the three corners)
-    double* pgrad_patch;           // [n_patches][waves][DCP_PATCH_GRADS]  E, nu, shear, contact, friction (element patches)
+    double* pgrad_patch;           // [element gather patches][waves][DCP_PATCH_GRADS]  E, nu, shear, contact, friction
     double* pgrad_inst;            // [B][DCP_N_PARAMS]
     unsigned long long* damp_fx;   // [B][2] dL/d damping, two-word fixed point
     double* emod_grad;             // [3][NE]
