@@ -812,9 +812,9 @@ static int build(dcp_ctx* c, const dcp_instance_desc* d, int B) {
     // *_big kernels leave them at zero
     if (c->alloc(&S.big_list, 2 * (size_t)std::max(1, c->n_spatches)) || c->upload(&S.big_n, std::vector<int>(4, 0)))
         return -1;
-    float *Dinv, *Dii, *emod;
-    if (c->alloc(&Dinv, 9 * (size_t)ne) || c->alloc(&Dii, 9 * (size_t)ne)) return -1;
-    S.Dinv = Dinv; S.Dinvinv = Dii;
+    float *Dinv, *emod;
+    if (c->alloc(&Dinv, 9 * (size_t)ne)) return -1;
+    S.Dinv = Dinv;
     if (c->opt.per_element_moduli) {
         if (c->alloc(&emod, 3 * (size_t)ne)) return -1;
         S.emod = emod;
@@ -1075,7 +1075,7 @@ extern "C" int dcp_reset(dcp_ctx* c, int32_t slot, const double* vx, const doubl
     }
     const int NP = c->NP, NE = c->NE;
     std::vector<float> st(c->slot_floats, 0.f);
-    std::vector<float> Dinv(9 * (size_t)NE), Dii(9 * (size_t)NE);
+    std::vector<float> Dinv(9 * (size_t)NE);
     size_t nv = 0, no = 0, nc = 0;
     std::vector<float> xv, xo, xc;   // this reset's positions (the creation ones unless given)
     for (int b = 0; b < c->B; ++b) {
@@ -1110,15 +1110,13 @@ extern "C" int dcp_reset(dcp_ctx* c, int32_t slot, const double* vx, const doubl
             float l3 = std::sqrt(c3[0] * c3[0] + c3[1] * c3[1] + c3[2] * c3[2] + 1e-6f);
             for (int k = 0; k < 3; ++k) n3[k] = c3[k] / l3;
             float D[9] = {d1[0], d2[0], n3[0], d1[1], d2[1], n3[1], d1[2], d2[2], n3[2]};
-            float Di[9], Dii9[9];
+            float Di[9];
             inv3h(D, Di);
-            inv3h(Di, Dii9);
-            for (int k = 0; k < 9; ++k) { Dinv[(size_t)k * NE + e] = Di[k]; Dii[(size_t)k * NE + e] = Dii9[k]; }
+            for (int k = 0; k < 9; ++k) Dinv[(size_t)k * NE + e] = Di[k];
             for (int k = 0; k < 3; ++k) st[15 * (size_t)NP + (size_t)(9 + k) * NE + e] = n3[k];
         }
     }
     HIP_OK(hipMemcpyAsync((void*)c->sim.Dinv, Dinv.data(), Dinv.size() * 4, hipMemcpyHostToDevice, c->stream));
-    HIP_OK(hipMemcpyAsync((void*)c->sim.Dinvinv, Dii.data(), Dii.size() * 4, hipMemcpyHostToDevice, c->stream));
     HIP_OK(hipMemcpyAsync(c->states + (size_t)slot * c->slot_floats, st.data(), st.size() * 4, hipMemcpyHostToDevice,
                           c->stream));
     if (c->reset_pool()) return set_err("dcp_reset: pool reset launch failed");
@@ -1131,7 +1129,7 @@ extern "C" int dcp_rebase_elements(dcp_ctx* c, int32_t slot) {
     if (!c || !slot_ok(c, slot)) return set_err("dcp_rebase_elements: bad slot");
     if (!c->f_old && c->alloc(&c->f_old, 9 * (size_t)std::max(1, c->NE))) return -1;
     hipLaunchKernelGGL(k_rebase, dim3(c->grid_elems()), dim3(DCP_WG), 0, c->stream, c->sim, c->state(slot), c->f_old,
-                       (float*)c->sim.Dinv, (float*)c->sim.Dinvinv);
+                       (float*)c->sim.Dinv);
     LAUNCH_OK();
     c->rebase_pending = true;
     return 0;

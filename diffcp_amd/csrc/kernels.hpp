@@ -96,9 +96,8 @@ __device__ __forceinline__ void element_kinematics(const SimDev& sim, const Inst
         K.F = meye<float>();
         K.d = K.dn;
     } else {
-        M3<float> Dii = load_m9(sim.Dinvinv, NE, e);
         K.F = mmul(K.dn, K.Dinv);                 // new_F = new_d @ Dinv       (:407)
-        K.d = mmul(K.F, Dii);                     // d = new_F @ inverse(Dinv)  (:429)
+        K.d = mmul(K.F, minv(K.Dinv));            // d = new_F @ inverse(Dinv)  (:429)
     }
     if (fresh) {
         K.xe = {(x1.x + x2.x + x3.x) / 3.0f, (x1.y + x2.y + x3.y) / 3.0f, (x1.z + x2.z + x3.z) / 3.0f};
@@ -116,7 +115,7 @@ __device__ __forceinline__ void apply_return_map(const SimDev& sim, const InstDe
     RMState rs;
     K.Fr = K.F;
     K.F = return_map_forward(K.Fr, prm.ks, prm.kc, I.friction, rs);
-    K.d = mmul(K.F, load_m9(sim.Dinvinv, sim.NE, e));
+    K.d = mmul(K.F, minv(K.Dinv));
 }
 
 __device__ __forceinline__ V3<float> particle_x(const SimDev& sim, const StateView& S, const Patch& P, int q) {
@@ -408,7 +407,7 @@ __global__ void k_loss_sum(const float* pmin, const int* pnat, const int* parg, 
                            float* tmin, int* targ, const int* toff, const int* tcount, double* loss, int B);
 __global__ void k_binning_keys(SimDev sim, StateView S, int* keys);
 __global__ void k_patch_order(SimDev sim, StateView S, int* out);
-__global__ void k_rebase(SimDev sim, StateView S, float* f_old, float* Dinv, float* Dinvinv);
+__global__ void k_rebase(SimDev sim, StateView S, float* f_old, float* Dinv);
 
 template <int GF> __global__ void k_g2p_adj(SimDev sim, StateView S, StateView G1, StateView Gout, int pending);
 template <int GF> __global__ void k_g2p_adj_big(SimDev sim, StateView S, StateView G1, StateView Gout, int pending);

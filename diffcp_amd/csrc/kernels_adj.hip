@@ -799,8 +799,7 @@ __device__ __forceinline__ void elem_adjoint(const SimDev& sim, const InstDev& I
     stress_adjoint<PT>(K.F, prm, SS, gP, sim.svd_grad, gF, gp);
     M3<float> gdn;
     if (!fresh) {
-        M3<float> Dii = load_m9(sim.Dinvinv, NE, e);
-        gF = madd(gF, mmul(gd, mT(Dii)));     // d = F Dinv^-1
+        gF = madd(gF, mmul(gd, mT(minv(K.Dinv))));     // d = F Dinv^-1
         if (RM) {
             // F = Q return_map(R) of the unmapped F (its QR recomputed here)
             RMState rs;

@@ -792,7 +792,7 @@ __global__ __launch_bounds__(DCP_WG) void k_loss_sum(const float* pmin, const in
 // reference overwrites d after it, so the stored column is (I + M)^-1 n.
 // f_old receives the F each element has under the old rest shape, which the
 // reference still uses for the stress of the next substep.
-__global__ __launch_bounds__(DCP_WG) void k_rebase(SimDev sim, StateView S, float* f_old, float* Dinv, float* Dinvinv) {
+__global__ __launch_bounds__(DCP_WG) void k_rebase(SimDev sim, StateView S, float* f_old, float* Dinv) {
     const int NE = sim.NE;
     for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < NE; e += gridDim.x * blockDim.x) {
         const InstDev I = sim.inst[sim.elem_inst[e]];
@@ -814,7 +814,6 @@ __global__ __launch_bounds__(DCP_WG) void k_rebase(SimDev sim, StateView S, floa
         d(0, 2) = n3.x; d(1, 2) = n3.y; d(2, 2) = n3.z;
         const M3<float> di = minv(d);
         store_m9(Dinv, NE, e, di);
-        store_m9(Dinvinv, NE, e, minv(di));
         store_v3(S.en(0), NE, e, mv(minv(madd(meye<float>(), K.M)), n3));
     }
 }

@@ -91,7 +91,6 @@ struct SimDev {
     int* big_n;                    // [4]: list lengths, finished-workgroup counters
     const int* tri;                // [3][NE] point indices
     const float* Dinv;             // [9][NE] inverse of the rest d
-    const float* Dinvinv;          // [9][NE] inverse(Dinv) (the d <- F @ Dinv^-1 round trip)
     const float* emod;             // [3][NE] per-element (E, shear, contact) or null
     const float* f_override;       // [9][NE] stress F of the first substep after a rest-shape re-base, or null
     const int* inc_off;            // [NP+1] vertex -> incident (element, corner)
