@@ -221,6 +221,9 @@ struct dcp_ctx {
     // hold its substep (read back after the segment's forward), so the adjoint
     // substeps restore their grids without launching P2G and the grid update
     bool lean_adjoint = false;
+    // ... and no patch of the segment's forward overflowed its tile, so none of
+    // its adjoint substeps does: k_g2p_adj_big is left out
+    bool adj_big_left_out = false;
     void drop_timing_events() {
         for (hipEvent_t e : tev) (void)hipEventDestroy(e);
         tev.clear();
@@ -446,9 +449,11 @@ template <int GF>
 static int launch_g2p_adj_form(dcp_ctx* c, StateView S, StateView G1, StateView Gout, int pending) {
     TIMED_LAUNCH(DCP_KERNEL_G2P_ADJ, k_g2p_adj<GF>, dim3(c->grid_spatches()), dim3(DCP_WG), 0, c->stream, c->sim, S,
                  G1, Gout, pending);
-    hipLaunchKernelGGL(k_g2p_adj_big<GF>, dim3(c->grid_big()), dim3(DCP_WG), 0, c->stream, c->sim, S, G1, Gout,
-                       pending);
-    LAUNCH_OK();
+    if (!c->adj_big_left_out) {
+        hipLaunchKernelGGL(k_g2p_adj_big<GF>, dim3(c->grid_big()), dim3(DCP_WG), 0, c->stream, c->sim, S, G1, Gout,
+                           pending);
+        LAUNCH_OK();
+    }
     return 0;
 }
 
@@ -463,7 +468,8 @@ static int launch_adjoint_head(dcp_ctx* c, StateView S, StateView G1, StateView 
         default: r = launch_g2p_adj_form<3>(c, S, G1, Gout, pending); break;
     }
     if (r) return -1;
-    TIMED_LAUNCH(DCP_KERNEL_GRIDOP_ADJ, k_gridop_adj, dim3(c->grid_gridop()), dim3(DCP_WG), 0, c->stream, c->sim);
+    TIMED_LAUNCH(DCP_KERNEL_GRIDOP_ADJ, k_gridop_adj, dim3(c->grid_gridop()), dim3(DCP_WG), 0, c->stream, c->sim,
+                 c->adj_big_left_out ? 1 : 0);
     return 0;
 }
 
@@ -1554,19 +1560,26 @@ static int segment_snap(const dcp_ctx* c, const Plan& p, int cseg, int f) {
 }
 
 // Whether every grid snapshot the adjoint walk of segment cseg reads holds its
-// substep, from the flags the segment's forward left (waits for it).
-static int segment_snapshots_hold(dcp_ctx* c, const Plan& p, int cseg, bool& hold) {
+// substep, from the flags the segment's forward left (waits for it), and
+// whether no scatter patch overflowed its tile since the last look (the count
+// k_p2g keeps in big_n[DCP_BIG_SEEN], cleared here).
+static int segment_snapshots_hold(dcp_ctx* c, const Plan& p, int cseg, bool& hold, bool& no_big) {
     hold = false;
+    no_big = false;
     const int base = cseg * p.K, end = std::min(base + p.K, p.L);
     if (end <= base || !c->snap_acc || !c->snap_prep) return 0;
     std::vector<int> ok((size_t)c->snap_slots, 0);
+    int seen = 1;
     HIP_OK(hipMemcpyAsync(ok.data(), c->snap_ok, ok.size() * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIP_OK(hipMemcpyAsync(&seen, c->sim.big_n + DCP_BIG_SEEN, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIP_OK(hipMemsetAsync(c->sim.big_n + DCP_BIG_SEEN, 0, sizeof(int), c->stream));
     HIP_OK(hipStreamSynchronize(c->stream));
     for (int f = base; f < end; ++f) {
         const int sn = segment_snap(c, p, cseg, f);
         if (sn >= 0 && !ok[sn]) return 0;
     }
     hold = true;
+    no_big = seen == 0;
     return 0;
 }
 
@@ -1608,15 +1621,19 @@ static int record_backward_segment(dcp_ctx* c, const Plan& p, int cseg, int g) {
 
 // Graph kinds: forward segment, adjoint walk (full: P2G and the grid update
 // launched on every substep, skipping themselves where the snapshot holds),
-// recompute of a segment's states, lean adjoint walk (no such launches).
-enum { REC_FORWARD = 0, REC_ADJOINT = 1, REC_RECOMPUTE = 2, REC_ADJOINT_LEAN = 3 };
+// recompute of a segment's states, lean adjoint walk (no such launches), and
+// the lean walk without the G2P adjoint's sub-box kernel (no tile overflowed
+// in the segment's forward).
+enum { REC_FORWARD = 0, REC_ADJOINT = 1, REC_RECOMPUTE = 2, REC_ADJOINT_LEAN = 3, REC_ADJOINT_LEAN_NOBIG = 4 };
 static int run_recorded(dcp_ctx* c, int kind, int cseg, int g, const Plan& p, int use_graph) {
     auto rec = [&]() {
         if (kind == REC_FORWARD) return record_forward_segment(c, p, cseg, g);
         if (kind == REC_RECOMPUTE) return record_recompute_segment(c, p, cseg, g);
-        c->lean_adjoint = kind == REC_ADJOINT_LEAN;
+        c->lean_adjoint = kind == REC_ADJOINT_LEAN || kind == REC_ADJOINT_LEAN_NOBIG;
+        c->adj_big_left_out = kind == REC_ADJOINT_LEAN_NOBIG;
         const int r = record_backward_segment(c, p, cseg, g);
         c->lean_adjoint = false;
+        c->adj_big_left_out = false;
         return r;
     };
     if (!use_graph) return rec();
@@ -1696,6 +1713,8 @@ extern "C" int dcp_rollout_grad(dcp_ctx* c, int32_t S, int32_t g, int32_t K, int
     // the reset state must already be in slot 0 (dcp_reset); pooled grids
     // start the rollout with an empty block table
     if (c->reset_pool()) return set_err("pool reset launch failed");
+    // the overflow count the adjoint walks consult starts with this rollout
+    HIP_OK(hipMemsetAsync(c->sim.big_n + DCP_BIG_SEEN, 0, sizeof(int), c->stream));
     // timing events restart with an eager rollout or with a plan's first capture
     if (!use_graph || !c->graphs.count(std::make_tuple(0, 0, g, p.S, p.K))) c->tev_used = 0;
     HIP_OK(hipEventRecord(c->ev0, c->stream));
@@ -1709,9 +1728,10 @@ extern "C" int dcp_rollout_grad(dcp_ctx* c, int32_t S, int32_t g, int32_t K, int
     int last = (p.L - 1) / p.K;
     for (int cs = last; cs >= 0; --cs) {
         if (cs != p.nseg - 1 && run_recorded(c, REC_RECOMPUTE, cs, g, p, use_graph)) return -1;
-        bool hold = false;
-        if (segment_snapshots_hold(c, p, cs, hold)) return -1;
-        if (run_recorded(c, hold ? REC_ADJOINT_LEAN : REC_ADJOINT, cs, g, p, use_graph)) return -1;
+        bool hold = false, no_big = false;
+        if (segment_snapshots_hold(c, p, cs, hold, no_big)) return -1;
+        const int kind = !hold ? REC_ADJOINT : no_big ? REC_ADJOINT_LEAN_NOBIG : REC_ADJOINT_LEAN;
+        if (run_recorded(c, kind, cs, g, p, use_graph)) return -1;
     }
     HIP_OK(hipEventRecord(c->ev1, c->stream));
     HIP_OK(hipEventSynchronize(c->ev1));

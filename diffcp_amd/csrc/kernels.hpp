@@ -411,7 +411,7 @@ __global__ void k_rebase(SimDev sim, StateView S, float* f_old, float* Dinv);
 
 template <int GF> __global__ void k_g2p_adj(SimDev sim, StateView S, StateView G1, StateView Gout, int pending);
 template <int GF> __global__ void k_g2p_adj_big(SimDev sim, StateView S, StateView G1, StateView Gout, int pending);
-__global__ void k_gridop_adj(SimDev sim);
+__global__ void k_gridop_adj(SimDev sim, int big_left_out);
 template <bool POOL> __global__ void k_p2g_adj(SimDev sim, StateView S, StateView Gout, int s, int g);
 template <typename PT, bool RM, bool POOL>
 __global__ void k_elem_adj(SimDev sim, StateView S, StateView G1, StateView Gout, int fresh, int s, int g);

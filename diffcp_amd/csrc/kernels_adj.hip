@@ -427,9 +427,13 @@ __global__ __launch_bounds__(DCP_WG, 2) void k_g2p_adj_big(SimDev sim, StateView
 // ---------------------------------------------------------------------------
 // grid update adjoint
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(DCP_WG) void k_gridop_adj(SimDev sim) {
-    // k_g2p_adj_big, just before on the stream, has read the overflow list
-    if (blockIdx.x == 0 && threadIdx.x == 0) sim.big_n[DCP_BIG_G2P_ADJ] = 0;
+__global__ __launch_bounds__(DCP_WG) void k_gridop_adj(SimDev sim, int big_left_out) {
+    // k_g2p_adj_big, just before on the stream, has read the overflow list;
+    // when it was left out (the forward listed no patch) the list is empty
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        if (big_left_out && sim.big_n[DCP_BIG_G2P_ADJ] != 0) atomicOr(sim.status, (unsigned)DCP_STATUS_INTERNAL);
+        sim.big_n[DCP_BIG_G2P_ADJ] = 0;
+    }
     const int lane = threadIdx.x & 63;
     ActiveWalk w;
     if (!active_walk(sim, sim.inst, w)) return;

@@ -310,7 +310,10 @@ __global__ __launch_bounds__(DCP_WG, (GF & DCP_GF_DET) ? 3 : 2) void k_p2g(SimDe
                 p2g_mark_blocks(sim, I, P, TB, tb_touched);
             }
         }
-        if (!tiled && tid == 0) sim.big_list[atomicAdd(&sim.big_n[DCP_BIG_P2G], 1)] = pi;   // left to k_p2g_big
+        if (!tiled && tid == 0) {   // left to k_p2g_big
+            sim.big_list[atomicAdd(&sim.big_n[DCP_BIG_P2G], 1)] = pi;
+            atomicAdd(&sim.big_n[DCP_BIG_SEEN], 1);
+        }
         // next patch: empty counts and box
         bin_clear(L);
         if (tid < 3) { L.box[tid] = 1 << 30; L.box[3 + tid] = -(1 << 30); }

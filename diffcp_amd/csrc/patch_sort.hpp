@@ -329,6 +329,11 @@ __device__ __forceinline__ bool in_sub_box(const PatchBox& S, int b0, int b1, in
 // list l at big_list[l * n_spatches ...].
 #define DCP_BIG_P2G 0
 #define DCP_BIG_G2P_ADJ 1
+// big_n[DCP_BIG_SEEN]: patches k_p2g listed since the host last cleared it
+// (after each forward or recompute segment): zero means no adjoint substep
+// of that segment lists one either (same states, same patches, same test),
+// so its adjoint graph can leave k_g2p_adj_big out
+#define DCP_BIG_SEEN 2
 
 // Box of the patch's base cells from one candidate per lane.
 __device__ __forceinline__ void box_reduce(int b0, int b1, int b2, bool act, int* box, PatchBox& B) {

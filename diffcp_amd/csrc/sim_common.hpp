@@ -88,7 +88,7 @@ struct SimDev {
     const Patch* gpatches;         // gather patches (G2P, P2G adjoint): full chunks, point patches first
     int n_gpatches, n_gpoint;
     int* big_list;                 // [2][n_spatches]: scatter patches whose tile exceeds the cap
-    int* big_n;                    // [4]: list lengths, finished-workgroup counters
+    int* big_n;                    // [4]: list lengths, patches listed since the host last looked
     const int* tri;                // [3][NE] point indices
     const float* Dinv;             // [9][NE] inverse of the rest d
     const float* emod;             // [3][NE] per-element (E, shear, contact) or null

@@ -46,6 +46,7 @@ typedef struct dcp_ctx dcp_ctx;
 #define DCP_STATUS_OUT_OF_GRID 2u   /* a particle stencil left [0, n_grid) (clamped) */
 #define DCP_STATUS_TILE_SPILL  4u   /* a patch of more than 256 particles used global atomics */
 #define DCP_STATUS_POOL_FULL   8u   /* the grid block pool ran out of slots (results invalid) */
+#define DCP_STATUS_INTERNAL   16u   /* an internal consistency check failed (results invalid) */
 
 /* One cloth scene.  Replaces the module globals of
  * src/initialization.py:21-193 and the fields of

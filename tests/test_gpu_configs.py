@@ -44,7 +44,7 @@ from oracle import sim_torch as O
 pytestmark = pytest.mark.gpu
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 GOLDEN = os.path.join(ROOT, "tests", "golden")
-BAD_STATUS = 1 | 2
+BAD_STATUS = 1 | 2 | 16          # non-finite, out of grid, internal check
 
 
 def _engine(insts, **kw):

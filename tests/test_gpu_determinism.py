@@ -100,7 +100,7 @@ def test_overflowing_tiles_same_bits(pool):
     eng.set_targets([inst.targets])
     a = _window(eng, S)
     _same(a, _window(eng, S), "overflow tiles")
-    assert eng.status() & 7 == 0
+    assert eng.status() & 23 == 0
     eng.close()
 
 

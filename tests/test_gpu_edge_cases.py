@@ -20,7 +20,7 @@ from diffcp_amd import scene
 from oracle import sim_torch as O
 
 pytestmark = pytest.mark.gpu
-BAD_STATUS = 1 | 2 | 4
+BAD_STATUS = 1 | 2 | 4 | 16
 PARAMS = ("E", "nu", "ks", "kc", "damping")
 
 

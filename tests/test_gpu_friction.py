@@ -18,7 +18,7 @@ from oracle import reduced as R
 from oracle import sim_torch as O
 
 pytestmark = pytest.mark.gpu
-BAD_STATUS = 1 | 2
+BAD_STATUS = 1 | 2 | 16
 PARAMS = ("E", "nu", "ks", "kc", "damping", "friction")
 
 

@@ -24,7 +24,7 @@ pytestmark = pytest.mark.gpu
 
 # status bits that mean a wrong result (NaN/Inf, stencil clamped at the grid
 # edge); bit 4 only records that a patch took the slower global-atomic path
-BAD_STATUS = 1 | 2
+BAD_STATUS = 1 | 2 | 16
 
 
 def _engine(insts, **kw):

@@ -101,7 +101,7 @@ def test_grid_snapshots_agree_with_recompute():
         l, g, _ = eng.rollout_grad(S, 0, K, use_graph=True)
         l2, g2, _ = eng.rollout_grad(S, 0, K, use_graph=True)
         assert _rel(l2, l) < 1e-5 and _rel(g2[:, :5], g[:, :5]) < 1e-5, snaps
-        assert eng.status() & 3 == 0
+        assert eng.status() & 19 == 0
         res.append((l, g))
         eng.close()
     for l, g in res[1:]:
@@ -198,11 +198,11 @@ def test_overflowing_patch_tiles_through_sub_boxes(n_grid, spread_xz, y_lo, y_hi
     eng.status(clear=True)
     l1, g1, _ = eng.rollout_grad(S, 0, S, use_graph=False)
     t1 = eng.trajectory_grads()[0]
-    assert eng.status(clear=True) & 7 == 0
+    assert eng.status(clear=True) & 23 == 0
     eng.reset(0)
     l2, g2, _ = eng.rollout_grad(S, 0, S, use_graph=True)
     t2 = eng.trajectory_grads()[0]
-    assert eng.status() & 7 == 0
+    assert eng.status() & 23 == 0
 
     st, static = O.initialize_state(inst)
     static["traj"] = static["traj"].clone().requires_grad_(True)
