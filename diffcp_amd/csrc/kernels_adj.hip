@@ -1,13 +1,14 @@
 // Adjoint substep kernels: the hand-written reverse of k_prep, k_p2g, k_gridop
 // and k_g2p (what Taichi's tape generates for simulation.py:500-506, with the
-// manual SVD backward of :71-94).  The adjoint of substep s runs after a
-// recompute of its grid (k_prep, k_p2g, k_gridop with keep=1):
+// manual SVD backward of :71-94).  The adjoint of substep s runs on its grid:
+// restored from the forward's snapshot (k_grid_restore), or recomputed
+// (k_prep, k_p2g, k_gridop with keep=1):
 //   k_g2p_adj       per patch: dL/d grid_v_out scattered through an LDS tile
 //                   (the same tiling as P2G), dL/dx through the weights
 //   k_gridop_adj    per active block: wall masks, damping, normalisation
 //   k_p2g_adj       per vertex / grasper / cube particle: 27-cell gather of
 //                   dL/d(m, mv) straight from the grid
-//   k_elem_adj      per element patch: the same gather for the element
+//   k_elem_adj      per element gather patch: the same gather for the element
 //                   particles feeding, in registers, the element prep adjoint
 //                   (stress, QR, polar (FP64), kinematics); parameter
 //                   gradients reduced per patch in FP64, fixed order

@@ -1,7 +1,9 @@
 // Forward substep kernels (simulation.py:9-430) and the point-cloud loss
 // (parameter_fitting_functions.py:3-15), batched over instances.
 //
-// One forward substep = 4 launches:
+// One forward substep = 4 main launches (and the list kernels between them:
+// k_p2g_big for patches whose tile overflows, k_active_list for the grid
+// update's block lists; k_pool_alloc before k_p2g on a pooled grid):
 //   k_prep     per element: x_e, v_e (vertex means), d and F rebuilt from the
 //              vertex positions and the stored normal column, QR + FP64 polar,
 //              stress P, the element's normal impulse vector and the three
