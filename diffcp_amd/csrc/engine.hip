@@ -749,6 +749,11 @@ static int build(dcp_ctx* c, const dcp_instance_desc* d, int B) {
     for (const Patch& P : c->patches_h)
         if (P.cls == CLS_GRASPER || P.cls == CLS_CUBE)
             c->spatches_h.push_back(SPatch{P.inst, P.cls, P.start, P.count, CLS_ELEMENT, 0, 0});
+    // largest first: the persistent scatter grid works them off in index order,
+    // so the last round holds the small patches (grasper, cube, sparse bins)
+    std::stable_sort(c->spatches_h.begin(), c->spatches_h.end(), [](const SPatch& x, const SPatch& y) {
+        return x.count_a + x.count_b > y.count_a + y.count_b;
+    });
     c->n_spatches = (int)c->spatches_h.size();
     // gather patches (G2P, P2G adjoint: per-particle stencil reads, no tile):
     // each class of an instance cut into full patches of ps particles in
