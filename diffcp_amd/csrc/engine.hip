@@ -773,6 +773,10 @@ static int build(dcp_ctx* c, const dcp_instance_desc* d, int B) {
         for (int o = 0; o < c->inst_h[b].e_count; o += ps)
             c->gpatches_h.push_back(Patch{b, CLS_ELEMENT, c->inst_h[b].e_start + o, std::min(ps, c->inst_h[b].e_count - o)});
     c->n_gpatches = (int)c->gpatches_h.size();
+    // the partial chunks last in each of the two ranges (as the scatter patches)
+    auto fuller = [](const Patch& x, const Patch& y) { return x.count > y.count; };
+    std::stable_sort(c->gpatches_h.begin(), c->gpatches_h.begin() + c->n_gpoint, fuller);
+    std::stable_sort(c->gpatches_h.begin() + c->n_gpoint, c->gpatches_h.end(), fuller);
     // 4^3 blocks the initial stencils touch (sizes the grid snapshots)
     c->init_blocks = 0;
     for (int b = 0; b < B; ++b) {
