@@ -224,6 +224,10 @@ struct dcp_ctx {
     // ... and no patch of the segment's forward overflowed its tile, so none of
     // its adjoint substeps does: k_g2p_adj_big is left out
     bool adj_big_left_out = false;
+    // forward segments of dcp_rollout_grad leave k_p2g_big out until a patch
+    // has overflowed its tile in this context (the window is then run again
+    // with it, and it stays in)
+    bool fwd_big_seen = false, fwd_big_left_out = false;
     void drop_timing_events() {
         for (hipEvent_t e : tev) (void)hipEventDestroy(e);
         tev.clear();
@@ -358,8 +362,10 @@ static int launch_p2g_form(dcp_ctx* c, StateView S, int s, int g, const int* ski
         TIMED_LAUNCH(DCP_KERNEL_P2G, k_p2g<GF>, dim3(c->grid_spatches()), dim3(DCP_WG), 0, c->stream, c->sim, S, s, g,
                      skip);
     }
-    hipLaunchKernelGGL(k_p2g_big<GF>, dim3(c->grid_big()), dim3(DCP_WG), 0, c->stream, c->sim, S, s, g, skip);
-    LAUNCH_OK();
+    if (!c->fwd_big_left_out) {
+        hipLaunchKernelGGL(k_p2g_big<GF>, dim3(c->grid_big()), dim3(DCP_WG), 0, c->stream, c->sim, S, s, g, skip);
+        LAUNCH_OK();
+    }
     hipLaunchKernelGGL(k_active_list, dim3(c->grid_gridop()), dim3(DCP_WG), 0, c->stream, c->sim, skip);
     LAUNCH_OK();
     return 0;
@@ -1633,10 +1639,16 @@ static int record_backward_segment(dcp_ctx* c, const Plan& p, int cseg, int g) {
 // recompute of a segment's states, lean adjoint walk (no such launches), and
 // the lean walk without the G2P adjoint's sub-box kernel (no tile overflowed
 // in the segment's forward).
-enum { REC_FORWARD = 0, REC_ADJOINT = 1, REC_RECOMPUTE = 2, REC_ADJOINT_LEAN = 3, REC_ADJOINT_LEAN_NOBIG = 4 };
+enum { REC_FORWARD = 0, REC_ADJOINT = 1, REC_RECOMPUTE = 2, REC_ADJOINT_LEAN = 3, REC_ADJOINT_LEAN_NOBIG = 4,
+       REC_FORWARD_NOBIG = 5 };
 static int run_recorded(dcp_ctx* c, int kind, int cseg, int g, const Plan& p, int use_graph) {
     auto rec = [&]() {
-        if (kind == REC_FORWARD) return record_forward_segment(c, p, cseg, g);
+        if (kind == REC_FORWARD || kind == REC_FORWARD_NOBIG) {
+            c->fwd_big_left_out = kind == REC_FORWARD_NOBIG;
+            const int r = record_forward_segment(c, p, cseg, g);
+            c->fwd_big_left_out = false;
+            return r;
+        }
         if (kind == REC_RECOMPUTE) return record_recompute_segment(c, p, cseg, g);
         c->lean_adjoint = kind == REC_ADJOINT_LEAN || kind == REC_ADJOINT_LEAN_NOBIG;
         c->adj_big_left_out = kind == REC_ADJOINT_LEAN_NOBIG;
@@ -1719,19 +1731,31 @@ extern "C" int dcp_rollout_grad(dcp_ctx* c, int32_t S, int32_t g, int32_t K, int
     if (plan_rollout(c, S, K, p)) return -1;
     if (ensure_snapshots(c, p.K)) return -1;
     if (c->n_targets_total == 0) return set_err("no targets set (dcp_set_targets)");
-    // the reset state must already be in slot 0 (dcp_reset); pooled grids
-    // start the rollout with an empty block table
-    if (c->reset_pool()) return set_err("pool reset launch failed");
-    // the overflow count the adjoint walks consult starts with this rollout
-    HIP_OK(hipMemsetAsync(c->sim.big_n + DCP_BIG_SEEN, 0, sizeof(int), c->stream));
-    // timing events restart with an eager rollout or with a plan's first capture
-    if (!use_graph || !c->graphs.count(std::make_tuple(0, 0, g, p.S, p.K))) c->tev_used = 0;
-    HIP_OK(hipEventRecord(c->ev0, c->stream));
-    // unit loss seeds (dcp_loss_adjoint may have left others)
-    HIP_OK(hipMemcpyAsync(c->seed_d, c->ones_h.data(), c->B * sizeof(double), hipMemcpyHostToDevice, c->stream));
-    if (dcp_zero_param_grads(c)) return -1;
-    for (int cs = 0; cs < p.nseg; ++cs)
-        if (run_recorded(c, REC_FORWARD, cs, g, p, use_graph)) return -1;
+    for (;;) {
+        const int fkind = c->fwd_big_seen ? REC_FORWARD : REC_FORWARD_NOBIG;
+        // the reset state must already be in slot 0 (dcp_reset); pooled grids
+        // start the rollout with an empty block table
+        if (c->reset_pool()) return set_err("pool reset launch failed");
+        // the overflow count the adjoint walks consult starts with this rollout
+        HIP_OK(hipMemsetAsync(c->sim.big_n + DCP_BIG_SEEN, 0, sizeof(int), c->stream));
+        // timing events restart with an eager rollout or with a plan's first capture
+        if (!use_graph || !c->graphs.count(std::make_tuple(fkind, 0, g, p.S, p.K))) c->tev_used = 0;
+        HIP_OK(hipEventRecord(c->ev0, c->stream));
+        // unit loss seeds (dcp_loss_adjoint may have left others)
+        HIP_OK(hipMemcpyAsync(c->seed_d, c->ones_h.data(), c->B * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        if (dcp_zero_param_grads(c)) return -1;
+        for (int cs = 0; cs < p.nseg; ++cs)
+            if (run_recorded(c, fkind, cs, g, p, use_graph)) return -1;
+        if (fkind == REC_FORWARD) break;
+        // a forward without the sub-box kernel is valid only if no patch was
+        // listed for it; otherwise run the window again with it (slot 0 still
+        // holds the reset state), and keep it from now on
+        int seen = 0;
+        HIP_OK(hipMemcpyAsync(&seen, c->sim.big_n + DCP_BIG_SEEN, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        HIP_OK(hipStreamSynchronize(c->stream));
+        if (seen == 0) break;
+        c->fwd_big_seen = true;
+    }
     HIP_OK(hipMemcpyAsync(loss_out ? loss_out : nullptr, c->loss_d, loss_out ? c->B * sizeof(double) : 0,
                           hipMemcpyDefault, c->stream));
     int last = (p.L - 1) / p.K;
